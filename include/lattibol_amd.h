@@ -1,0 +1,269 @@
+/*
+ * lattibol_amd.h -- C ABI of the MI355X-native lattice-Boltzmann core.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Drop-in replacements for the reference's extern "C" launchers, same
+ *    names, same argument order and meaning, reference semantics (in-place
+ *    swap streaming, full-way bounce-back, Zou-He inlet/outlet, D2Q9
+ *    Shan-Chen).  The Data structs keep the reference's field order
+ *    (lattibol.h:29-97); HIP's dim3 and hipStream_t are replaced by the
+ *    layout-identical lb_dim3 and an opaque lb_stream_t so this header needs
+ *    no GPU headers.  grid*block must equal the subdomain shape, as in the
+ *    reference (simulation.cu:43-44); the library validates that and then
+ *    launches its own shapes.  These functions return void like the
+ *    reference; lb_last_error() reports the first failure since the last call.
+ *
+ * 2. New int-returning entry points (0 = success, >0 a hipError_t, <0 an
+ *    LB_ERR_* code): the reference step loop as a host driver (lb_refsim_*),
+ *    the fused single-sweep pull kernels with FP32/FP16/BF16/FP64 storage,
+ *    halo planes, FP64 reductions, hipGraph-captured stepping (lb_*), the
+ *    batched multi-lattice solver (lb_batch_*) and the U-Net surrogate
+ *    (lb_unet_*).  Device memory is owned by opaque handles; every handle is
+ *    single-threaded and stream-ordered on its own HIP stream.
+ */
+#ifndef LATTIBOL_AMD_H
+#define LATTIBOL_AMD_H
+
+#include <stdint.h>
+#include <stdbool.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Types shared with the reference                                         */
+/* ---------------------------------------------------------------------- */
+typedef struct { unsigned int x, y, z; } lb_dim3;   /* layout of HIP's dim3 */
+typedef void *lb_stream_t;                            /* a hipStream_t       */
+
+/* lattibol.h:29-63 */
+typedef struct {
+    float *ux, *uy, *rho;
+    bool *solid, *solid_bound;
+    float *f0, *f1, *f2, *f3, *f4;
+    float *f5, *f6, *f7, *f8;
+    float *border_f1, *border_f5, *border_f8;
+    float *border_f3, *border_f6, *border_f7;
+    float *nb_border_f1, *nb_border_f5, *nb_border_f8;
+    float *nb_border_f3, *nb_border_f6, *nb_border_f7;
+    int width, height;
+    int deviceCount, dev;
+    float *plotBuffer;
+    float *valueCur;
+    lb_stream_t stream;
+    float *psi;
+    float *ueq_x, *ueq_y;
+} DataD2Q9;
+
+/* lattibol.h:65-97 */
+typedef struct {
+    float *ux, *uy, *uz, *rho;
+    bool *solid, *solid_bound;
+    float *f0, *f1, *f2, *f3, *f4;
+    float *f5, *f6, *f7, *f8, *f9;
+    float *f10, *f11, *f12, *f13, *f14;
+    float *f15, *f16, *f17, *f18;
+    float *border_f1, *border_f7, *border_f8, *border_f9, *border_f10;
+    float *border_f2, *border_f11, *border_f12, *border_f13, *border_f14;
+    float *nb_border_f1, *nb_border_f7, *nb_border_f8, *nb_border_f9, *nb_border_f10;
+    float *nb_border_f2, *nb_border_f11, *nb_border_f12, *nb_border_f13, *nb_border_f14;
+    int width, height, depth;
+    int deviceCount, dev;
+    float *plotBuffer;
+    float *valueCur;
+    lb_stream_t stream;
+} DataD3Q19;
+
+/* ---------------------------------------------------------------------- */
+/* Error codes                                                             */
+/* ---------------------------------------------------------------------- */
+enum {
+    LB_OK = 0,
+    LB_ERR_INVALID = -1,      /* bad argument / null pointer               */
+    LB_ERR_SHAPE = -2,        /* grid*block does not match the lattice     */
+    LB_ERR_UNSUPPORTED = -3,  /* configuration not implemented             */
+    LB_ERR_NO_DEVICE = -4,    /* no usable GPU                             */
+    LB_ERR_NOT_READY = -5     /* state missing (e.g. macros never written)  */
+};
+int lb_last_error(void);                 /* returns and clears the sticky error */
+const char *lb_error_string(int code);
+int lb_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* 1. Reference entry points (reference semantics, legacy default stream)  */
+/* ---------------------------------------------------------------------- */
+/* streaming.h:31-35 */
+void streaming_d2q9(DataD2Q9 *data, lb_dim3 grid, lb_dim3 block);
+void streaming_d3q19(DataD3Q19 *data, lb_dim3 grid, lb_dim3 block);
+/* collision.h:27-36 (u_min..rho_max are accepted and unused, as in the
+ * reference; the D2Q9 Shan-Chen second pass runs only for G < 0) */
+void collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float w1, float w5,
+                    float tau_inv, float ux_in, float uy_in, float rho_out, float u_min,
+                    float u_max, float rho_min, float rho_max, lb_dim3 grid, lb_dim3 block,
+                    float Fx, float Fy, float G, float Gads);
+void collision_d3q19(float *plot_data, DataD3Q19 *data, float w0, float w1, float w7,
+                     float tau_inv, float ux_in, float uy_in, float uz_in, float rho_out,
+                     float u_min, float u_max, float rho_min, float rho_max, lb_dim3 grid,
+                     lb_dim3 block);
+/* get_border.h:29-33 */
+void get_border_d2q9(DataD2Q9 *data, lb_dim3 grid, lb_dim3 block);
+void get_border_d3q19(DataD3Q19 *data, lb_dim3 grid, lb_dim3 block);
+/* copy_border.h:30-47.  The reference passes cudaStream_t& (a pointer at the
+ * ABI level); the D3Q19 variant had C++ linkage there and is extern "C" here. */
+void copy_border_devices_d2q9(float *nb_border_f1, float *nb_border_f5, float *nb_border_f8,
+                              float *nb_border_f3, float *nb_border_f6, float *nb_border_f7,
+                              float *border_f1, float *border_f5, float *border_f8,
+                              float *border_f3, float *border_f6, float *border_f7,
+                              int height, lb_stream_t *s);
+void copy_border_devices_d3q19(float *nb_border_f1, float *nb_border_f7, float *nb_border_f8,
+                               float *nb_border_f9, float *nb_border_f10, float *nb_border_f2,
+                               float *nb_border_f11, float *nb_border_f12, float *nb_border_f13,
+                               float *nb_border_f14, float *border_f1, float *border_f7,
+                               float *border_f8, float *border_f9, float *border_f10,
+                               float *border_f2, float *border_f11, float *border_f12,
+                               float *border_f13, float *border_f14, int height, int depth,
+                               lb_stream_t *s);
+/* apply_border.h:25-29 */
+void apply_border_d2q9(DataD2Q9 *data, lb_dim3 grid, lb_dim3 block);
+void apply_border_d3q19(DataD3Q19 *data, lb_dim3 grid, lb_dim3 block);
+/* read_value.h:27-31 (synchronises the device, like the reference) */
+void read_value_2d(float *plot_data, DataD2Q9 *data, int x, int y, int z);
+void read_value_3d(float *plot_data, DataD3Q19 *data, int x, int y, int z);
+
+/* ---------------------------------------------------------------------- */
+/* 1b. The reference host driver: Simulation{D2Q9,D3Q19} (simulation_d2q9.cu
+ *     :52-560, simulation_d3q19.cu:50-436) as an opaque handle.           */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int deviceCount;              /* number of x-slab subdomains             */
+    int width, height, depth;     /* full lattice; depth = 1 selects D2Q9    */
+    float ux, uy, uz, rho;        /* inlet velocity, outlet / initial density */
+    float viscosity;              /* tau_inv = 1/(3 nu + 0.5)                */
+    float Fx, Fy, Fz;             /* body force (D2Q9)                       */
+    float G, Gads;                /* Shan-Chen coupling / adhesion (D2Q9)    */
+    int zou_he;                   /* 1 = reference inlet/outlet (always on there) */
+    int cursor_x, cursor_y, cursor_z;
+} lb_ref_params;
+typedef struct lb_refsim lb_refsim;
+int lb_refsim_create(const lb_ref_params *p, const uint8_t *solids, lb_refsim **out);
+int lb_refsim_destroy(lb_refsim *s);
+/* runLBM(): updateSteps reference steps (stream, halo, collide, probe). */
+int lb_refsim_run(lb_refsim *s, int updateSteps);
+/* valueCur after the last step: (ux, uy, uz, rho) at the cursor. */
+int lb_refsim_value_cur(lb_refsim *s, float out[4]);
+/* Copies subdomain dev to host: f (Q*N), ux, uy, uz (NULL for D2Q9 ok), rho,
+ * plot (8N floats for D2Q9, 4N for D3Q19).  Any pointer may be NULL. */
+int lb_refsim_download(lb_refsim *s, int dev, float *f, float *ux, float *uy, float *uz,
+                       float *rho, float *plot);
+/* Access to the reference-layout structs (device pointers inside). */
+int lb_refsim_data_d2q9(lb_refsim *s, int dev, DataD2Q9 *out);
+int lb_refsim_data_d3q19(lb_refsim *s, int dev, DataD3Q19 *out);
+
+/* ---------------------------------------------------------------------- */
+/* 2. Fused pull-scheme solver                                              */
+/* ---------------------------------------------------------------------- */
+enum { LB_BC_PERIODIC = 0, LB_BC_WALL = 1, LB_BC_MOVING_WALL = 2, LB_BC_INLET_EQ = 3,
+       LB_BC_OUTFLOW = 4 };
+enum { LB_STORE_F32 = 0, LB_STORE_F16 = 1, LB_STORE_BF16 = 2, LB_STORE_F64 = 3 };
+
+/* Faces are 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z in storage index space (D2Q9 rows
+ * grow downward as in the reference, physical y = -row).  Populations use
+ * the reference numbering (streaming.cu, collision.cu). */
+typedef struct {
+    int lattice;            /* 9 or 19 */
+    int nx, ny, nz;         /* nz = 1 for D2Q9 */
+    int storage;            /* LB_STORE_*; FP16/BF16 store f_i - w_i */
+    float tau_inv;
+    int bc[6];
+    float bc_u[6][3];       /* physical wall / inlet velocity */
+    float bc_rho[6];        /* inlet density */
+    float force[3];         /* body force, velocity-shift forcing (collision.cu:281-284) */
+    float sc_G;             /* Shan-Chen coupling (D3Q19), 0 = off */
+    float sc_rho0;          /* psi = rho0 (1 - exp(-rho/rho0)) */
+    float sc_psi_solid;
+    int bb_full;            /* 1: full-way bounce-back at solid nodes (reference) */
+    int zou_he;             /* 1: reference Zou-He inlet x=0 / outlet x=nx-1 */
+    float zh_ux_in, zh_rho_out;
+    int halo_z;             /* D3Q19 periodic z through packed halo planes:
+                               pack (fused in the sweep) -> copy -> unpack (fused) */
+    int block_x, block_y;   /* launch shape; 0 = autotune */
+    int xcd_remap;          /* -1 autotune, 0 off, 1 on */
+    int graph_chunk;        /* steps per captured hipGraph; 0 = default */
+} lb_config;
+
+typedef struct lb_ctx lb_ctx;
+int lb_create(const lb_config *cfg, lb_ctx **out);
+int lb_destroy(lb_ctx *ctx);
+int lb_get_config(lb_ctx *ctx, lb_config *out);     /* with autotuned fields */
+/* Solid mask (host, N bytes, 1 = solid).  Also rebuilds the node flags. */
+int lb_set_solid(lb_ctx *ctx, const uint8_t *solid);
+/* Equilibrium from host fields (N floats each; uz may be NULL), computed on
+ * the GPU; writes both population buffers and clears the step counter. */
+int lb_init_equilibrium(lb_ctx *ctx, const float *rho, const float *ux, const float *uy,
+                        const float *uz);
+/* Same from device fields. */
+int lb_init_equilibrium_device(lb_ctx *ctx, const float *d_rho, const float *d_ux,
+                               const float *d_uy, const float *d_uz);
+int lb_init_uniform(lb_ctx *ctx, float rho, float ux, float uy, float uz);
+/* Raw populations in the storage format (Q*N elements), host side. */
+int lb_set_populations(lb_ctx *ctx, const void *host);
+int lb_get_populations(lb_ctx *ctx, void *host);
+/* nsteps time steps; the last one also writes rho,u (the macros). */
+int lb_step(lb_ctx *ctx, int nsteps);
+int lb_sync(lb_ctx *ctx);
+/* Macros of the last step to host (N floats each, any may be NULL). */
+int lb_get_macros(lb_ctx *ctx, float *rho, float *ux, float *uy, float *uz);
+/* Device pointer to the macro planes (rho, ux, uy, uz; 4*N floats). */
+int lb_macros_device(lb_ctx *ctx, float **out);
+/* FP64 totals over fluid nodes of the current populations:
+ * out = {mass, momentum_x, momentum_y, momentum_z} (physical axes). */
+int lb_reduce(lb_ctx *ctx, double out[4]);
+/* ||u - u_snap|| / ||u|| over fluid nodes (FP64), then u_snap := u.  The
+ * first call after init returns 1.0 (no snapshot yet). */
+int lb_residual(lb_ctx *ctx, double *out);
+/* (ux, uy, uz, rho) at one node from the last macros. */
+int lb_read_value(lb_ctx *ctx, int x, int y, int z, float out[4]);
+int lb_steps_done(lb_ctx *ctx, long long *out);
+/* Timing helper: average duration (ms) of one step kernel, measured with HIP
+ * events over n back-to-back launches on the handle's stream (no macros). */
+int lb_time_step_kernel(lb_ctx *ctx, int n, float *ms_per_launch);
+/* Halo buffers of the z-slab (device pointers): send/recv planes of the five
+ * +z and five -z populations (5*nx*ny each).  For a multi-GPU ring the
+ * caller moves send_hi -> neighbour's recv_lo and send_lo -> recv_hi. */
+int lb_halo_buffers(lb_ctx *ctx, void **send_lo, void **send_hi, void **recv_lo,
+                    void **recv_hi);
+
+/* ---------------------------------------------------------------------- */
+/* 3. Batched ragged D2Q9 channels                                          */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int nx, ny;
+    float tau_inv;
+    float u_in;             /* equilibrium inlet velocity at x = 0 (rho 1) */
+    int64_t solid_offset;   /* offset of this lattice's mask in the packed mask array */
+} lb_batch_lattice;
+typedef struct lb_batch lb_batch;
+/* Walls (half-way bounce-back) at rows 0 and ny-1, equilibrium inlet at x=0,
+ * zero-gradient outflow at x=nx-1, solid masks packed back to back. */
+int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t *solids, lb_batch **out);
+int lb_batch_destroy(lb_batch *b);
+/* Start from rest (rho 1, u 0) or from device fields packed like the masks
+ * (rho, ux, uy planes per lattice: 3*N_b floats per lattice, back to back). */
+int lb_batch_init_rest(lb_batch *b);
+int lb_batch_init_fields_device(lb_batch *b, const float *d_fields);
+int lb_batch_step(lb_batch *b, int nsteps);
+/* Run to residual < tol (checked every k steps) or max_steps; per lattice
+ * steps taken and final residual.  Converged lattices retire. */
+int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *steps_out,
+                           double *res_out);
+/* Macros (rho, ux, uy) of lattice i to host. */
+int lb_batch_get_macros(lb_batch *b, int i, float *rho, float *ux, float *uy);
+int lb_batch_get_populations(lb_batch *b, int i, float *host);
+int64_t lb_batch_total_nodes(lb_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LATTIBOL_AMD_H */

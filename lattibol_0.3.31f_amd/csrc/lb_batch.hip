@@ -1,0 +1,385 @@
+// lb_batch.hip -- many independent D2Q9 channels of uneven size stepped
+// together: one launch per time step covers every tile of every active
+// lattice (a block table maps workgroup -> (lattice, tile)), the step loop
+// between convergence checks is captured in a hipGraph, and lattices whose
+// residual falls below the tolerance retire from the block table.
+//
+// Each lattice runs the exact per-node code of the single-lattice solver
+// (step_body in lb_pull.hpp) with the configuration
+//   face -x: equilibrium inlet (rho 1, u = (u_in, 0)),  face +x: zero-gradient
+//   outflow, rows 0 and ny-1: half-way bounce-back walls, solids: half-way
+//   bounce-back,
+// so a lattice of the batch is bit-identical to lb_create() with that
+// configuration.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "lb_pull.hpp"
+
+using namespace lb;
+
+namespace {
+
+constexpr int kBX = 64, kBY = 4;   // 256-thread tiles of 64 x 4 nodes
+
+struct BatchLat {
+    int nx, ny;
+    uint32_t N;
+    float ti, u_in;
+    uint64_t pop_off;     // element offset of population 0 in A / B
+    uint64_t node_off;    // node offset in flags / fields
+    int ntx, nty;
+};
+
+LB_DEV StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, const uint8_t *flags, float *macros) {
+    StepArgs<StF32> a{};
+    a.A = A + L.pop_off;
+    a.B = B + L.pop_off;
+    a.flags = flags + L.node_off;
+    a.macros = macros + 4 * L.node_off;
+    a.N = L.N;
+    a.nxy = L.N;
+    a.P = (uint32_t)L.nx;
+    a.nx = L.nx; a.ny = L.ny; a.nz = 1;
+    a.ti = L.ti;
+    a.bc[0] = LB_BC_INLET_EQ; a.bc[1] = LB_BC_OUTFLOW; a.bc[2] = LB_BC_WALL; a.bc[3] = LB_BC_WALL;
+    a.bc[4] = LB_BC_PERIODIC; a.bc[5] = LB_BC_PERIODIC;
+    a.bcu[0][0] = L.u_in;
+    a.bcrho[0] = 1.0f;
+    a.corr_lo[0] = 1; a.corr_hi[0] = -1;
+    a.bx_shift = 6; a.by = kBY;
+    a.ntx = L.ntx; a.nty = L.nty;
+    a.nblocks = L.ntx * L.nty;
+    return a;
+}
+
+template <bool MACROS>
+__global__ __launch_bounds__(kThreads) void k_batch_step(const BatchLat *lats, const int2 *table, float *A, float *B,
+                                                         const uint8_t *flags, float *macros) {
+    const int2 e = table[blockIdx.x];          // (lattice, local tile)
+    const BatchLat L = lats[e.x];
+    const StepArgs<StF32> a = lattice_args(L, A, B, flags, macros);
+    step_body<9, StF32, false, MACROS>(a, node_of(a, e.y), nullptr);
+}
+
+// Per-tile residual partials of the velocity change since the snapshot.
+__global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lats, const int2 *table,
+                                                             const float *macros, float *snap,
+                                                             const uint8_t *flags, double *partial) {
+    const int2 e = table[blockIdx.x];
+    const BatchLat L = lats[e.x];
+    int tx = e.y % L.ntx, ty = e.y / L.ntx;
+    int x = tx * kBX + (threadIdx.x & (kBX - 1)), y = ty * kBY + (threadIdx.x >> 6);
+    double num = 0, den = 0;
+    if (x < L.nx && y < L.ny) {
+        uint32_t id = (uint32_t)x + (uint32_t)L.nx * (uint32_t)y;
+        if (!(flags[L.node_off + id] & F_SOLID)) {
+            const float *m = macros + 4 * L.node_off;
+            float *sn = snap + 2 * L.node_off;
+            for (int c = 0; c < 2; c++) {
+                double u = m[(size_t)(1 + c) * L.N + id];
+                double s = sn[(size_t)c * L.N + id];
+                num += (u - s) * (u - s);
+                den += u * u;
+                sn[(size_t)c * L.N + id] = (float)u;
+            }
+        }
+    }
+    __shared__ double sh[2][kThreads / 64];
+    double s0 = wave_sum(num), s1 = wave_sum(den);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][w] = s0; sh[1][w] = s1; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0;
+        for (int k = 0; k < kThreads / 64; k++) t += sh[threadIdx.x][k];
+        partial[(size_t)blockIdx.x * 2 + threadIdx.x] = t;
+    }
+}
+
+// One thread per active lattice: fixed-order sum over its tiles.
+__global__ void k_batch_final(const int2 *ranges, int nact, const double *partial, double *res) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nact) return;
+    int2 r = ranges[k];   // (first block, count)
+    double num = 0, den = 0;
+    for (int b = r.x; b < r.x + r.y; b++) { num += partial[2 * b]; den += partial[2 * b + 1]; }
+    res[k] = den > 0 ? sqrt(num / den) : 0.0;
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, const int2 *table, float *A, float *B,
+                                                         const float *fields) {
+    const int2 e = table[blockIdx.x];
+    const BatchLat L = lats[e.x];
+    int tx = e.y % L.ntx, ty = e.y / L.ntx;
+    int x = tx * kBX + (threadIdx.x & (kBX - 1)), y = ty * kBY + (threadIdx.x >> 6);
+    if (x >= L.nx || y >= L.ny) return;
+    uint32_t id = (uint32_t)x + (uint32_t)L.nx * (uint32_t)y;
+    float rho = 1.0f, ux = 0.0f, uy = 0.0f;
+    if (fields) {
+        const float *F = fields + 3 * L.node_off;
+        rho = F[id]; ux = F[L.N + id]; uy = F[2 * (size_t)L.N + id];
+    }
+    float f[9];
+    feq9<float>(f, rho, ux, uy, weight<9, float>(0), weight<9, float>(1), weight<9, float>(2));
+    for (int i = 0; i < 9; i++) {
+        A[L.pop_off + (size_t)i * L.N + id] = f[i];
+        B[L.pop_off + (size_t)i * L.N + id] = f[i];
+    }
+}
+
+}  // namespace
+
+struct lb_batch {
+    int n = 0;
+    std::vector<BatchLat> lats;
+    std::vector<int> cur;          // buffer holding each lattice's current state
+    std::vector<char> active;
+    std::vector<char> snap_valid;
+    int parity = 0;                // buffer the active lattices read next
+    uint64_t pop_total = 0, node_total = 0;
+    hipStream_t stream = nullptr;
+    BatchLat *d_lats = nullptr;
+    int2 *d_table = nullptr, *d_ranges = nullptr;
+    float *A = nullptr, *B = nullptr, *macros = nullptr, *snap = nullptr;
+    uint8_t *flags = nullptr;
+    double *partial = nullptr, *d_res = nullptr;
+    std::vector<int2> table, ranges;
+    std::vector<int> act_list;
+    int nblocks = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_k = 0, graph_parity = -1;
+    std::vector<char> macros_valid;
+
+    ~lb_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        void *ps[] = {d_lats, d_table, d_ranges, A, B, macros, snap, flags, partial, d_res};
+        for (void *p : ps) if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Block table over the active lattices (all lattices when all_lattices).
+int rebuild_table(lb_batch *b, bool all_lattices = false) {
+    b->table.clear();
+    b->ranges.clear();
+    b->act_list.clear();
+    for (int i = 0; i < b->n; i++) {
+        if (!all_lattices && !b->active[i]) continue;
+        const BatchLat &L = b->lats[i];
+        int nt = L.ntx * L.nty;
+        b->ranges.push_back(int2{(int)b->table.size(), nt});
+        b->act_list.push_back(i);
+        for (int t = 0; t < nt; t++) b->table.push_back(int2{i, t});
+    }
+    b->nblocks = (int)b->table.size();
+    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+    if (b->nblocks == 0) return 0;
+    int rc = (int)hipMemcpyAsync(b->d_table, b->table.data(), b->table.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(b->d_ranges, b->ranges.data(), b->ranges.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+void launch_batch_step(lb_batch *b, int parity, bool macros) {
+    float *src = parity == 0 ? b->A : b->B;
+    float *dst = parity == 0 ? b->B : b->A;
+    if (macros) hipLaunchKernelGGL(k_batch_step<true>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, src, dst, b->flags, b->macros);
+    else hipLaunchKernelGGL(k_batch_step<false>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, src, dst, b->flags, b->macros);
+}
+
+void launch_batch_residual(lb_batch *b) {
+    hipLaunchKernelGGL(k_batch_residual, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->macros,
+                       b->snap, b->flags, b->partial);
+    int na = (int)b->act_list.size();
+    hipLaunchKernelGGL(k_batch_final, dim3((na + 63) / 64), dim3(64), 0, b->stream, b->d_ranges, na, b->partial, b->d_res);
+}
+
+}  // namespace
+
+extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t *solids, lb_batch **out) {
+    if (n <= 0 || !lat || !solids || !out) return LB_ERR_INVALID;
+    *out = nullptr;
+    if (lb_device_count() <= 0) return LB_ERR_NO_DEVICE;
+    lb_batch *b = new (std::nothrow) lb_batch();
+    if (!b) return LB_ERR_INVALID;
+    b->n = n;
+    b->lats.resize(n);
+    uint64_t pop = 0, node = 0;
+    size_t total_tiles = 0;
+    std::vector<uint8_t> flags_all;
+    for (int i = 0; i < n; i++) {
+        const lb_batch_lattice &s = lat[i];
+        if (s.nx < 2 || s.ny < 3 || (uint64_t)s.nx * s.ny * 9 >= (1ull << 31)) { delete b; return LB_ERR_INVALID; }
+        BatchLat &L = b->lats[i];
+        L.nx = s.nx; L.ny = s.ny; L.N = (uint32_t)s.nx * s.ny;
+        L.ti = s.tau_inv; L.u_in = s.u_in;
+        L.pop_off = pop; L.node_off = node;
+        L.ntx = (s.nx + kBX - 1) / kBX; L.nty = (s.ny + kBY - 1) / kBY;
+        total_tiles += (size_t)L.ntx * L.nty;
+        // node flags with the same rules as the single-lattice solver
+        lb_config cfg{};
+        cfg.lattice = 9; cfg.nx = s.nx; cfg.ny = s.ny; cfg.nz = 1;
+        cfg.bc[0] = LB_BC_INLET_EQ; cfg.bc[1] = LB_BC_OUTFLOW; cfg.bc[2] = LB_BC_WALL; cfg.bc[3] = LB_BC_WALL;
+        std::vector<uint8_t> sol(solids + s.solid_offset, solids + s.solid_offset + L.N), fl;
+        for (auto &v : sol) v = v ? 1 : 0;
+        build_flags(cfg, sol, fl);
+        flags_all.insert(flags_all.end(), fl.begin(), fl.end());
+        pop += 9ull * L.N;
+        node += L.N;
+    }
+    b->pop_total = pop;
+    b->node_total = node;
+    b->cur.assign(n, 0);
+    b->active.assign(n, 1);
+    b->snap_valid.assign(n, 0);
+    b->macros_valid.assign(n, 0);
+    int rc = (int)hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_lats, n * sizeof(BatchLat));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_table, total_tiles * sizeof(int2));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_ranges, n * sizeof(int2));
+    if (!rc) rc = (int)hipMalloc((void **)&b->A, pop * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&b->B, pop * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&b->macros, 4 * node * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&b->snap, 2 * node * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&b->flags, node);
+    if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 2 * sizeof(double));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
+    if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipMemsetAsync(b->macros, 0, 4 * node * sizeof(float), b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    if (!rc) rc = rebuild_table(b);
+    if (!rc) rc = lb_batch_init_rest(b);
+    if (rc) { delete b; return rc; }
+    *out = b;
+    return 0;
+}
+
+extern "C" int lb_batch_destroy(lb_batch *b) {
+    if (!b) return LB_ERR_INVALID;
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return 0;
+}
+
+static int batch_init(lb_batch *b, const float *fields) {
+    std::fill(b->active.begin(), b->active.end(), 1);
+    std::fill(b->snap_valid.begin(), b->snap_valid.end(), 0);
+    std::fill(b->macros_valid.begin(), b->macros_valid.end(), 0);
+    std::fill(b->cur.begin(), b->cur.end(), 0);
+    b->parity = 0;
+    int rc = rebuild_table(b);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_batch_init, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->A, b->B, fields);
+    rc = (int)hipGetLastError();
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+extern "C" int lb_batch_init_rest(lb_batch *b) {
+    if (!b) return LB_ERR_INVALID;
+    return batch_init(b, nullptr);
+}
+
+extern "C" int lb_batch_init_fields_device(lb_batch *b, const float *d_fields) {
+    if (!b || !d_fields) return LB_ERR_INVALID;
+    return batch_init(b, d_fields);
+}
+
+extern "C" int lb_batch_step(lb_batch *b, int nsteps) {
+    if (!b || nsteps < 0) return LB_ERR_INVALID;
+    if (b->nblocks == 0 || nsteps == 0) return 0;
+    for (int s = 0; s < nsteps; s++) {
+        launch_batch_step(b, b->parity, s == nsteps - 1);
+        b->parity = 1 - b->parity;
+    }
+    for (int i : b->act_list) { b->cur[i] = b->parity; b->macros_valid[i] = 1; }
+    int rc = (int)hipGetLastError();
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+extern "C" int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *steps_out, double *res_out) {
+    if (!b || k < 2 || max_steps < 0) return LB_ERR_INVALID;
+    std::vector<int> steps(b->n, 0);
+    std::vector<double> res(b->n, 1.0);
+    std::vector<double> hres(b->n);
+    int done = 0;
+    int rc = 0;
+    while (b->nblocks > 0 && done + k <= max_steps) {
+        // graph: k-1 plain steps, one step writing macros, residual kernels
+        if (!b->graph || b->graph_k != k || b->graph_parity != b->parity) {
+            if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+            hipGraph_t g = nullptr;
+            rc = (int)hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+            if (rc) return rc;
+            int par = b->parity;
+            for (int s = 0; s < k; s++) { launch_batch_step(b, par, s == k - 1); par = 1 - par; }
+            launch_batch_residual(b);
+            rc = (int)hipStreamEndCapture(b->stream, &g);
+            if (rc) return rc;
+            rc = (int)hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (rc) { b->graph = nullptr; return rc; }
+            b->graph_k = k;
+            b->graph_parity = b->parity;
+        }
+        rc = (int)hipGraphLaunch(b->graph, b->stream);
+        if (!rc) rc = (int)hipMemcpyAsync(hres.data(), b->d_res, b->act_list.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
+        if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+        if (rc) return rc;
+        if (k & 1) b->parity = 1 - b->parity;
+        done += k;
+        bool changed = false;
+        for (size_t a = 0; a < b->act_list.size(); a++) {
+            int i = b->act_list[a];
+            steps[i] += k;
+            b->cur[i] = b->parity;
+            b->macros_valid[i] = 1;
+            res[i] = b->snap_valid[i] ? hres[a] : 1.0;
+            if (b->snap_valid[i] && !(hres[a] >= tol)) {   // converged (or NaN: retire and report)
+                b->active[i] = 0;
+                changed = true;
+            }
+            b->snap_valid[i] = 1;
+        }
+        if (changed) {
+            rc = rebuild_table(b);
+            if (rc) return rc;
+        }
+    }
+    if (steps_out) for (int i = 0; i < b->n; i++) steps_out[i] = steps[i];
+    if (res_out) for (int i = 0; i < b->n; i++) res_out[i] = res[i];
+    return 0;
+}
+
+extern "C" int lb_batch_get_macros(lb_batch *b, int i, float *rho, float *ux, float *uy) {
+    if (!b || i < 0 || i >= b->n) return LB_ERR_INVALID;
+    if (!b->macros_valid[i]) return LB_ERR_NOT_READY;
+    const BatchLat &L = b->lats[i];
+    float *dst[3] = {rho, ux, uy};
+    int rc = 0;
+    for (int c = 0; c < 3 && !rc; c++)
+        if (dst[c]) rc = (int)hipMemcpyAsync(dst[c], b->macros + 4 * L.node_off + (size_t)c * L.N, L.N * sizeof(float),
+                                             hipMemcpyDeviceToHost, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+extern "C" int lb_batch_get_populations(lb_batch *b, int i, float *host) {
+    if (!b || !host || i < 0 || i >= b->n) return LB_ERR_INVALID;
+    const BatchLat &L = b->lats[i];
+    const float *src = (b->cur[i] == 0 ? b->A : b->B) + L.pop_off;
+    int rc = (int)hipMemcpyAsync(host, src, 9ull * L.N * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+extern "C" int64_t lb_batch_total_nodes(lb_batch *b) { return b ? (int64_t)b->node_total : -1; }

@@ -1,0 +1,340 @@
+// lb_common.hpp -- device-side numerics shared by every LBM kernel.
+//
+// Everything here is compiled with -ffp-contract=off: each float/double
+// operation rounds exactly as written, so results match a host evaluation of
+// the same expressions bit for bit.  Expression order follows the reference
+// kernels (src/collision.cu), cited per function.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "../../include/lattibol_amd.h"
+
+#define LB_DEV __device__ __forceinline__
+
+namespace lb {
+
+// ---------------------------------------------------------------------------
+// Velocity sets.  D2Q9 storage rows grow downward (streaming.cu:48-54) while
+// the moments use physical y up (collision.cu:274): physical c_y = -row c_y.
+// D3Q19 directions per collision.cu:463-478 / streaming.cu:116-161.
+// ---------------------------------------------------------------------------
+// Tables as constexpr functions so device code can evaluate them with
+// compile-time indices (every loop over directions is unrolled by
+// static_for below).
+template <int Q> struct Vel;
+template <> struct Vel<9> {
+    static constexpr __host__ __device__ int cx(int i) {
+        return (i == 1 || i == 5 || i == 8) ? 1 : ((i == 3 || i == 6 || i == 7) ? -1 : 0);
+    }
+    static constexpr __host__ __device__ int cy(int i) {   // storage rows
+        return (i == 4 || i == 7 || i == 8) ? 1 : ((i == 2 || i == 5 || i == 6) ? -1 : 0);
+    }
+    static constexpr __host__ __device__ int cz(int) { return 0; }
+    static constexpr __host__ __device__ int opp(int i) {
+        return i == 0 ? 0 : (i <= 4 ? ((i + 1) % 4 + 1) : ((i - 5 + 2) % 4 + 5));
+    }
+    static constexpr __host__ __device__ int wc(int i) { return i == 0 ? 0 : (i < 5 ? 1 : 2); }
+    static constexpr __host__ __device__ int pcy(int i) { return -cy(i); }   // physical y
+};
+template <> struct Vel<19> {
+    static constexpr __host__ __device__ int cx(int i) {
+        return (i == 1 || (i >= 7 && i <= 10)) ? 1 : ((i == 2 || (i >= 11 && i <= 14)) ? -1 : 0);
+    }
+    static constexpr __host__ __device__ int cy(int i) {
+        return (i == 3 || i == 7 || i == 11 || i == 15 || i == 16) ? 1
+             : ((i == 4 || i == 8 || i == 12 || i == 17 || i == 18) ? -1 : 0);
+    }
+    static constexpr __host__ __device__ int cz(int i) {
+        return (i == 5 || i == 9 || i == 13 || i == 15 || i == 17) ? 1
+             : ((i == 6 || i == 10 || i == 14 || i == 16 || i == 18) ? -1 : 0);
+    }
+    static constexpr __host__ __device__ int opp(int i) {
+        return i == 0 ? 0 : (i <= 6 ? (i % 2 ? i + 1 : i - 1)
+             : i == 7 ? 12 : i == 8 ? 11 : i == 9 ? 14 : i == 10 ? 13
+             : i == 11 ? 8 : i == 12 ? 7 : i == 13 ? 10 : i == 14 ? 9 : 33 - i);
+    }
+    static constexpr __host__ __device__ int wc(int i) { return i == 0 ? 0 : (i < 7 ? 1 : 2); }
+    static constexpr __host__ __device__ int pcy(int i) { return cy(i); }
+};
+static_assert(Vel<9>::opp(1) == 3 && Vel<9>::opp(2) == 4 && Vel<9>::opp(3) == 1 && Vel<9>::opp(4) == 2, "opp9");
+static_assert(Vel<9>::opp(5) == 7 && Vel<9>::opp(6) == 8 && Vel<9>::opp(7) == 5 && Vel<9>::opp(8) == 6, "opp9");
+static_assert(Vel<19>::opp(7) == 12 && Vel<19>::opp(8) == 11 && Vel<19>::opp(9) == 14 && Vel<19>::opp(10) == 13, "opp19");
+static_assert(Vel<19>::opp(15) == 18 && Vel<19>::opp(16) == 17 && Vel<19>::opp(1) == 2 && Vel<19>::opp(6) == 5, "opp19");
+
+// Position of a direction inside the five +z (5,9,13,15,17) or five -z
+// (6,10,14,16,18) populations: the halo-plane slot.
+constexpr __host__ __device__ int zslot(int i) {
+    return (i == 5 || i == 6) ? 0 : (i == 9 || i == 10) ? 1 : (i == 13 || i == 14) ? 2
+         : (i == 15 || i == 16) ? 3 : 4;
+}
+
+template <int B, int E, class F> LB_DEV void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// Lattice weights as float members assigned from double quotients
+// (simulation_d2q9.cu:56-58, simulation_d3q19.cu:54-56); exact double
+// quotients for FP64 runs.
+template <int Q, class T> LB_DEV T weight(int wclass);
+template <> LB_DEV float weight<9, float>(int c) {
+    return c == 0 ? (float)(16. / 36.) : (c == 1 ? (float)(4. / 36.) : (float)(1. / 36.));
+}
+template <> LB_DEV float weight<19, float>(int c) {
+    return c == 0 ? (float)(12. / 36.) : (c == 1 ? (float)(2. / 36.) : (float)(1. / 36.));
+}
+template <> LB_DEV double weight<9, double>(int c) {
+    return c == 0 ? 16. / 36. : (c == 1 ? 4. / 36. : 1. / 36.);
+}
+template <> LB_DEV double weight<19, double>(int c) {
+    return c == 0 ? 12. / 36. : (c == 1 ? 2. / 36. : 1. / 36.);
+}
+
+// ---------------------------------------------------------------------------
+// Storage policies.  Shifted 16-bit storage keeps f_i - w_i (float weight).
+// ---------------------------------------------------------------------------
+LB_DEV uint16_t f32_to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+LB_DEV float bf16_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
+LB_DEV uint16_t f32_to_f16(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+struct StF32 {
+    typedef float raw; typedef float T;
+    static LB_DEV T ld(raw v, float) { return v; }
+    static LB_DEV raw st(T f, float) { return f; }
+};
+struct StF64 {
+    typedef double raw; typedef double T;
+    static LB_DEV T ld(raw v, float) { return v; }
+    static LB_DEV raw st(T f, float) { return f; }
+};
+struct StF16 {
+    typedef uint16_t raw; typedef float T;
+    static LB_DEV T ld(raw v, float w) { return f16_to_f32(v) + w; }
+    static LB_DEV raw st(T f, float w) { return f32_to_f16(f - w); }
+};
+struct StBF16 {
+    typedef uint16_t raw; typedef float T;
+    static LB_DEV T ld(raw v, float w) { return bf16_to_f32(v) + w; }
+    static LB_DEV raw st(T f, float w) { return f32_to_bf16(f - w); }
+};
+
+// ---------------------------------------------------------------------------
+// exp: Cody-Waite reduction with fdlibm's split ln2, degree-13 Taylor
+// polynomial on |r| <= ln2/2, exact 2^k scaling.  The same IEEE operation
+// sequence as the host evaluation, hence bit-identical.
+// ---------------------------------------------------------------------------
+LB_DEV double exp_cw(double x) {
+    const double INV_LN2 = 1.44269504088896338700e+00;
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    if (x != x) return x;
+    if (x > 709.0) return __builtin_huge_val();
+    if (x < -708.0) return 0.0;
+    double kd = __builtin_rint(x * INV_LN2);
+    double r = (x - kd * LN2_HI) - kd * LN2_LO;
+    double p = 1.0 / 6227020800.0;
+    p = p * r + 1.0 / 479001600.0;
+    p = p * r + 1.0 / 39916800.0;
+    p = p * r + 1.0 / 3628800.0;
+    p = p * r + 1.0 / 362880.0;
+    p = p * r + 1.0 / 40320.0;
+    p = p * r + 1.0 / 5040.0;
+    p = p * r + 1.0 / 720.0;
+    p = p * r + 1.0 / 120.0;
+    p = p * r + 1.0 / 24.0;
+    p = p * r + 1.0 / 6.0;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    int k = (int)kd;
+    uint64_t bits = (uint64_t)(int64_t)(k + 1023) << 52;
+    return p * __builtin_bit_cast(double, bits);
+}
+
+// psi = rho0 (1 - exp(-rho/rho0)) in double, rounded to T.
+template <class T> LB_DEV T sc_psi(T rho, float rho0) {
+    double r = (double)rho / (double)rho0;
+    return (T)((double)rho0 * (1.0 - exp_cw(-r)));
+}
+
+// ---------------------------------------------------------------------------
+// BGK and equilibria with the reference expression order.
+// D2Q9: collision.cu:299-330.  D3Q19: collision.cu:481-541.
+// ---------------------------------------------------------------------------
+template <class T>
+LB_DEV void bgk9(T *f, T rho, T ux, T uy, T ti, T w0, T w1, T w5) {
+    T uxs = ux * ux, uys = uy * uy;
+    T usq = uxs + uys;
+    T a0 = w0 * rho, a1 = w1 * rho, a5 = w5 * rho;
+    T s5 = +ux + uy;
+    T s6 = -ux + uy;
+    const T one = T(1), three = T(3), c45 = T(4.5), c15 = T(1.5);
+    f[0] = f[0] - ti * (f[0] - a0 * (one - c15 * usq));
+    f[1] = f[1] - ti * (f[1] - a1 * (one + three * ux + c45 * uxs - c15 * usq));
+    f[2] = f[2] - ti * (f[2] - a1 * (one + three * uy + c45 * uys - c15 * usq));
+    f[3] = f[3] - ti * (f[3] - a1 * (one - three * ux + c45 * uxs - c15 * usq));
+    f[4] = f[4] - ti * (f[4] - a1 * (one - three * uy + c45 * uys - c15 * usq));
+    f[5] = f[5] - ti * (f[5] - a5 * (one + three * s5 + c45 * s5 * s5 - c15 * usq));
+    f[6] = f[6] - ti * (f[6] - a5 * (one + three * s6 + c45 * s6 * s6 - c15 * usq));
+    f[7] = f[7] - ti * (f[7] - a5 * (one - three * s5 + c45 * s5 * s5 - c15 * usq));
+    f[8] = f[8] - ti * (f[8] - a5 * (one - three * s6 + c45 * s6 * s6 - c15 * usq));
+}
+template <class T>
+LB_DEV void feq9(T *f, T rho, T ux, T uy, T w0, T w1, T w5) {
+    T uxs = ux * ux, uys = uy * uy;
+    T usq = uxs + uys;
+    T a0 = w0 * rho, a1 = w1 * rho, a5 = w5 * rho;
+    T s5 = +ux + uy;
+    T s6 = -ux + uy;
+    const T one = T(1), three = T(3), c45 = T(4.5), c15 = T(1.5);
+    f[0] = a0 * (one - c15 * usq);
+    f[1] = a1 * (one + three * ux + c45 * uxs - c15 * usq);
+    f[2] = a1 * (one + three * uy + c45 * uys - c15 * usq);
+    f[3] = a1 * (one - three * ux + c45 * uxs - c15 * usq);
+    f[4] = a1 * (one - three * uy + c45 * uys - c15 * usq);
+    f[5] = a5 * (one + three * s5 + c45 * s5 * s5 - c15 * usq);
+    f[6] = a5 * (one + three * s6 + c45 * s6 * s6 - c15 * usq);
+    f[7] = a5 * (one - three * s5 + c45 * s5 * s5 - c15 * usq);
+    f[8] = a5 * (one - three * s6 + c45 * s6 * s6 - c15 * usq);
+}
+template <class T>
+LB_DEV void bgk19(T *f, T rho, T ux, T uy, T uz, T ti, T w0, T w1, T w7) {
+    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
+    T usq = uxsq + uysq + uzsq;
+    T a0 = w0 * rho, a1 = w1 * rho, a7 = w7 * rho;
+    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
+    const T one = T(1), three = T(3), c45 = T(4.5), c15 = T(1.5);
+    f[0] = f[0] - ti * (f[0] - a0 * (one - c15 * usq));
+    f[1] = f[1] - ti * (f[1] - a1 * (one + three * ux + c45 * uxsq - c15 * usq));
+    f[2] = f[2] - ti * (f[2] - a1 * (one - three * ux + c45 * uxsq - c15 * usq));
+    f[3] = f[3] - ti * (f[3] - a1 * (one + three * uy + c45 * uysq - c15 * usq));
+    f[4] = f[4] - ti * (f[4] - a1 * (one - three * uy + c45 * uysq - c15 * usq));
+    f[5] = f[5] - ti * (f[5] - a1 * (one + three * uz + c45 * uzsq - c15 * usq));
+    f[6] = f[6] - ti * (f[6] - a1 * (one - three * uz + c45 * uzsq - c15 * usq));
+    f[7] = f[7] - ti * (f[7] - a7 * (one + three * s7 + c45 * s7 * s7 - c15 * usq));
+    f[8] = f[8] - ti * (f[8] - a7 * (one + three * s8 + c45 * s8 * s8 - c15 * usq));
+    f[9] = f[9] - ti * (f[9] - a7 * (one + three * s9 + c45 * s9 * s9 - c15 * usq));
+    f[10] = f[10] - ti * (f[10] - a7 * (one + three * s10 + c45 * s10 * s10 - c15 * usq));
+    f[11] = f[11] - ti * (f[11] - a7 * (one - three * s8 + c45 * s8 * s8 - c15 * usq));
+    f[12] = f[12] - ti * (f[12] - a7 * (one - three * s7 + c45 * s7 * s7 - c15 * usq));
+    f[13] = f[13] - ti * (f[13] - a7 * (one - three * s10 + c45 * s10 * s10 - c15 * usq));
+    f[14] = f[14] - ti * (f[14] - a7 * (one - three * s9 + c45 * s9 * s9 - c15 * usq));
+    f[15] = f[15] - ti * (f[15] - a7 * (one + three * s15 + c45 * s15 * s15 - c15 * usq));
+    f[16] = f[16] - ti * (f[16] - a7 * (one + three * s16 + c45 * s16 * s16 - c15 * usq));
+    f[17] = f[17] - ti * (f[17] - a7 * (one - three * s16 + c45 * s16 * s16 - c15 * usq));
+    f[18] = f[18] - ti * (f[18] - a7 * (one - three * s15 + c45 * s15 * s15 - c15 * usq));
+}
+template <class T>
+LB_DEV void feq19(T *f, T rho, T ux, T uy, T uz, T w0, T w1, T w7) {
+    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
+    T usq = uxsq + uysq + uzsq;
+    T a0 = w0 * rho, a1 = w1 * rho, a7 = w7 * rho;
+    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
+    const T one = T(1), three = T(3), c45 = T(4.5), c15 = T(1.5);
+    f[0] = a0 * (one - c15 * usq);
+    f[1] = a1 * (one + three * ux + c45 * uxsq - c15 * usq);
+    f[2] = a1 * (one - three * ux + c45 * uxsq - c15 * usq);
+    f[3] = a1 * (one + three * uy + c45 * uysq - c15 * usq);
+    f[4] = a1 * (one - three * uy + c45 * uysq - c15 * usq);
+    f[5] = a1 * (one + three * uz + c45 * uzsq - c15 * usq);
+    f[6] = a1 * (one - three * uz + c45 * uzsq - c15 * usq);
+    f[7] = a7 * (one + three * s7 + c45 * s7 * s7 - c15 * usq);
+    f[8] = a7 * (one + three * s8 + c45 * s8 * s8 - c15 * usq);
+    f[9] = a7 * (one + three * s9 + c45 * s9 * s9 - c15 * usq);
+    f[10] = a7 * (one + three * s10 + c45 * s10 * s10 - c15 * usq);
+    f[11] = a7 * (one - three * s8 + c45 * s8 * s8 - c15 * usq);
+    f[12] = a7 * (one - three * s7 + c45 * s7 * s7 - c15 * usq);
+    f[13] = a7 * (one - three * s10 + c45 * s10 * s10 - c15 * usq);
+    f[14] = a7 * (one - three * s9 + c45 * s9 * s9 - c15 * usq);
+    f[15] = a7 * (one + three * s15 + c45 * s15 * s15 - c15 * usq);
+    f[16] = a7 * (one + three * s16 + c45 * s16 * s16 - c15 * usq);
+    f[17] = a7 * (one - three * s16 + c45 * s16 * s16 - c15 * usq);
+    f[18] = a7 * (one - three * s15 + c45 * s15 * s15 - c15 * usq);
+}
+
+// Moments: collision.cu:265-275 (D2Q9, physical uy) and :454-478 (D3Q19).
+template <class T> LB_DEV void mom9(const T *f, T &rho, T &ux, T &uy) {
+    rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8];
+    ux = (f[1] - f[3] + f[5] - f[6] - f[7] + f[8]) / rho;
+    uy = (f[2] - f[4] + f[5] + f[6] - f[7] - f[8]) / rho;
+}
+template <class T> LB_DEV void mom19(const T *f, T &rho, T &ux, T &uy, T &uz) {
+    rho = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] +
+          f[12] + f[13] + f[14] + f[15] + f[16] + f[17] + f[18];
+    ux = (f[1] + f[7] + f[8] + f[9] + f[10] - f[2] - f[11] - f[12] - f[13] - f[14]) / rho;
+    uy = (f[3] + f[7] + f[11] + f[15] + f[16] - f[4] - f[8] - f[12] - f[17] - f[18]) / rho;
+    uz = (f[5] + f[9] + f[13] + f[15] + f[17] - f[6] - f[10] - f[14] - f[16] - f[18]) / rho;
+}
+
+// Zou-He with the reference's double promotions (collision.cu:211-262 D2Q9,
+// :413-451 D3Q19 incl. the f11-for-f17 z correction and the outlet's reuse
+// of the freshly written f11).
+template <class T> LB_DEV void zouhe9_in(T *f, float ux_in) {
+    T rho = (T)(((double)(f[0] + f[2] + f[4]) + 2. * (double)(f[3] + f[7] + f[6])) /
+                (1. - (double)ux_in));
+    T ru = rho * (T)ux_in;
+    f[1] = (T)((double)f[3] + (2. / 3.) * (double)ru);
+    f[5] = (T)((double)f[7] + (1. / 6.) * (double)ru - 0.5 * (double)(f[2] - f[4]));
+    f[8] = (T)((double)f[6] + (1. / 6.) * (double)ru - 0.5 * (double)(f[4] - f[2]));
+}
+template <class T> LB_DEV void zouhe9_out(T *f, float rho_out) {
+    T ux = (T)(-1. + ((double)(f[0] + f[2] + f[4]) + 2. * (double)(f[1] + f[5] + f[8])) /
+                         (double)rho_out);
+    T ru = (T)rho_out * ux;
+    f[3] = (T)((double)f[1] - (2. / 3.) * (double)ru);
+    f[7] = (T)((double)f[5] - (1. / 6.) * (double)ru + 0.5 * (double)(f[2] - f[4]));
+    f[6] = (T)((double)f[8] - (1. / 6.) * (double)ru + 0.5 * (double)(f[4] - f[2]));
+}
+template <class T> LB_DEV void zouhe19_in(T *f, float ux_in) {
+    T rho = (T)(((double)(f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18]) +
+                 2. * (double)(f[2] + f[11] + f[12] + f[13] + f[14])) /
+                (1. - (double)ux_in));
+    T ru = rho * (T)ux_in;
+    double ny = (double)(f[3] + f[15] + f[16] - f[4] - f[17] - f[18]);
+    double nz = (double)(f[5] + f[11] + f[15] - f[6] - f[16] - f[18]);
+    f[1] = (T)((double)f[2] + (1. / 3.) * (double)ru);
+    f[8] = (T)((double)f[11] + (1. / 6.) * (double)ru + 0.5 * ny);
+    f[7] = (T)((double)f[12] + (1. / 6.) * (double)ru - 0.5 * ny);
+    f[9] = (T)((double)f[14] + (1. / 6.) * (double)ru - 0.5 * nz);
+    f[10] = (T)((double)f[13] + (1. / 6.) * (double)ru + 0.5 * nz);
+}
+template <class T> LB_DEV void zouhe19_out(T *f, float rho_out) {
+    T ux = (T)(-1 + ((double)(f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18]) +
+                     2. * (double)(f[1] + f[7] + f[8] + f[9] + f[10])) /
+                        (double)rho_out);
+    T ru = (T)rho_out * ux;
+    double ny = (double)(f[3] + f[15] + f[16] - f[4] - f[17] - f[18]);
+    f[2] = (T)((double)f[1] - (1. / 3.) * (double)ru);
+    f[11] = (T)((double)f[8] - (1. / 6.) * (double)ru - 0.5 * ny);
+    f[12] = (T)((double)f[7] - (1. / 6.) * (double)ru + 0.5 * ny);
+    double nz = (double)(f[5] + f[11] + f[15] - f[6] - f[16] - f[18]);
+    f[14] = (T)((double)f[9] - (1. / 6.) * (double)ru + 0.5 * nz);
+    f[13] = (T)((double)f[10] - (1. / 6.) * (double)ru - 0.5 * nz);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 wavefront reduction (64 lanes) via cross-lane shuffles, then LDS.
+// ---------------------------------------------------------------------------
+LB_DEV double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+}  // namespace lb
+
+// Sticky error state for the void reference entry points (lb_runtime.cpp).
+extern "C" void lb_set_error(int code);
+int lb_hip_err(hipError_t e);

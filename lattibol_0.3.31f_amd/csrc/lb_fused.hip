@@ -1,0 +1,650 @@
+// lb_fused.hip -- the lb_* solver handle around the fused single-sweep pull
+// kernels of lb_pull.hpp (D2Q9 / D3Q19, A -> B double buffering).
+//
+// Data layout in HBM: population i of node n = x + nx (y + ny z) at
+// element i*N + n of one buffer per time level; 1-byte node flags; macro
+// planes rho,ux,uy,uz; Shan-Chen psi plane; z halo planes (send/recv).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "lb_pull.hpp"
+
+using namespace lb;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Kernels: one thread per node, 256-thread workgroups = BX x BY tiles of one
+// z-plane, tiles optionally remapped so each XCD sweeps a contiguous slab.
+// ---------------------------------------------------------------------------
+template <int Q, class S, bool SC, bool MACROS>
+__global__ __launch_bounds__(kThreads) void k_step(StepArgs<S> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int b = tile_of_block(a);
+    const NodePos p = node_of(a, b);
+    if constexpr (SC) {
+        load_psi_tile<Q, S>(a, b, (typename S::T *)lds_raw, 1 << a.bx_shift, a.by);
+        __syncthreads();
+    }
+    step_body<Q, S, SC, MACROS>(a, p, lds_raw);
+}
+
+template <int Q, class S>
+__global__ __launch_bounds__(kThreads) void k_density(StepArgs<S> a) {
+    density_body<Q, S>(a, node_of(a, tile_of_block(a)));
+}
+
+// Equilibrium initialisation from device fields.
+template <int Q, class S>
+__global__ __launch_bounds__(kThreads) void k_init_eq(typename S::raw *A, typename S::raw *B, uint32_t N,
+                                                      const float *rho, const float *ux, const float *uy,
+                                                      const float *uz) {
+    typedef typename S::T T;
+    uint32_t id = blockIdx.x * kThreads + threadIdx.x;
+    if (id >= N) return;
+    T f[Q];
+    feq<Q, T>(f, (T)rho[id], (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        auto v = S::st(f[i], weight<Q, float>(Vel<Q>::wc(i)));
+        A[(size_t)i * N + id] = v;
+        B[(size_t)i * N + id] = v;
+    });
+}
+
+// FP64 totals of the populations over fluid nodes: per-block partials.
+template <int Q, class S>
+__global__ __launch_bounds__(kThreads) void k_reduce(const typename S::raw *A, const uint8_t *flags, uint32_t N,
+                                                     double *partial) {
+    typedef Vel<Q> L;
+    double m = 0, px = 0, py = 0, pz = 0;
+    for (uint32_t id = blockIdx.x * kThreads + threadIdx.x; id < N; id += gridDim.x * kThreads) {
+        if (flags[id] & F_SOLID) continue;
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            double v = (double)S::ld(A[(size_t)i * N + id], weight<Q, float>(L::wc(i)));
+            m += v;
+            if constexpr (L::cx(i) > 0) px += v; else if constexpr (L::cx(i) < 0) px -= v;
+            if constexpr (L::pcy(i) > 0) py += v; else if constexpr (L::pcy(i) < 0) py -= v;
+            if constexpr (L::cz(i) > 0) pz += v; else if constexpr (L::cz(i) < 0) pz -= v;
+        });
+    }
+    __shared__ double sh[4][kThreads / 64];
+    double v4[4] = {wave_sum(m), wave_sum(px), wave_sum(py), wave_sum(pz)};
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) for (int k = 0; k < 4; k++) sh[k][w] = v4[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double s = 0;
+        for (int k = 0; k < kThreads / 64; k++) s += sh[threadIdx.x][k];
+        partial[blockIdx.x * 4 + threadIdx.x] = s;
+    }
+}
+
+// Residual accumulators: sum |u - u_snap|^2 and |u|^2 over fluid nodes,
+// then u_snap := u.
+__global__ __launch_bounds__(kThreads) void k_residual(const float *macros, float *snap, const uint8_t *flags,
+                                                       uint32_t N, int ncomp, double *partial) {
+    double num = 0, den = 0;
+    for (uint32_t id = blockIdx.x * kThreads + threadIdx.x; id < N; id += gridDim.x * kThreads) {
+        if (flags[id] & F_SOLID) continue;
+        for (int c = 0; c < ncomp; c++) {
+            double u = macros[(size_t)(1 + c) * N + id];
+            double s = snap[(size_t)c * N + id];
+            num += (u - s) * (u - s);
+            den += u * u;
+            snap[(size_t)c * N + id] = (float)u;
+        }
+    }
+    __shared__ double sh[2][kThreads / 64];
+    double a = wave_sum(num), b = wave_sum(den);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) { sh[0][w] = a; sh[1][w] = b; }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double s = 0;
+        for (int k = 0; k < kThreads / 64; k++) s += sh[threadIdx.x][k];
+        partial[blockIdx.x * 2 + threadIdx.x] = s;
+    }
+}
+
+// Final fixed-order sum of per-block partials (one wavefront).
+__global__ void k_final_sum(const double *partial, int nblk, int width, double *out) {
+    int c = threadIdx.x;
+    if (c >= width) return;
+    double s = 0;
+    for (int b = 0; b < nblk; b++) s += partial[b * width + c];
+    out[c] = s;
+}
+
+constexpr int kReduceBlocks = 1024;
+
+}  // namespace
+
+// ===========================================================================
+// Handle.
+// ===========================================================================
+struct lb_ctx {
+    lb_config cfg{};
+    int Q = 19;
+    uint32_t N = 0, P = 0;
+    size_t esz = 4;
+    hipStream_t stream = nullptr;
+    void *buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t *flags = nullptr;
+    float *macros = nullptr;
+    float *snap = nullptr;
+    void *psi = nullptr;
+    void *send = nullptr, *recv = nullptr;
+    double *partial = nullptr, *dout = nullptr;
+    float *tmp_fields = nullptr;
+    bool macros_valid = false, snap_valid = false;
+    long long steps = 0;
+    std::vector<uint8_t> solid;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_steps = 0;
+    int bx_shift = 6, by = 4;
+    int xcd = 0;
+    int corr_lo[3] = {0, 0, 0}, corr_hi[3] = {0, 0, 0};
+
+    ~lb_ctx() {
+        for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
+        void *ps[] = {buf[0], buf[1], flags, macros, snap, psi, send, recv, partial, dout, tmp_fields};
+        for (void *p : ps) if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void drop_graphs() {
+        for (auto &g : graph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    }
+};
+
+namespace {
+
+int ilog2(int v) { int s = 0; while ((1 << s) < v) s++; return s; }
+
+template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
+    StepArgs<S> a{};
+    a.A = (const typename S::raw *)c->buf[src];
+    a.B = (typename S::raw *)c->buf[1 - src];
+    a.flags = c->flags;
+    a.macros = c->macros;
+    a.psi = (typename S::T *)c->psi;
+    a.send = (typename S::raw *)c->send;
+    a.recv = (const typename S::raw *)c->recv;
+    a.N = c->N;
+    a.nxy = (uint32_t)c->cfg.nx * (uint32_t)c->cfg.ny;
+    a.P = c->P;
+    a.nx = c->cfg.nx; a.ny = c->cfg.ny; a.nz = c->cfg.nz;
+    a.ti = c->cfg.tau_inv;
+    for (int k = 0; k < 6; k++) {
+        a.bc[k] = c->cfg.bc[k];
+        a.bcrho[k] = c->cfg.bc_rho[k];
+        for (int d = 0; d < 3; d++) a.bcu[k][d] = c->cfg.bc_u[k][d];
+    }
+    for (int d = 0; d < 3; d++) { a.corr_lo[d] = c->corr_lo[d]; a.corr_hi[d] = c->corr_hi[d]; a.force[d] = c->cfg.force[d]; }
+    a.scG = c->cfg.sc_G; a.scrho0 = c->cfg.sc_rho0; a.scpsi_solid = c->cfg.sc_psi_solid;
+    a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he;
+    a.halo = c->cfg.halo_z && c->Q == 19;
+    a.zh_ux = c->cfg.zh_ux_in; a.zh_rho = c->cfg.zh_rho_out;
+    a.bx_shift = c->bx_shift; a.by = c->by;
+    a.ntx = (c->cfg.nx + (1 << c->bx_shift) - 1) >> c->bx_shift;
+    a.nty = (c->cfg.ny + c->by - 1) / c->by;
+    a.nblocks = a.ntx * a.nty * c->cfg.nz;
+    a.xcd = c->xcd;
+    return a;
+}
+
+template <int Q, class S>
+int launch_step_t(lb_ctx *c, int src, bool macros) {
+    StepArgs<S> a = make_args<S>(c, src);
+    const bool sc = c->cfg.sc_G != 0.0f;
+    size_t lds = 0;
+    if (sc) {
+        lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
+        hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
+    }
+    if (sc) {
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, a);
+        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, a);
+    } else {
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
+    }
+    return (int)hipGetLastError();
+}
+
+template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros) {
+    switch (c->cfg.storage) {
+    case LB_STORE_F32: return launch_step_t<Q, StF32>(c, src, macros);
+    case LB_STORE_F16: return launch_step_t<Q, StF16>(c, src, macros);
+    case LB_STORE_BF16: return launch_step_t<Q, StBF16>(c, src, macros);
+    default: return launch_step_t<Q, StF64>(c, src, macros);
+    }
+}
+
+// One time step from buffer src: sweep (+ halo copy).
+int launch_step(lb_ctx *c, int src, bool macros) {
+    int rc = c->Q == 9 ? launch_step_q<9>(c, src, macros) : launch_step_q<19>(c, src, macros);
+    if (rc) return rc;
+    if (c->cfg.halo_z && c->Q == 19) {
+        // copy: with one subdomain on the ring the -z and +z neighbour is this
+        // slab itself, so recv_hi <- send_lo and recv_lo <- send_hi, one copy.
+        rc = (int)hipMemcpyAsync(c->recv, c->send, (size_t)10 * c->P * c->esz, hipMemcpyDeviceToDevice, c->stream);
+    }
+    return rc;
+}
+
+template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
+    hipLaunchKernelGGL((k_reduce<Q, S>), dim3(kReduceBlocks), dim3(kThreads), 0, c->stream,
+                       (const typename S::raw *)c->buf[c->cur], c->flags, c->N, c->partial);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, c->stream, c->partial, kReduceBlocks, 4, c->dout);
+    return (int)hipGetLastError();
+}
+template <int Q> int launch_reduce_q(lb_ctx *c) {
+    switch (c->cfg.storage) {
+    case LB_STORE_F32: return launch_reduce_t<Q, StF32>(c);
+    case LB_STORE_F16: return launch_reduce_t<Q, StF16>(c);
+    case LB_STORE_BF16: return launch_reduce_t<Q, StBF16>(c);
+    default: return launch_reduce_t<Q, StF64>(c);
+    }
+}
+
+template <int Q, class S> int launch_init_t(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
+    hipLaunchKernelGGL((k_init_eq<Q, S>), dim3((c->N + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
+                       (typename S::raw *)c->buf[0], (typename S::raw *)c->buf[1], c->N, r, ux, uy, uz);
+    return (int)hipGetLastError();
+}
+template <int Q> int launch_init_q(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
+    switch (c->cfg.storage) {
+    case LB_STORE_F32: return launch_init_t<Q, StF32>(c, r, ux, uy, uz);
+    case LB_STORE_F16: return launch_init_t<Q, StF16>(c, r, ux, uy, uz);
+    case LB_STORE_BF16: return launch_init_t<Q, StBF16>(c, r, ux, uy, uz);
+    default: return launch_init_t<Q, StF64>(c, r, ux, uy, uz);
+    }
+}
+
+int upload_flags(lb_ctx *c) {
+    std::vector<uint8_t> fl;
+    build_flags(c->cfg, c->solid, fl);
+    int rc = (int)hipMemcpyAsync(c->flags, fl.data(), c->N, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+int run_graph_or_eager(lb_ctx *c, int nsteps) {
+    // nsteps plain steps (no macros), chunks of graph_steps through a graph.
+    const int chunk = c->graph_steps;
+    while (nsteps >= chunk && chunk > 0) {
+        hipGraphExec_t &ge = c->graph[c->cur];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+            if (rc) return rc;
+            int src = c->cur;
+            for (int s = 0; s < chunk; s++) { launch_step(c, src, false); src = 1 - src; }
+            rc = (int)hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (rc) { ge = nullptr; return rc; }
+        }
+        int rc = (int)hipGraphLaunch(ge, c->stream);
+        if (rc) return rc;
+        if (chunk & 1) c->cur = 1 - c->cur;
+        nsteps -= chunk;
+        c->steps += chunk;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        int rc = launch_step(c, c->cur, false);
+        if (rc) return rc;
+        c->cur = 1 - c->cur;
+        c->steps++;
+    }
+    return 0;
+}
+
+// Launch-shape autotuner: times each candidate (block shape x XCD mapping)
+// over a few launches on the real lattice and keeps the fastest.
+int autotune(lb_ctx *c) {
+    struct Cand { int bx, by, xcd; };
+    std::vector<Cand> cands;
+    const int bxs[4] = {32, 64, 128, 256};
+    for (int bx : bxs) {
+        if (bx > 2 * c->cfg.nx && bx != 32) continue;
+        for (int xcd = 0; xcd < 2; xcd++) {
+            if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
+            cands.push_back({bx, kThreads / bx, xcd});
+        }
+    }
+    if (cands.empty()) cands.push_back({64, 4, 0});
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
+    float best = 1e30f;
+    Cand bestc = cands[0];
+    const int reps = 6;
+    for (const Cand &k : cands) {
+        c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd;
+        launch_step(c, 0, false);   // warm
+        (void)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < reps; r++) launch_step(c, r & 1, false);
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) { best = ms; bestc = k; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd;
+    return (int)hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
+    if (!cfg || !out) return LB_ERR_INVALID;
+    *out = nullptr;
+    if (lb_device_count() <= 0) return LB_ERR_NO_DEVICE;
+    if (cfg->lattice != 9 && cfg->lattice != 19) return LB_ERR_INVALID;
+    if (cfg->nx < 2 || cfg->ny < 1 || cfg->nz < 1) return LB_ERR_INVALID;
+    if (cfg->lattice == 9 && cfg->nz != 1) return LB_ERR_INVALID;
+    if (cfg->storage < 0 || cfg->storage > 3) return LB_ERR_INVALID;
+    for (int k = 0; k < 6; k++) if (cfg->bc[k] < 0 || cfg->bc[k] > 4) return LB_ERR_INVALID;
+    uint64_t N64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
+    if (N64 * (uint64_t)cfg->lattice >= (1ull << 31)) return LB_ERR_UNSUPPORTED;   // 32-bit node indices
+    if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
+        return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
+    if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
+    lb_ctx *c = new (std::nothrow) lb_ctx();
+    if (!c) return LB_ERR_INVALID;
+    c->cfg = *cfg;
+    c->Q = cfg->lattice;
+    c->N = (uint32_t)N64;
+    c->P = (uint32_t)cfg->nx * cfg->ny;
+    c->esz = cfg->storage == LB_STORE_F32 ? 4 : (cfg->storage == LB_STORE_F64 ? 8 : 2);
+    c->graph_steps = cfg->graph_chunk > 0 ? cfg->graph_chunk : 64;
+    c->solid.assign(c->N, 0);
+    face_corrections(*cfg, c->corr_lo, c->corr_hi);
+    int rc = (int)hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    const size_t pop_bytes = (size_t)c->Q * c->N * c->esz;
+    if (!rc) rc = (int)hipMalloc(&c->buf[0], pop_bytes);
+    if (!rc) rc = (int)hipMalloc(&c->buf[1], pop_bytes);
+    if (!rc) rc = (int)hipMalloc((void **)&c->flags, c->N);
+    if (!rc) rc = (int)hipMalloc((void **)&c->macros, (size_t)4 * c->N * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&c->snap, (size_t)3 * c->N * sizeof(float));
+    if (!rc) rc = (int)hipMalloc((void **)&c->partial, (size_t)kReduceBlocks * 4 * sizeof(double));
+    if (!rc) rc = (int)hipMalloc((void **)&c->dout, 8 * sizeof(double));
+    if (!rc) rc = (int)hipMalloc((void **)&c->tmp_fields, (size_t)4 * c->N * sizeof(float));
+    if (!rc && cfg->sc_G != 0.0f) rc = (int)hipMalloc(&c->psi, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4));
+    if (!rc && cfg->halo_z && c->Q == 19) {
+        rc = (int)hipMalloc(&c->send, (size_t)10 * c->P * c->esz);
+        if (!rc) rc = (int)hipMalloc(&c->recv, (size_t)10 * c->P * c->esz);
+        if (!rc) rc = (int)hipMemsetAsync(c->send, 0, (size_t)10 * c->P * c->esz, c->stream);
+        if (!rc) rc = (int)hipMemsetAsync(c->recv, 0, (size_t)10 * c->P * c->esz, c->stream);
+    }
+    if (!rc) rc = (int)hipMemsetAsync(c->buf[0], 0, pop_bytes, c->stream);
+    if (!rc) rc = (int)hipMemsetAsync(c->buf[1], 0, pop_bytes, c->stream);
+    if (!rc) rc = (int)hipMemsetAsync(c->macros, 0, (size_t)4 * c->N * sizeof(float), c->stream);
+    if (!rc && c->psi) rc = (int)hipMemsetAsync(c->psi, 0, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4), c->stream);
+    if (!rc) rc = upload_flags(c);
+    if (!rc) {
+        if (cfg->block_x > 0) {
+            c->bx_shift = ilog2(cfg->block_x);
+            c->by = std::max(1, kThreads / cfg->block_x);
+            c->xcd = cfg->xcd_remap > 0 ? 1 : 0;
+        } else {
+            rc = autotune(c);
+        }
+    }
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    if (rc) { delete c; return rc; }
+    c->cfg.block_x = 1 << c->bx_shift;
+    c->cfg.block_y = c->by;
+    c->cfg.xcd_remap = c->xcd;
+    c->cfg.graph_chunk = c->graph_steps;
+    *out = c;
+    return 0;
+}
+
+extern "C" int lb_destroy(lb_ctx *c) {
+    if (!c) return LB_ERR_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int lb_get_config(lb_ctx *c, lb_config *out) {
+    if (!c || !out) return LB_ERR_INVALID;
+    *out = c->cfg;
+    return 0;
+}
+
+extern "C" int lb_set_solid(lb_ctx *c, const uint8_t *solid) {
+    if (!c || !solid) return LB_ERR_INVALID;
+    for (uint32_t k = 0; k < c->N; k++) c->solid[k] = solid[k] ? 1 : 0;
+    c->drop_graphs();
+    return upload_flags(c);
+}
+
+extern "C" int lb_init_equilibrium_device(lb_ctx *c, const float *r, const float *ux, const float *uy,
+                                          const float *uz) {
+    if (!c || !r || !ux || !uy) return LB_ERR_INVALID;
+    int rc = c->Q == 9 ? launch_init_q<9>(c, r, ux, uy, uz) : launch_init_q<19>(c, r, ux, uy, uz);
+    if (rc) return rc;
+    c->cur = 0;
+    c->steps = 0;
+    c->macros_valid = false;
+    c->snap_valid = false;
+    if (c->send) {
+        rc = (int)hipMemsetAsync(c->send, 0, (size_t)10 * c->P * c->esz, c->stream);
+        if (rc) return rc;
+    }
+    // A fresh halo: the planes the first sweep pulls across z come from the
+    // initial state, so pack them now (recv_hi <- plane 0 -z pops, recv_lo <-
+    // plane nz-1 +z pops), as the copy after a sweep would.
+    if (c->recv) {
+        const int minus[5] = {6, 10, 14, 16, 18}, plus[5] = {5, 9, 13, 15, 17};
+        const size_t pb = (size_t)c->P * c->esz;
+        for (int k = 0; k < 5 && !rc; k++) {
+            const char *A = (const char *)c->buf[0];
+            rc = (int)hipMemcpyAsync((char *)c->recv + k * pb, A + ((size_t)minus[k] * c->N) * c->esz, pb,
+                                     hipMemcpyDeviceToDevice, c->stream);
+            if (!rc)
+                rc = (int)hipMemcpyAsync((char *)c->recv + (5 + k) * pb,
+                                         A + ((size_t)plus[k] * c->N + (size_t)(c->cfg.nz - 1) * c->P) * c->esz, pb,
+                                         hipMemcpyDeviceToDevice, c->stream);
+        }
+    }
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_init_equilibrium(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
+    if (!c || !r || !ux || !uy) return LB_ERR_INVALID;
+    const size_t nb = (size_t)c->N * sizeof(float);
+    float *t = c->tmp_fields;
+    int rc = (int)hipMemcpyAsync(t, r, nb, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(t + c->N, ux, nb, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(t + 2 * (size_t)c->N, uy, nb, hipMemcpyHostToDevice, c->stream);
+    if (!rc && uz) rc = (int)hipMemcpyAsync(t + 3 * (size_t)c->N, uz, nb, hipMemcpyHostToDevice, c->stream);
+    if (rc) return rc;
+    return lb_init_equilibrium_device(c, t, t + c->N, t + 2 * (size_t)c->N, uz ? t + 3 * (size_t)c->N : nullptr);
+}
+
+extern "C" int lb_init_uniform(lb_ctx *c, float rho, float ux, float uy, float uz) {
+    if (!c) return LB_ERR_INVALID;
+    std::vector<float> h((size_t)4 * c->N);
+    std::fill(h.begin(), h.begin() + c->N, rho);
+    std::fill(h.begin() + c->N, h.begin() + 2 * (size_t)c->N, ux);
+    std::fill(h.begin() + 2 * (size_t)c->N, h.begin() + 3 * (size_t)c->N, uy);
+    std::fill(h.begin() + 3 * (size_t)c->N, h.end(), uz);
+    return lb_init_equilibrium(c, h.data(), h.data() + c->N, h.data() + 2 * (size_t)c->N, h.data() + 3 * (size_t)c->N);
+}
+
+extern "C" int lb_set_populations(lb_ctx *c, const void *host) {
+    if (!c || !host) return LB_ERR_INVALID;
+    size_t bytes = (size_t)c->Q * c->N * c->esz;
+    int rc = (int)hipMemcpyAsync(c->buf[0], host, bytes, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(c->buf[1], host, bytes, hipMemcpyHostToDevice, c->stream);
+    c->cur = 0;
+    c->steps = 0;
+    c->macros_valid = false;
+    c->snap_valid = false;
+    if (!rc && c->recv) {
+        const int minus[5] = {6, 10, 14, 16, 18}, plus[5] = {5, 9, 13, 15, 17};
+        const size_t pb = (size_t)c->P * c->esz;
+        for (int k = 0; k < 5 && !rc; k++) {
+            const char *A = (const char *)c->buf[0];
+            rc = (int)hipMemcpyAsync((char *)c->recv + k * pb, A + ((size_t)minus[k] * c->N) * c->esz, pb,
+                                     hipMemcpyDeviceToDevice, c->stream);
+            if (!rc)
+                rc = (int)hipMemcpyAsync((char *)c->recv + (5 + k) * pb,
+                                         A + ((size_t)plus[k] * c->N + (size_t)(c->cfg.nz - 1) * c->P) * c->esz, pb,
+                                         hipMemcpyDeviceToDevice, c->stream);
+        }
+    }
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_get_populations(lb_ctx *c, void *host) {
+    if (!c || !host) return LB_ERR_INVALID;
+    int rc = (int)hipMemcpyAsync(host, c->buf[c->cur], (size_t)c->Q * c->N * c->esz, hipMemcpyDeviceToHost, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_step(lb_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return LB_ERR_INVALID;
+    if (nsteps == 0) return 0;
+    int rc = run_graph_or_eager(c, nsteps - 1);
+    if (rc) return rc;
+    rc = launch_step(c, c->cur, true);
+    if (rc) return rc;
+    c->cur = 1 - c->cur;
+    c->steps++;
+    c->macros_valid = true;
+    return 0;
+}
+
+extern "C" int lb_sync(lb_ctx *c) {
+    if (!c) return LB_ERR_INVALID;
+    return (int)hipStreamSynchronize(c->stream);
+}
+
+extern "C" int lb_get_macros(lb_ctx *c, float *rho, float *ux, float *uy, float *uz) {
+    if (!c) return LB_ERR_INVALID;
+    if (!c->macros_valid) return LB_ERR_NOT_READY;
+    float *dst[4] = {rho, ux, uy, uz};
+    int rc = 0;
+    for (int k = 0; k < 4 && !rc; k++)
+        if (dst[k]) rc = (int)hipMemcpyAsync(dst[k], c->macros + (size_t)k * c->N, (size_t)c->N * 4, hipMemcpyDeviceToHost, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_macros_device(lb_ctx *c, float **out) {
+    if (!c || !out) return LB_ERR_INVALID;
+    *out = c->macros;
+    return 0;
+}
+
+extern "C" int lb_reduce(lb_ctx *c, double out[4]) {
+    if (!c || !out) return LB_ERR_INVALID;
+    int rc = c->Q == 9 ? launch_reduce_q<9>(c) : launch_reduce_q<19>(c);
+    if (!rc) rc = (int)hipMemcpyAsync(out, c->dout, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_residual(lb_ctx *c, double *out) {
+    if (!c || !out) return LB_ERR_INVALID;
+    if (!c->macros_valid) return LB_ERR_NOT_READY;
+    const int ncomp = c->Q == 9 ? 2 : 3;
+    hipLaunchKernelGGL(k_residual, dim3(kReduceBlocks), dim3(kThreads), 0, c->stream, c->macros, c->snap, c->flags,
+                       c->N, ncomp, c->partial);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, c->stream, c->partial, kReduceBlocks, 2, c->dout);
+    double h[2] = {0, 0};
+    int rc = (int)hipGetLastError();
+    if (!rc) rc = (int)hipMemcpyAsync(h, c->dout, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    if (rc) return rc;
+    *out = !c->snap_valid ? 1.0 : (h[1] > 0 ? std::sqrt(h[0] / h[1]) : 0.0);
+    c->snap_valid = true;
+    return 0;
+}
+
+extern "C" int lb_read_value(lb_ctx *c, int x, int y, int z, float out[4]) {
+    if (!c || !out) return LB_ERR_INVALID;
+    if (x < 0 || y < 0 || z < 0 || x >= c->cfg.nx || y >= c->cfg.ny || z >= c->cfg.nz) return LB_ERR_INVALID;
+    if (!c->macros_valid) return LB_ERR_NOT_READY;
+    size_t id = x + (size_t)c->cfg.nx * (y + (size_t)c->cfg.ny * z);
+    const int comp[4] = {1, 2, 3, 0};   // ux, uy, uz, rho (read_value.cu order)
+    int rc = 0;
+    for (int k = 0; k < 4 && !rc; k++)
+        rc = (int)hipMemcpyAsync(out + k, c->macros + (size_t)comp[k] * c->N + id, 4, hipMemcpyDeviceToHost, c->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    return rc;
+}
+
+extern "C" int lb_steps_done(lb_ctx *c, long long *out) {
+    if (!c || !out) return LB_ERR_INVALID;
+    *out = c->steps;
+    return 0;
+}
+
+extern "C" int lb_time_step_kernel(lb_ctx *c, int n, float *ms) {
+    if (!c || !ms || n <= 0) return LB_ERR_INVALID;
+    // Only the sweep kernels (the halo copy is excluded), back to back
+    // through a captured graph so launch gaps do not count as kernel time.
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (rc) return rc;
+    int src = c->cur;
+    for (int s = 0; s < n; s++) {
+        if (c->Q == 9) launch_step_q<9>(c, src, false);
+        else launch_step_q<19>(c, src, false);
+        src = 1 - src;
+    }
+    rc = (int)hipStreamEndCapture(c->stream, &g);
+    if (rc) return rc;
+    rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    rc = (int)hipGraphLaunch(ge, c->stream);   // warm
+    if (!rc) rc = (int)hipEventRecord(e0, c->stream);
+    if (!rc) rc = (int)hipGraphLaunch(ge, c->stream);
+    if (!rc) rc = (int)hipEventRecord(e1, c->stream);
+    if (!rc) rc = (int)hipEventSynchronize(e1);
+    float t = 0;
+    if (!rc) rc = (int)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipGraphExecDestroy(ge);
+    // 2n sweeps ran: the state advanced; keep the buffer parity consistent.
+    if ((2 * n) & 1) c->cur = 1 - c->cur;
+    c->steps += 2 * (long long)n;
+    c->macros_valid = false;
+    *ms = t / (float)n;
+    return rc;
+}
+
+extern "C" int lb_halo_buffers(lb_ctx *c, void **send_lo, void **send_hi, void **recv_lo, void **recv_hi) {
+    if (!c) return LB_ERR_INVALID;
+    if (!c->send) return LB_ERR_UNSUPPORTED;
+    const size_t pb5 = (size_t)5 * c->P * c->esz;
+    if (send_lo) *send_lo = c->send;
+    if (send_hi) *send_hi = (char *)c->send + pb5;
+    if (recv_hi) *recv_hi = c->recv;
+    if (recv_lo) *recv_lo = (char *)c->recv + pb5;
+    return 0;
+}

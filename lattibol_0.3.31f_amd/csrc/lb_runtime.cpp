@@ -1,0 +1,43 @@
+// lb_runtime.cpp -- error state and small host utilities of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+
+#include "../../include/lattibol_amd.h"
+
+static std::atomic<int> g_last_error{0};
+
+extern "C" void lb_set_error(int code) {
+    int expected = 0;
+    if (code != 0) g_last_error.compare_exchange_strong(expected, code);
+}
+
+int lb_hip_err(hipError_t e) {
+    if (e != hipSuccess) {
+        lb_set_error((int)e);
+        return (int)e;
+    }
+    return 0;
+}
+
+extern "C" int lb_last_error(void) { return g_last_error.exchange(0); }
+
+extern "C" const char *lb_error_string(int code) {
+    switch (code) {
+    case LB_OK: return "success";
+    case LB_ERR_INVALID: return "invalid argument";
+    case LB_ERR_SHAPE: return "grid*block does not match the lattice shape";
+    case LB_ERR_UNSUPPORTED: return "unsupported configuration";
+    case LB_ERR_NO_DEVICE: return "no usable GPU device";
+    case LB_ERR_NOT_READY: return "requested state has not been computed";
+    default: break;
+    }
+    if (code > 0) return hipGetErrorString((hipError_t)code);
+    return "unknown error";
+}
+
+extern "C" int lb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
