@@ -32,6 +32,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
+    ap.add_argument("--workload", default="sphere",
+                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart"],
+                    help="sphere = D3Q19 channel past a sphere (the headline); the others print their own line")
+    ap.add_argument("--train-samples", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--heldout", type=int, default=256)
+    ap.add_argument("--max-steady-steps", type=int, default=300000)
     return ap.parse_args()
 
 
@@ -71,12 +78,142 @@ def cpu_baseline(cfg, solid, init, steps_hint):
                       "same arithmetic as the GPU path), %.1f s" % (steps, cfg["nx"], cfg["ny"], cfg["nz"], dt)}
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def common(metric, value, unit, args, steps, ms, dtype, config, higher=True, **extra):
+    d = {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+         "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+         "data": "synthetic", "config": config}
+    d.update(extra)
+    return d
+
+
+def bench_cavity(args):
+    """C1: D2Q9 lid-driven cavity 256^2, tau 0.6, lid u=0.1, 10,000 steps (launch-bound parity config)."""
+    import lattibol_amd as lb
+    from lattibol_amd import scenes
+    cfg, solid, init = scenes.cavity()
+    sol = lb.Solver(cfg, solid)
+    sol.init_equilibrium(init["rho"], init["ux"], init["uy"])
+    sol.step(args.warmup)
+    sol.sync()
+    steps = args.steps
+    t0 = time.perf_counter()
+    sol.step(steps)
+    sol.sync()
+    dt = time.perf_counter() - t0
+    n = cfg["nx"] * cfg["ny"]
+    ms = sol.time_step_kernel(200)
+    res = sol.reduce()
+    return common("D2Q9 MLUPS (lid-driven cavity)", round(n * steps / dt / 1e6, 1), "MLUPS", args, steps,
+                  round(dt / steps * 1e3, 5), "fp32",
+                  {"workload": "D2Q9 BGK lid-driven cavity 256x256, tau=0.6, lid u=0.1, half-way bounce-back"},
+                  roofline={"bound": "hbm", "achieved": round(72 * n / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
+                            "unit": "GB/s", "frac": round(72 * n / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                            "traffic": None, "kernel_ms": round(ms, 5)},
+                  mass=float(res[0]))
+
+
+def bench_spinodal(args):
+    """C3: D3Q19 Shan-Chen 192^3; FP32 / FP16 / BF16 storage against the FP64 run."""
+    import numpy as np
+    import lattibol_amd as lb
+    from lattibol_amd import scenes
+    steps = args.steps
+    out = {}
+    fields = {}
+    for st in ("f64", "f32", "bf16", "f16"):
+        cfg, solid, init = scenes.spinodal(storage=st)
+        sol = lb.Solver(cfg, solid)
+        sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+        m0 = sol.reduce()[0]
+        sol.sync()
+        t0 = time.perf_counter()
+        sol.step(steps)
+        sol.sync()
+        dt = time.perf_counter() - t0
+        m1 = sol.reduce()[0]
+        rho = sol.macros()["rho"]
+        fields[st] = rho.astype(np.float64)
+        n = rho.size
+        out[st] = {"mlups": round(n * steps / dt / 1e6, 1), "seconds": round(dt, 3),
+                   "mass_drift_rel": float((m1 - m0) / m0), "rho_min": float(rho.min()), "rho_max": float(rho.max())}
+        log("%s: %.1f MLUPS, drift %.2e" % (st, out[st]["mlups"], out[st]["mass_drift_rel"]))
+        sol.close()
+    ref = fields["f64"]
+    for st in ("f32", "bf16", "f16"):
+        d = fields[st] - ref
+        out[st]["rho_err_l2_rel_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(ref))
+        out[st]["rho_err_max_vs_fp64"] = float(np.abs(d).max())
+    return common("D3Q19 Shan-Chen MLUPS (FP32 storage) + error of FP32/FP16/BF16 vs FP64", out["f32"]["mlups"],
+                  "MLUPS", args, steps, round(out["f32"]["seconds"] / steps * 1e3, 5), "fp32",
+                  {"workload": "D3Q19 Shan-Chen spinodal decomposition 192^3 periodic, psi=1-exp(-rho), G=-5, "
+                               "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out)
+
+
+def bench_ensemble(args):
+    """C4: 512 ragged D2Q9 channels in one launch per step, to residual < 1e-6."""
+    import numpy as np
+    from lattibol_amd import scenes, Batch
+    specs = scenes.ensemble(512, seed=1234)
+    b = Batch(specs)
+    n = b.total_nodes
+    b.step(args.warmup)
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    dt = time.perf_counter() - t0
+    mlups = n * args.steps / dt / 1e6
+    b.init_rest()
+    t0 = time.perf_counter()
+    steps, res = b.run_to_steady(tol=1e-6, k=100, max_steps=args.max_steady_steps)
+    wall = time.perf_counter() - t0
+    conv = res < 1e-6
+    upd = float(sum(int(steps[i]) * specs[i]["nx"] * specs[i]["ny"] for i in range(len(specs))))
+    return common("D2Q9 batched-ensemble MLUPS", round(mlups, 1), "MLUPS", args, args.steps,
+                  round(dt / args.steps * 1e3, 5), "fp32",
+                  {"workload": "512 D2Q9 channels, sizes {64x64,128x64,128x128,256x128}, 1-4 circles, "
+                               "u_in in [0.02,0.08], tau in [0.52,0.9], one launch per step", "nodes": n},
+                  steady={"converged": int(conv.sum()), "of": len(specs), "wall_s": round(wall, 2),
+                          "max_steps": args.max_steady_steps, "updates": upd,
+                          "effective_mlups": round(upd / wall / 1e6, 1),
+                          "median_steps_converged": float(np.median(steps[conv])) if conv.any() else None})
+
+
+def bench_warmstart(args):
+    """C5: U-Net warm start vs plain LBM from rest, time to residual < 1e-6."""
+    from lattibol_amd import warmstart
+    from lattibol_amd.unet import UNet
+    t0 = time.perf_counter()
+    data = warmstart.dataset(args.train_samples, seed=4242, max_steps=args.max_steady_steps)
+    gen = time.perf_counter() - t0
+    log("training set: %d/%d converged in %.1f s" % (data["converged"].sum(), args.train_samples, gen))
+    net = UNet(levels=4, base=32, in_ch=3, out_ch=3, height=128, width=128, max_batch=16, seed=1)
+    hist = warmstart.train(net, data, epochs=args.epochs, batch=16, lr=1e-3, log=log)
+    ev = warmstart.evaluate(net, n=args.heldout, seed=777, max_steps=args.max_steady_steps)
+    ev.pop("per_lattice")
+    return common("time-to-steady-state, warm-start / plain LBM (wall-time ratio)", round(ev["wall_ratio"], 4),
+                  "ratio", args, 0, None, "fp32 LBM, bf16 U-Net",
+                  {"workload": "D2Q9 channels 128x128 (1-4 circles, u_in in [0.02,0.08], tau in [0.52,0.9]); "
+                               "4-level 32-channel U-Net trained on the batched solver's steady fields; "
+                               "%d held-out geometries" % args.heldout},
+                  higher=False, train={"samples": int(data["converged"].sum()), "generation_s": round(gen, 1),
+                                       "epochs": args.epochs, "train_s": round(hist["seconds"], 1),
+                                       "final_loss": hist["loss"][-1]}, evaluation=ev)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; --gpus %d is not supported "
               "(the z-slab halo exchange is exercised on one device)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.workload != "sphere":
+        fn = {"cavity": bench_cavity, "spinodal": bench_spinodal, "ensemble": bench_ensemble,
+              "warmstart": bench_warmstart}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
     import numpy as np
     import lattibol_amd as lb
     from lattibol_amd import scenes

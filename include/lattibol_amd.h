@@ -25,6 +25,7 @@
 #ifndef LATTIBOL_AMD_H
 #define LATTIBOL_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 #include <stdbool.h>
 
@@ -90,6 +91,12 @@ enum {
 int lb_last_error(void);                 /* returns and clears the sticky error */
 const char *lb_error_string(int code);
 int lb_device_count(void);
+/* Device-memory plumbing for hosts without a GPU framework. */
+int lb_device_alloc(size_t bytes, void **out);
+int lb_device_free(void *p);
+int lb_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int lb_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int lb_device_synchronize(void);
 
 /* ---------------------------------------------------------------------- */
 /* 1. Reference entry points (reference semantics, legacy default stream)  */
@@ -262,6 +269,45 @@ int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *s
 int lb_batch_get_macros(lb_batch *b, int i, float *rho, float *ux, float *uy);
 int lb_batch_get_populations(lb_batch *b, int i, float *host);
 int64_t lb_batch_total_nodes(lb_batch *b);
+
+/* ---------------------------------------------------------------------- */
+/* 4. U-Net surrogate (steady (ux, uy, rho) from mask, u_in, tau)           */
+/*    No counterpart in the reference: the learned warm start.              */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int levels;             /* resolution levels (4) */
+    int base;               /* channels at level 0 (32); doubles per level */
+    int in_ch, out_ch;      /* 3 -> 3 */
+    int height, width;      /* 128 x 128 */
+    int max_batch;
+} lb_unet_desc;
+typedef struct lb_unet lb_unet;
+/* He-uniform weights from SplitMix64(seed).  Parameter vector layout, in
+ * forward conv order: W[co][tap][ci] (ci padded to 16 for the first conv),
+ * then bias[co]; finally the 1x1 head W[out][base], bias[out]. */
+int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet **out);
+int lb_unet_destroy(lb_unet *u);
+int64_t lb_unet_param_count(lb_unet *u);
+int lb_unet_get_params(lb_unet *u, float *host_params, float *host_grads);
+int lb_unet_set_params(lb_unet *u, const float *host_params);
+/* x: device (B, H, W, in_ch) FP32; y: device (B, H, W, out_ch) FP32 (may be NULL). */
+int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y);
+/* Masked MSE of the last forward vs d_t (B,H,W,out_ch) with weights d_mask
+ * (B,H,W); fills every parameter gradient; *loss = mean squared error. */
+int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mask, int B, double *loss);
+int lb_unet_adam(lb_unet *u, float lr, float beta1, float beta2, float eps);
+int lb_unet_sync(lb_unet *u);
+/* Warm-start fields from a standardised prediction d_y (B,H,W,3): per sample
+ * b the planes [rho | ux | uy] at d_fields + 3*H*W*b, with
+ * ux = u_in (y0 s0 + m0), uy = u_in (y1 s1 + m1), rho = 1 + u_in (y2 s2 + m2),
+ * stats6 = {m0, s0, m1, s1, m2, s2} (host), d_uin = u_in per sample (device). */
+int lb_unet_to_fields(lb_unet *u, const float *d_y, int B, const float *d_uin, const float *stats6,
+                      float *d_fields);
+/* One 3x3 convolution (pad 1) on device NHWC tensors, BF16 matrix cores:
+ * mode 0: BF16 in/out, bias + ReLU; mode 1: FP32 in/out, no bias.
+ * W is BF16 [Co][9][Ci]; Ci % 16 == 0, Co % 32 == 0. */
+int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B, int H,
+                    int W, int Ci, int Co, lb_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -41,3 +41,22 @@ extern "C" int lb_device_count(void) {
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
 }
+
+// Device-memory plumbing for hosts without a GPU framework (Python via ctypes).
+extern "C" int lb_device_alloc(size_t bytes, void **out) {
+    if (!out) return LB_ERR_INVALID;
+    *out = nullptr;
+    return (int)hipMalloc(out, bytes ? bytes : 1);
+}
+
+extern "C" int lb_device_free(void *p) { return p ? (int)hipFree(p) : 0; }
+
+extern "C" int lb_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    return (int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+}
+
+extern "C" int lb_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    return (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+}
+
+extern "C" int lb_device_synchronize(void) { return (int)hipDeviceSynchronize(); }

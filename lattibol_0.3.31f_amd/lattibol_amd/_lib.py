@@ -101,6 +101,11 @@ _PROTOS = {
     "lb_last_error": (_i, []),
     "lb_error_string": (C.c_char_p, [_i]),
     "lb_device_count": (_i, []),
+    "lb_device_alloc": (_i, [C.c_size_t, C.POINTER(_vp)]),
+    "lb_device_free": (_i, [_vp]),
+    "lb_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),
+    "lb_memcpy_d2h": (_i, [_vp, _vp, C.c_size_t]),
+    "lb_device_synchronize": (_i, []),
     # reference launchers
     "streaming_d2q9": (None, [C.POINTER(DataD2Q9), _d3, _d3]),
     "streaming_d3q19": (None, [C.POINTER(DataD3Q19), _d3, _d3]),
@@ -152,6 +157,18 @@ _PROTOS = {
     "lb_batch_get_macros": (_i, [_vp, _i, _vp, _vp, _vp]),
     "lb_batch_get_populations": (_i, [_vp, _i, _vp]),
     "lb_batch_total_nodes": (C.c_int64, [_vp]),
+    # U-Net surrogate
+    "lb_unet_create": (_i, [C.POINTER(LbUnetDesc), C.c_uint64, C.POINTER(_vp)]),
+    "lb_unet_destroy": (_i, [_vp]),
+    "lb_unet_param_count": (C.c_int64, [_vp]),
+    "lb_unet_get_params": (_i, [_vp, _vp, _vp]),
+    "lb_unet_set_params": (_i, [_vp, _vp]),
+    "lb_unet_forward": (_i, [_vp, _vp, _i, _vp]),
+    "lb_unet_backward": (_i, [_vp, _vp, _vp, _i, C.POINTER(C.c_double)]),
+    "lb_unet_adam": (_i, [_vp, _f, _f, _f, _f]),
+    "lb_unet_sync": (_i, [_vp]),
+    "lb_unet_to_fields": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "lb_conv3x3_bf16": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
 }
 
 EXPORTED = sorted(_PROTOS)
@@ -191,3 +208,46 @@ def check_sticky(what=""):
 
 def device_count():
     return int(lib.lb_device_count())
+
+
+class DeviceArray:
+    """A device allocation with numpy upload/download (plumbing for ctypes hosts)."""
+
+    def __init__(self, shape, dtype):
+        import numpy as np
+        self.shape = tuple(shape) if hasattr(shape, "__len__") else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(lib.lb_device_alloc(self.nbytes, C.byref(self.ptr)), "lb_device_alloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        import numpy as np
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        import numpy as np
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib.lb_memcpy_h2d(self.ptr, a.ctypes.data_as(C.c_void_p), self.nbytes), "lb_memcpy_h2d")
+
+    def numpy(self):
+        import numpy as np
+        out = np.empty(self.shape, self.dtype)
+        check(lib.lb_memcpy_d2h(out.ctypes.data_as(C.c_void_p), self.ptr, self.nbytes), "lb_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib.lb_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

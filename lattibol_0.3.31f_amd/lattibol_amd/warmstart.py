@@ -1,0 +1,160 @@
+"""Learned warm start: the batched solver produces steady channel fields, the
+U-Net learns them, and held-out channels are initialised at equilibrium with
+the predicted (rho, u) and relaxed by the plain LBM to the same residual.
+
+Residual: ||u(t) - u(t-k)||_2 / ||u(t)||_2 over fluid nodes, k = 100 steps,
+FP64 reduction on the GPU (lb_batch_run_to_steady).
+"""
+import time
+
+import numpy as np
+
+from . import scenes
+from ._lib import DeviceArray
+from .batch import Batch
+from .unet import UNet, make_inputs, make_targets
+
+
+def solve_ensemble(specs, tol=1e-6, k=100, max_steps=300000, fields_dev=None, unet=None):
+    """Run every channel to steady state; returns per-lattice steps, residuals,
+    macros and the wall time (including an optional U-Net inference)."""
+    b = Batch(specs)
+    t0 = time.perf_counter()
+    if unet is not None:
+        x = make_inputs(np.stack([s["solid"] for s in specs]), [s["u_in"] for s in specs], [s["tau"] for s in specs])
+        n = len(specs)
+        H, W = x.shape[1:3]
+        bsz = unet.desc.max_batch
+        xd = DeviceArray((bsz, H, W, 3), np.float32)
+        yd = DeviceArray((bsz, H, W, 3), np.float32)
+        ud = DeviceArray((bsz,), np.float32)
+        fields = DeviceArray((n, 3, H, W), np.float32)
+        for s in range(0, n, bsz):
+            m = min(bsz, n - s)
+            xb = np.zeros((bsz, H, W, 3), np.float32)
+            xb[:m] = x[s:s + m]
+            xd.upload(xb)
+            uin = np.zeros(bsz, np.float32)
+            uin[:m] = [sp["u_in"] for sp in specs[s:s + m]]
+            ud.upload(uin)
+            unet.forward(xd, m, yd)
+            # predicted planes go straight into the packed field array (device)
+            sub = _Offset(fields, s * 3 * H * W * 4)
+            unet.to_fields(yd, m, ud, sub)
+        b.init_fields_device(fields.ptr.value)
+    t_init = time.perf_counter() - t0
+    steps, res = b.run_to_steady(tol=tol, k=k, max_steps=max_steps)
+    wall = time.perf_counter() - t0
+    macros = [b.macros(i) for i in range(len(specs))]
+    b.close()
+    return dict(steps=steps, res=res, macros=macros, wall=wall, t_init=t_init)
+
+
+class _Offset:
+    """A view of a DeviceArray at a byte offset (for to_fields)."""
+
+    def __init__(self, arr, byte_off):
+        import ctypes as C
+        self.ptr = C.c_void_p(arr.ptr.value + byte_off)
+
+
+def dataset(n, seed, size=(128, 128), tol=1e-6, k=100, max_steps=300000):
+    specs = scenes.ensemble(n, seed=seed, size=size)
+    out = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps)
+    ok = out["res"] < tol
+    solid = np.stack([s["solid"] for s in specs])
+    u_in = np.array([s["u_in"] for s in specs])
+    tau = np.array([s["tau"] for s in specs])
+    rho = np.stack([m["rho"] for m in out["macros"]])
+    ux = np.stack([m["ux"] for m in out["macros"]])
+    uy = np.stack([m["uy"] for m in out["macros"]])
+    return dict(specs=specs, converged=ok, steps=out["steps"], wall=out["wall"],
+                x=make_inputs(solid, u_in, tau), t=make_targets(rho, ux, uy, u_in),
+                mask=(1 - solid).astype(np.float32), u_in=u_in, tau=tau)
+
+
+def fit_stats(t, mask):
+    m = mask.astype(bool)
+    st = []
+    for c in range(t.shape[-1]):
+        v = t[..., c][m]
+        st += [float(v.mean()), float(v.std() + 1e-8)]
+    return np.array(st, np.float32)
+
+
+def standardise(t, stats):
+    out = np.empty_like(t)
+    for c in range(t.shape[-1]):
+        out[..., c] = (t[..., c] - stats[2 * c]) / stats[2 * c + 1]
+    return out
+
+
+def train(net, data, epochs=60, batch=16, lr=1e-3, seed=0, log=None):
+    ok = data["converged"]
+    x, t, m = data["x"][ok], data["t"][ok], data["mask"][ok]
+    net.stats = fit_stats(t, m)
+    ts = standardise(t, net.stats)
+    n = x.shape[0]
+    H, W = x.shape[1:3]
+    xd = DeviceArray((batch, H, W, 3), np.float32)
+    td = DeviceArray((batch, H, W, 3), np.float32)
+    md = DeviceArray((batch, H, W), np.float32)
+    rng = np.random.default_rng(seed)
+    hist = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(n)
+        tot = 0.0
+        nb = 0
+        lr_ep = lr * 0.5 * (1 + np.cos(np.pi * ep / epochs))
+        for s in range(0, n - batch + 1, batch):
+            idx = order[s:s + batch]
+            xd.upload(x[idx])
+            td.upload(ts[idx])
+            md.upload(m[idx])
+            net.forward(xd, batch)
+            tot += net.backward(td, md, batch)
+            net.adam(lr=float(lr_ep))
+            nb += 1
+        hist.append(tot / max(nb, 1))
+        if log:
+            log("epoch %d loss %.5f (%.1f s)" % (ep, hist[-1], time.perf_counter() - t0))
+    net.sync()
+    return dict(loss=hist, seconds=time.perf_counter() - t0, samples=n)
+
+
+def velocity_l2(a, b, mask):
+    """||u_a - u_b|| / ||u_b|| over fluid nodes."""
+    m = mask.astype(bool)
+    da = np.concatenate([(a["ux"] - b["ux"])[m], (a["uy"] - b["uy"])[m]]).astype(np.float64)
+    nb = np.concatenate([b["ux"][m], b["uy"][m]]).astype(np.float64)
+    return float(np.linalg.norm(da) / max(np.linalg.norm(nb), 1e-30))
+
+
+def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128, 128)):
+    specs = scenes.ensemble(n, seed=seed, size=size)
+    plain = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps)
+    warm = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net)
+    both = (plain["res"] < tol) & (warm["res"] < tol)
+    l2 = np.array([velocity_l2(warm["macros"][i], plain["macros"][i], 1 - specs[i]["solid"]) for i in range(n)])
+    # prediction error of the network itself (before relaxation)
+    x = make_inputs(np.stack([s["solid"] for s in specs]), [s["u_in"] for s in specs], [s["tau"] for s in specs])
+    y = net.predict(x)
+    pred_err = []
+    for i in range(n):
+        u = specs[i]["u_in"]
+        pr = dict(ux=u * (y[i, ..., 0] * net.stats[1] + net.stats[0]), uy=u * (y[i, ..., 1] * net.stats[3] + net.stats[2]))
+        pred_err.append(velocity_l2(pr, plain["macros"][i], 1 - specs[i]["solid"]))
+    pred_err = np.array(pred_err)
+    return dict(
+        n=n, converged_plain=int((plain["res"] < tol).sum()), converged_warm=int((warm["res"] < tol).sum()),
+        converged_both=int(both.sum()),
+        steps_plain=int(plain["steps"][both].sum()), steps_warm=int(warm["steps"][both].sum()),
+        step_ratio=float(warm["steps"][both].sum() / max(plain["steps"][both].sum(), 1)),
+        wall_plain=plain["wall"], wall_warm=warm["wall"], wall_ratio=float(warm["wall"] / plain["wall"]),
+        inference_s=warm["t_init"],
+        l2_max=float(l2[both].max()) if both.any() else None, l2_median=float(np.median(l2[both])) if both.any() else None,
+        pred_err_median=float(np.median(pred_err[both])) if both.any() else None,
+        per_lattice=dict(steps_plain=plain["steps"].tolist(), steps_warm=warm["steps"].tolist(),
+                         l2=l2.tolist(), pred_err=pred_err.tolist()),
+    )

@@ -452,6 +452,34 @@ inline void moments(const T *f, T &rho, T &ux, T &uy, T &uz) {
     else mom19(f, rho, ux, uy, uz);
 }
 
+/* Nodes away from every face whose neighbours are all fluid pull with plain
+ * index offsets (the same values resolve_src would find). */
+template <int Q>
+std::vector<uint8_t> simple_nodes(const orc_cfg *cfg, const Geo &g, const uint8_t *solid, int dims) {
+    std::vector<uint8_t> m(g.N, 0);
+    for (int z = 0; z < g.n[2]; z++)
+        for (int y = 0; y < g.n[1]; y++)
+            for (int x = 0; x < g.n[0]; x++) {
+                if (x == 0 || x == g.n[0] - 1 || y == 0 || y == g.n[1] - 1) continue;
+                if (dims == 3 && (z == 0 || z == g.n[2] - 1)) continue;
+                int64_t id = g.idx(x, y, z);
+                bool ok = true;
+                if (!cfg->bb_full)
+                    for (int i = 1; i < Q && ok; i++)
+                        ok = !solid[g.idx(x - Lat<Q>::c(i, 0), y - Lat<Q>::c(i, 1), z - Lat<Q>::c(i, 2))];
+                m[id] = ok;
+            }
+    return m;
+}
+
+template <int Q, class S>
+inline void pull_simple(const Geo &g, const typename S::raw *A, int64_t id, typename S::T *f) {
+    for (int i = 0; i < Q; i++) {
+        int64_t off = Lat<Q>::c(i, 0) + (int64_t)g.n[0] * (Lat<Q>::c(i, 1) + (int64_t)g.n[1] * Lat<Q>::c(i, 2));
+        f[i] = S::ld(A[(int64_t)i * g.N + id - off], (float)wq<Q, float>(i));
+    }
+}
+
 template <int Q, class S>
 typename S::T node_density(const orc_cfg *cfg, const Geo &g, const typename S::raw *A,
                            const uint8_t *solid, const int p[3], int64_t id, int dims) {
@@ -477,10 +505,12 @@ template <class T> inline T sc_psi(T rho, float rho0) {
 
 template <int Q, class S>
 void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::raw *B,
-                    const uint8_t *solid, typename S::T *macros) {
+                    const uint8_t *solid, typename S::T *macros, const uint8_t *simple) {
     typedef typename S::T T;
     Geo g(cfg);
     const int dims = Q == 9 ? 2 : 3;
+    std::vector<uint8_t> own;
+    if (!simple) { own = simple_nodes<Q>(cfg, g, solid, dims); simple = own.data(); }
     const T ti = (T)cfg->tau_inv;
     const bool sc = cfg->sc_G != 0.0f;
     std::vector<T> psi;
@@ -526,7 +556,8 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     }
                     continue;
                 }
-                pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
+                if (simple[id]) pull_simple<Q, S>(g, A, id, f);
+                else pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
                 if (cfg->zou_he) {
                     if (x == 0) { if (Q == 9) zouhe9_in(f, cfg->zh_ux_in); else zouhe19_in(f, cfg->zh_ux_in); }
                     else if (x == g.n[0] - 1) { if (Q == 9) zouhe9_out(f, cfg->zh_rho_out); else zouhe19_out(f, cfg->zh_rho_out); }
@@ -1134,15 +1165,22 @@ void orc_pull_init_eq(const orc_cfg *cfg, const double *rho, const double *ux, c
 }
 
 void orc_pull_step(const orc_cfg *cfg, const void *A, void *B, const uint8_t *solid, void *macros) {
-    ORC_DISPATCH(cfg, (pull_step_impl<Q, S>(cfg, (const S::raw *)A, (S::raw *)B, solid, (S::T *)macros)));
+    ORC_DISPATCH(cfg, (pull_step_impl<Q, S>(cfg, (const S::raw *)A, (S::raw *)B, solid, (S::T *)macros, nullptr)));
 }
 
 void orc_pull_run(const orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps, void *macros) {
     size_t esz = cfg->storage == ORC_STORE_F32 ? 4 : (cfg->storage == ORC_STORE_F64 ? 8 : 2);
     size_t bytes = esz * (size_t)cfg->lattice * cfg->nx * cfg->ny * cfg->nz;
     void *a = A, *b = B;
+    std::vector<uint8_t> simple;
+    {
+        Geo g(cfg);
+        simple = cfg->lattice == 9 ? simple_nodes<9>(cfg, g, solid, 2) : simple_nodes<19>(cfg, g, solid, 3);
+    }
+    const uint8_t *sp = simple.data();
     for (int s = 0; s < steps; s++) {
-        orc_pull_step(cfg, a, b, solid, s == steps - 1 ? macros : nullptr);
+        void *mac = s == steps - 1 ? macros : nullptr;
+        ORC_DISPATCH(cfg, (pull_step_impl<Q, S>(cfg, (const S::raw *)a, (S::raw *)b, solid, (S::T *)mac, sp)));
         std::swap(a, b);
     }
     if (a != A) std::memcpy(A, a, bytes);

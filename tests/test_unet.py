@@ -1,0 +1,169 @@
+"""U-Net surrogate: the CPU oracle pinned by finite differences and torch's own
+conv2d; the GPU kernels (conv on BF16 matrix cores, full forward/backward,
+Adam) checked against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import unet_oracle as UO
+from lattibol_amd.unet import conv_layout
+
+SMALL = dict(levels=3, base=32, in_ch=3, out_ch=3)
+
+
+def _rand_params(levels, base, in_ch, out_ch, seed=0, scale=0.15):
+    layout, n = conv_layout(levels, base, in_ch, out_ch)
+    rng = np.random.default_rng(seed)
+    p = rng.standard_normal(n) * scale
+    # the padded input channels of the first conv carry zero weights
+    kind, co, ci, w, b = layout[0]
+    W = p[w:b].reshape(co, 9, ci)
+    W[:, :, in_ch:] = 0
+    p[w:b] = W.ravel()
+    return p.astype(np.float32), layout
+
+
+def test_oracle_layout_matches_binding():
+    convs, _ = UO.conv_layout(4, 32, 3, 3)
+    layout, n = conv_layout(4, 32, 3, 3)
+    assert [(ci, co) for ci, co in convs] == [(l[2], l[1]) for l in layout[:-1]]
+    # the 4-level, 32-channel network of the warm start: ~5.3 GFLOP per 128x128 sample
+    flops = 0
+    H = 128
+    for (ci, co), l in zip(convs, [0, 0, 1, 1, 2, 2, 3, 3, 2, 2, 1, 1, 0, 0]):
+        flops += 2 * (H >> l) ** 2 * 9 * ci * co
+    assert 5.0e9 < flops < 5.8e9
+
+
+def test_oracle_gradient_matches_finite_differences():
+    torch.manual_seed(0)
+    p, _ = _rand_params(**SMALL, seed=1)
+    params = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    x = torch.rand(1, 8, 8, 3, dtype=torch.float64)
+    t = torch.rand(1, 8, 8, 3, dtype=torch.float64)
+    m = (torch.rand(1, 8, 8) > 0.2).double()
+    loss = UO.masked_mse(UO.forward(params, x, **SMALL, emulate=False), t, m)
+    (g,) = torch.autograd.grad(loss, params)
+    rng = np.random.default_rng(2)
+    idx = rng.choice(p.size, 40, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = torch.zeros_like(params)
+        e[i] = h
+        with torch.no_grad():
+            lp = UO.masked_mse(UO.forward(params + e, x, **SMALL, emulate=False), t, m)
+            lm = UO.masked_mse(UO.forward(params - e, x, **SMALL, emulate=False), t, m)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd.item() - g[i].item()) <= 1e-6 + 1e-5 * abs(fd.item()), (i, fd.item(), g[i].item())
+
+
+def test_conv_reference_matches_direct_sum():
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((1, 5, 6, 16))
+    w = rng.standard_normal((32, 9, 16))
+    y = UO.conv3x3_ref(x, w)
+    n, h, ww, co = 0, 2, 0, 7
+    s = 0.0
+    for tap in range(9):
+        ky, kx = divmod(tap, 3)
+        hh, wx = h + ky - 1, ww + kx - 1
+        if 0 <= hh < 5 and 0 <= wx < 6:
+            s += float(np.dot(x[n, hh, wx], w[co, tap]))
+    assert abs(y[n, h, ww, co] - s) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+def _bf16_round(a):
+    return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ci,co", [(16, 32), (32, 64), (96, 32), (384, 128)])
+def test_gpu_conv3x3_matches_reference(gpu, ci, co):
+    from lattibol_amd import _lib
+    rng = np.random.default_rng(ci * 1000 + co)
+    B, H, W = 2, 16, 16
+    x = _bf16_round(rng.standard_normal((B, H, W, ci)))
+    w = _bf16_round(rng.standard_normal((co, 9, ci)) * 0.1)
+    bias = rng.standard_normal(co).astype(np.float32) * 0.1
+    to_u16 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).view(torch.int16).numpy()
+    xd = _lib.DeviceArray.from_numpy(to_u16(x))
+    wd = _lib.DeviceArray.from_numpy(to_u16(w))
+    bd = _lib.DeviceArray.from_numpy(bias)
+    yd = _lib.DeviceArray((B, H, W, co), np.int16)
+    _lib.check(_lib.lib.lb_conv3x3_bf16(0, xd.ptr, wd.ptr, bd.ptr, yd.ptr, B, H, W, ci, co, None), "conv")
+    _lib.check(_lib.lib.lb_device_synchronize(), "sync")
+    y = torch.from_numpy(yd.numpy()).view(torch.bfloat16).to(torch.float32).numpy()
+    ref = UO.conv3x3_ref(x, w, bias, relu=True)
+    # one BF16 rounding of an FP32-accumulated value: <= 2^-8 relative
+    err = np.abs(y - ref)
+    assert (err <= 2.0 ** -8 * np.abs(ref) + 1e-6).all(), err.max()
+    # FP32 in / FP32 out (the backward-data form)
+    xf = rng.standard_normal((B, H, W, ci)).astype(np.float32)
+    xfd = _lib.DeviceArray.from_numpy(xf)
+    yfd = _lib.DeviceArray((B, H, W, co), np.float32)
+    _lib.check(_lib.lib.lb_conv3x3_bf16(1, xfd.ptr, wd.ptr, None, yfd.ptr, B, H, W, ci, co, None), "conv")
+    _lib.check(_lib.lib.lb_device_synchronize(), "sync")
+    ref = UO.conv3x3_ref(_bf16_round(xf), w)
+    np.testing.assert_allclose(yfd.numpy(), ref, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.gpu
+def test_gpu_unet_forward_backward_match_oracle(gpu):
+    from lattibol_amd import _lib
+    from lattibol_amd.unet import UNet
+    B, H = 2, 32
+    net = UNet(**SMALL, height=H, width=H, max_batch=B, seed=7)
+    p, _ = _rand_params(**SMALL, seed=4)
+    net.set_params(p)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    x[..., 0] = rng.uniform(size=(B, H, H)) < 0.2
+    t = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    m = (rng.uniform(size=(B, H, H)) > 0.1).astype(np.float32)
+    xd, td, md = (_lib.DeviceArray.from_numpy(a) for a in (x, t, m))
+    yd = _lib.DeviceArray((B, H, H, 3), np.float32)
+    net.forward(xd, B, yd)
+    loss = net.backward(td, md, B)
+    y = yd.numpy()
+    _, g = net.params(grads=True)
+    params = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    yo = UO.forward(params, torch.from_numpy(x).double(), **SMALL)
+    lo = UO.masked_mse(yo, torch.from_numpy(t).double(), torch.from_numpy(m).double())
+    (go,) = torch.autograd.grad(lo, params)
+    yo = yo.detach().numpy()
+    rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
+    assert rel < 2e-2, rel
+    assert abs(loss - lo.item()) < 3e-2 * lo.item()
+    go = go.numpy()
+    layout, _ = conv_layout(**SMALL)
+    for kind, co, ci, w, b in layout:
+        gw, ow = g[w:b], go[w:b]
+        num = np.linalg.norm(gw - ow)
+        den = np.linalg.norm(ow) + 1e-30
+        cos = float(np.dot(gw, ow) / (np.linalg.norm(gw) * np.linalg.norm(ow) + 1e-30))
+        assert num / den < 8e-2 and cos > 0.995, (kind, co, ci, num / den, cos)
+    net.close()
+
+
+@pytest.mark.gpu
+def test_gpu_unet_training_reduces_loss(gpu):
+    from lattibol_amd import _lib
+    from lattibol_amd.unet import UNet
+    B, H = 4, 32
+    net = UNet(**SMALL, height=H, width=H, max_batch=B, seed=11)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    t = np.tanh(x[..., ::-1]).astype(np.float32).copy()
+    m = np.ones((B, H, H), np.float32)
+    xd, td, md = (_lib.DeviceArray.from_numpy(a) for a in (x, t, m))
+    losses = []
+    for it in range(60):
+        net.forward(xd, B)
+        losses.append(net.backward(td, md, B))
+        net.adam(lr=2e-3)
+    net.sync()
+    assert losses[-1] < 0.3 * losses[0], (losses[0], losses[-1])
+    net.close()
