@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--workload", default="sphere",
-                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart"],
+                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "copy"],
                     help="sphere = D3Q19 channel past a sphere (the headline); the others print their own line")
     ap.add_argument("--train-samples", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=60)
@@ -203,6 +203,24 @@ def bench_warmstart(args):
                                        "final_loss": hist["loss"][-1]}, evaluation=ev)
 
 
+def bench_copy(args):
+    """Plain device copy of 512 MiB: HBM bandwidth reference and the
+    calibration run for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    import numpy as np
+    from lattibol_amd import _lib
+    nbytes = 512 << 20
+    a = _lib.DeviceArray((nbytes,), np.uint8)
+    b = _lib.DeviceArray((nbytes,), np.uint8)
+    ms = C.c_float()
+    reps = max(1, args.steps)
+    _lib.check(_lib.lib.lb_copy_device(b.ptr, a.ptr, nbytes, reps, C.byref(ms)), "lb_copy_device")
+    gbs = 2 * nbytes / (ms.value * 1e-3) / 1e9
+    return common("device copy bandwidth (read + write)", round(gbs, 1), "GB/s", args, reps, round(ms.value, 5),
+                  "u8", {"workload": "k_copy16, 512 MiB read + 512 MiB written per launch"},
+                  bytes_read_per_launch=nbytes, bytes_written_per_launch=nbytes)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -211,7 +229,7 @@ def main():
         sys.exit(2)
     if args.workload != "sphere":
         fn = {"cavity": bench_cavity, "spinodal": bench_spinodal, "ensemble": bench_ensemble,
-              "warmstart": bench_warmstart}[args.workload]
+              "warmstart": bench_warmstart, "copy": bench_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     import numpy as np

@@ -97,6 +97,10 @@ int lb_device_free(void *p);
 int lb_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int lb_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int lb_device_synchronize(void);
+/* A plain 16-byte-vector copy kernel of `bytes` (multiple of 16), launched
+ * repeats times after one warm launch; average ms per copy.  It reads and
+ * writes exactly `bytes`: the calibration point for HBM traffic counters. */
+int lb_copy_device(void *dst, const void *src, size_t bytes, int repeats, float *ms_per_copy);
 
 /* ---------------------------------------------------------------------- */
 /* 1. Reference entry points (reference semantics, legacy default stream)  */

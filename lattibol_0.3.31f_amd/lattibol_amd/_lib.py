@@ -106,6 +106,7 @@ _PROTOS = {
     "lb_memcpy_h2d": (_i, [_vp, _vp, C.c_size_t]),
     "lb_memcpy_d2h": (_i, [_vp, _vp, C.c_size_t]),
     "lb_device_synchronize": (_i, []),
+    "lb_copy_device": (_i, [_vp, _vp, C.c_size_t, _i, FP]),
     # reference launchers
     "streaming_d2q9": (None, [C.POINTER(DataD2Q9), _d3, _d3]),
     "streaming_d3q19": (None, [C.POINTER(DataD3Q19), _d3, _d3]),

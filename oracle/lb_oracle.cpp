@@ -526,10 +526,29 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     psi[id] = sc_psi<T>(node_density<Q, S>(cfg, g, A, solid, p, id, dims), cfg->sc_rho0);
                 }
     }
+    // Rows are processed in bursts: the pulls of the simple nodes are staged
+    // per population (one contiguous stream at a time) and the results are
+    // written back per population, which keeps the CPU's prefetchers fed.
+    // The arithmetic per node is unchanged.
+    const int nx = g.n[0];
+    int64_t offs[19];
+    for (int i = 0; i < Q; i++)
+        offs[i] = Lat<Q>::c(i, 0) + (int64_t)g.n[0] * (Lat<Q>::c(i, 1) + (int64_t)g.n[1] * Lat<Q>::c(i, 2));
 #pragma omp parallel for schedule(static) num_threads(g_threads)
-    for (int z = 0; z < g.n[2]; z++)
-        for (int y = 0; y < g.n[1]; y++)
-            for (int x = 0; x < g.n[0]; x++) {
+    for (int z = 0; z < g.n[2]; z++) {
+        std::vector<T> rin((size_t)Q * nx);
+        std::vector<typename S::raw> rout((size_t)Q * nx);
+        std::vector<uint8_t> wrote(nx);
+        for (int y = 0; y < g.n[1]; y++) {
+            const int64_t rowbase = g.idx(0, y, z);
+            for (int i = 0; i < Q; i++) {
+                const typename S::raw *src = A + (int64_t)i * g.N + rowbase - offs[i];
+                const float w = (float)wq<Q, float>(i);
+                for (int x = 0; x < nx; x++)
+                    if (simple[rowbase + x]) rin[(size_t)i * nx + x] = S::ld(src[x], w);
+            }
+            std::fill(wrote.begin(), wrote.end(), 0);
+            for (int x = 0; x < nx; x++) {
                 int p[3] = {x, y, z};
                 int64_t id = g.idx(x, y, z);
                 T f[19];
@@ -538,8 +557,9 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                         pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
                         for (int i = 0; i < Q; i++) {
                             int j = Lat<Q>::opp(i);
-                            B[(int64_t)j * g.N + id] = S::st(f[i], (float)wq<Q, float>(j));
+                            rout[(size_t)j * nx + x] = S::st(f[i], (float)wq<Q, float>(j));
                         }
+                        wrote[x] = 1;
                     }
                     if (macros) for (int k = 0; k < 4; k++) macros[k * g.N + id] = T(0);
                     continue;
@@ -550,13 +570,14 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                       uz = (T)cfg->bc_u[inf][2];
                     if (Q == 9) feq9<T>(f, rho, ux, uy, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                     else feq19<T>(f, rho, ux, uy, uz, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
-                    for (int i = 0; i < Q; i++) B[(int64_t)i * g.N + id] = S::st(f[i], (float)wq<Q, float>(i));
+                    for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
+                    wrote[x] = 1;
                     if (macros) {
                         macros[id] = rho; macros[g.N + id] = ux; macros[2 * g.N + id] = uy; macros[3 * g.N + id] = uz;
                     }
                     continue;
                 }
-                if (simple[id]) pull_simple<Q, S>(g, A, id, f);
+                if (simple[id]) for (int i = 0; i < Q; i++) f[i] = rin[(size_t)i * nx + x];
                 else pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
                 if (cfg->zou_he) {
                     if (x == 0) { if (Q == 9) zouhe9_in(f, cfg->zh_ux_in); else zouhe19_in(f, cfg->zh_ux_in); }
@@ -595,8 +616,16 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                 }
                 if (Q == 9) bgk9<T>(f, rho, ex, ey, ti, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                 else bgk19<T>(f, rho, ex, ey, ez, ti, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
-                for (int i = 0; i < Q; i++) B[(int64_t)i * g.N + id] = S::st(f[i], (float)wq<Q, float>(i));
+                for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
+                wrote[x] = 1;
             }
+            for (int i = 0; i < Q; i++) {
+                typename S::raw *dst = B + (int64_t)i * g.N + rowbase;
+                for (int x = 0; x < nx; x++)
+                    if (wrote[x]) dst[x] = rout[(size_t)i * nx + x];
+            }
+        }
+    }
 }
 
 template <int Q, class S>
