@@ -202,6 +202,8 @@ typedef struct {
     int block_x, block_y;   /* launch shape; 0 = autotune */
     int xcd_remap;          /* -1 autotune, 0 off, 1 on */
     int graph_chunk;        /* steps per captured hipGraph; 0 = default */
+    int rest_closure;       /* 1: feq_0 = rho - sum_{i>0} feq_i (exact mass balance);
+                               0: the reference's equilibrium for every direction */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;
@@ -252,12 +254,14 @@ int lb_halo_buffers(lb_ctx *ctx, void **send_lo, void **send_hi, void **recv_lo,
 typedef struct {
     int nx, ny;
     float tau_inv;
-    float u_in;             /* equilibrium inlet velocity at x = 0 (rho 1) */
+    float u_in;             /* Zou-He inlet velocity at x = 0 */
     int64_t solid_offset;   /* offset of this lattice's mask in the packed mask array */
 } lb_batch_lattice;
 typedef struct lb_batch lb_batch;
-/* Walls (half-way bounce-back) at rows 0 and ny-1, equilibrium inlet at x=0,
- * zero-gradient outflow at x=nx-1, solid masks packed back to back. */
+/* Walls (half-way bounce-back) at rows 0 and ny-1, Zou-He velocity inlet
+ * u = (u_in, 0) at x = 0 and Zou-He pressure outlet rho = 1 at x = nx-1 (the
+ * reference's inlet/outlet, collision.cu:211-262), rest closure on; solid
+ * masks packed back to back. */
 int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t *solids, lb_batch **out);
 int lb_batch_destroy(lb_batch *b);
 /* Start from rest (rho 1, u 0) or from device fields packed like the masks

@@ -5,10 +5,10 @@
 // residual falls below the tolerance retire from the block table.
 //
 // Each lattice runs the exact per-node code of the single-lattice solver
-// (step_body in lb_pull.hpp) with the configuration
-//   face -x: equilibrium inlet (rho 1, u = (u_in, 0)),  face +x: zero-gradient
-//   outflow, rows 0 and ny-1: half-way bounce-back walls, solids: half-way
-//   bounce-back,
+// (step_body in lb_pull.hpp) with the configuration of lattice_config():
+//   Zou-He velocity inlet u = (u_in, 0) at x = 0 and pressure outlet rho = 1
+//   at x = nx-1 (the reference's pair, collision.cu:211-262), rows 0 and ny-1
+//   half-way bounce-back walls, solids half-way bounce-back, rest closure,
 // so a lattice of the batch is bit-identical to lb_create() with that
 // configuration.
 #include <hip/hip_runtime.h>
@@ -35,22 +35,40 @@ struct BatchLat {
     int ntx, nty;
 };
 
-LB_DEV StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, const uint8_t *flags, float *macros) {
+// Configuration of every lattice of the batch (also used to build its node
+// flags, so a member is identical to lb_create() with this configuration).
+inline lb_config lattice_config(int nx, int ny, float ti, float u_in) {
+    lb_config c{};
+    c.lattice = 9; c.nx = nx; c.ny = ny; c.nz = 1;
+    c.storage = LB_STORE_F32;
+    c.tau_inv = ti;
+    c.bc[0] = LB_BC_OUTFLOW; c.bc[1] = LB_BC_OUTFLOW; c.bc[2] = LB_BC_WALL; c.bc[3] = LB_BC_WALL;
+    c.zou_he = 1; c.zh_ux_in = u_in; c.zh_rho_out = 1.0f;
+    c.rest_closure = 1;
+    return c;
+}
+
+// Argument block of lattice L reading buffer `src` (A = 0 / B = 1).
+StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags, float *macros) {
+    const lb_config c = lattice_config(L.nx, L.ny, L.ti, L.u_in);
     StepArgs<StF32> a{};
-    a.A = A + L.pop_off;
-    a.B = B + L.pop_off;
+    float *S0 = src == 0 ? A : B, *S1 = src == 0 ? B : A;
+    a.A = S0 + L.pop_off;
+    a.B = S1 + L.pop_off;
     a.flags = flags + L.node_off;
     a.macros = macros + 4 * L.node_off;
     a.N = L.N;
     a.nxy = L.N;
     a.P = (uint32_t)L.nx;
     a.nx = L.nx; a.ny = L.ny; a.nz = 1;
-    a.ti = L.ti;
-    a.bc[0] = LB_BC_INLET_EQ; a.bc[1] = LB_BC_OUTFLOW; a.bc[2] = LB_BC_WALL; a.bc[3] = LB_BC_WALL;
-    a.bc[4] = LB_BC_PERIODIC; a.bc[5] = LB_BC_PERIODIC;
-    a.bcu[0][0] = L.u_in;
-    a.bcrho[0] = 1.0f;
-    a.corr_lo[0] = 1; a.corr_hi[0] = -1;
+    a.ti = c.tau_inv;
+    for (int k = 0; k < 6; k++) {
+        a.bc[k] = c.bc[k];
+        a.bcrho[k] = c.bc_rho[k];
+    }
+    face_corrections(c, a.corr_lo, a.corr_hi);
+    a.zou_he = 1; a.zh_ux = c.zh_ux_in; a.zh_rho = c.zh_rho_out;
+    a.rest_closure = 1;
     a.bx_shift = 6; a.by = kBY;
     a.ntx = L.ntx; a.nty = L.nty;
     a.nblocks = L.ntx * L.nty;
@@ -58,11 +76,10 @@ LB_DEV StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, const
 }
 
 template <bool MACROS>
-__global__ __launch_bounds__(kThreads) void k_batch_step(const BatchLat *lats, const int2 *table, float *A, float *B,
-                                                         const uint8_t *flags, float *macros) {
+__global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32> *__restrict__ args,
+                                                         const int2 *__restrict__ table, int src) {
     const int2 e = table[blockIdx.x];          // (lattice, local tile)
-    const BatchLat L = lats[e.x];
-    const StepArgs<StF32> a = lattice_args(L, A, B, flags, macros);
+    const StepArgs<StF32> &a = args[2 * e.x + src];
     step_body<9, StF32, false, MACROS>(a, node_of(a, e.y), nullptr);
 }
 
@@ -144,6 +161,7 @@ struct lb_batch {
     uint64_t pop_total = 0, node_total = 0;
     hipStream_t stream = nullptr;
     BatchLat *d_lats = nullptr;
+    StepArgs<StF32> *d_args = nullptr;   // 2 per lattice (source buffer A / B)
     int2 *d_table = nullptr, *d_ranges = nullptr;
     float *A = nullptr, *B = nullptr, *macros = nullptr, *snap = nullptr;
     uint8_t *flags = nullptr;
@@ -157,7 +175,7 @@ struct lb_batch {
 
     ~lb_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        void *ps[] = {d_lats, d_table, d_ranges, A, B, macros, snap, flags, partial, d_res};
+        void *ps[] = {d_lats, d_args, d_table, d_ranges, A, B, macros, snap, flags, partial, d_res};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -188,10 +206,8 @@ int rebuild_table(lb_batch *b, bool all_lattices = false) {
 }
 
 void launch_batch_step(lb_batch *b, int parity, bool macros) {
-    float *src = parity == 0 ? b->A : b->B;
-    float *dst = parity == 0 ? b->B : b->A;
-    if (macros) hipLaunchKernelGGL(k_batch_step<true>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, src, dst, b->flags, b->macros);
-    else hipLaunchKernelGGL(k_batch_step<false>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, src, dst, b->flags, b->macros);
+    if (macros) hipLaunchKernelGGL(k_batch_step<true>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_args, b->d_table, parity);
+    else hipLaunchKernelGGL(k_batch_step<false>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_args, b->d_table, parity);
 }
 
 void launch_batch_residual(lb_batch *b) {
@@ -224,9 +240,7 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
         L.ntx = (s.nx + kBX - 1) / kBX; L.nty = (s.ny + kBY - 1) / kBY;
         total_tiles += (size_t)L.ntx * L.nty;
         // node flags with the same rules as the single-lattice solver
-        lb_config cfg{};
-        cfg.lattice = 9; cfg.nx = s.nx; cfg.ny = s.ny; cfg.nz = 1;
-        cfg.bc[0] = LB_BC_INLET_EQ; cfg.bc[1] = LB_BC_OUTFLOW; cfg.bc[2] = LB_BC_WALL; cfg.bc[3] = LB_BC_WALL;
+        const lb_config cfg = lattice_config(s.nx, s.ny, s.tau_inv, s.u_in);
         std::vector<uint8_t> sol(solids + s.solid_offset, solids + s.solid_offset + L.N), fl;
         for (auto &v : sol) v = v ? 1 : 0;
         build_flags(cfg, sol, fl);
@@ -242,6 +256,7 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     b->macros_valid.assign(n, 0);
     int rc = (int)hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (!rc) rc = (int)hipMalloc((void **)&b->d_lats, n * sizeof(BatchLat));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_args, 2 * n * sizeof(StepArgs<StF32>));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_table, total_tiles * sizeof(int2));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_ranges, n * sizeof(int2));
     if (!rc) rc = (int)hipMalloc((void **)&b->A, pop * sizeof(float));
@@ -252,6 +267,11 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 2 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
     if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
+    std::vector<StepArgs<StF32>> hargs(2 * (size_t)n);
+    if (!rc)
+        for (int i = 0; i < n; i++)
+            for (int src = 0; src < 2; src++) hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->macros);
+    if (!rc) rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32>), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemsetAsync(b->macros, 0, 4 * node * sizeof(float), b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);

@@ -21,9 +21,13 @@ namespace {
 // Kernels: one thread per node, 256-thread workgroups = BX x BY tiles of one
 // z-plane, tiles optionally remapped so each XCD sweeps a contiguous slab.
 // ---------------------------------------------------------------------------
+// The argument block lives in device memory (uniform pointer -> scalar loads);
+// passed by value it would be copied to per-lane scratch because the device
+// functions index its arrays.
 template <int Q, class S, bool SC, bool MACROS>
-__global__ __launch_bounds__(kThreads) void k_step(StepArgs<S> a) {
+__global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    const StepArgs<S> &a = *pa;
     const int b = tile_of_block(a);
     const NodePos p = node_of(a, b);
     if constexpr (SC) {
@@ -34,7 +38,8 @@ __global__ __launch_bounds__(kThreads) void k_step(StepArgs<S> a) {
 }
 
 template <int Q, class S>
-__global__ __launch_bounds__(kThreads) void k_density(StepArgs<S> a) {
+__global__ __launch_bounds__(kThreads) void k_density(const StepArgs<S> *__restrict__ pa) {
+    const StepArgs<S> &a = *pa;
     density_body<Q, S>(a, node_of(a, tile_of_block(a)));
 }
 
@@ -177,10 +182,12 @@ struct lb_ctx {
     int bx_shift = 6, by = 4;
     int xcd = 0;
     int corr_lo[3] = {0, 0, 0}, corr_hi[3] = {0, 0, 0};
+    void *dargs = nullptr;            // two argument blocks (source buffer 0 / 1)
+    static constexpr size_t kArgBytes = 512;
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
-        void *ps[] = {buf[0], buf[1], flags, macros, snap, psi, send, recv, partial, dout, tmp_fields};
+        void *ps[] = {buf[0], buf[1], flags, macros, snap, psi, send, recv, partial, dout, tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -214,7 +221,7 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     }
     for (int d = 0; d < 3; d++) { a.corr_lo[d] = c->corr_lo[d]; a.corr_hi[d] = c->corr_hi[d]; a.force[d] = c->cfg.force[d]; }
     a.scG = c->cfg.sc_G; a.scrho0 = c->cfg.sc_rho0; a.scpsi_solid = c->cfg.sc_psi_solid;
-    a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he;
+    a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he; a.rest_closure = c->cfg.rest_closure;
     a.halo = c->cfg.halo_z && c->Q == 19;
     a.zh_ux = c->cfg.zh_ux_in; a.zh_rho = c->cfg.zh_rho_out;
     a.bx_shift = c->bx_shift; a.by = c->by;
@@ -225,21 +232,41 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     return a;
 }
 
+template <class S> int upload_args_t(lb_ctx *c) {
+    static_assert(sizeof(StepArgs<S>) <= lb_ctx::kArgBytes, "argument block size");
+    StepArgs<S> h[2] = {make_args<S>(c, 0), make_args<S>(c, 1)};
+    int rc = (int)hipStreamSynchronize(c->stream);
+    if (!rc) rc = (int)hipMemcpy(c->dargs, &h[0], sizeof(StepArgs<S>), hipMemcpyHostToDevice);
+    if (!rc) rc = (int)hipMemcpy((char *)c->dargs + lb_ctx::kArgBytes, &h[1], sizeof(StepArgs<S>), hipMemcpyHostToDevice);
+    return rc;
+}
+
+// (Re)write both argument blocks after any change of shape, tuning or flags.
+int upload_args(lb_ctx *c) {
+    switch (c->cfg.storage) {
+    case LB_STORE_F32: return upload_args_t<StF32>(c);
+    case LB_STORE_F16: return upload_args_t<StF16>(c);
+    case LB_STORE_BF16: return upload_args_t<StBF16>(c);
+    default: return upload_args_t<StF64>(c);
+    }
+}
+
 template <int Q, class S>
 int launch_step_t(lb_ctx *c, int src, bool macros) {
-    StepArgs<S> a = make_args<S>(c, src);
+    StepArgs<S> a = make_args<S>(c, src);   // host copy: launch geometry only
+    const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
     size_t lds = 0;
     if (sc) {
         lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
-        hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
+        hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
     }
     if (sc) {
-        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, a);
-        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, a);
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa);
+        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa);
     } else {
-        if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
-        else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, a);
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
     }
     return (int)hipGetLastError();
 }
@@ -355,6 +382,7 @@ int autotune(lb_ctx *c) {
     const int reps = 6;
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd;
+        if (int rc = upload_args(c)) return rc;
         launch_step(c, 0, false);   // warm
         (void)hipEventRecord(e0, c->stream);
         for (int r = 0; r < reps; r++) launch_step(c, r & 1, false);
@@ -367,7 +395,9 @@ int autotune(lb_ctx *c) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd;
-    return (int)hipGetLastError();
+    int rc = (int)hipGetLastError();
+    if (!rc) rc = upload_args(c);
+    return rc;
 }
 
 }  // namespace
@@ -406,6 +436,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipMalloc((void **)&c->partial, (size_t)kReduceBlocks * 4 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&c->dout, 8 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&c->tmp_fields, (size_t)4 * c->N * sizeof(float));
+    if (!rc) rc = (int)hipMalloc(&c->dargs, 2 * lb_ctx::kArgBytes);
     if (!rc && cfg->sc_G != 0.0f) rc = (int)hipMalloc(&c->psi, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4));
     if (!rc && cfg->halo_z && c->Q == 19) {
         rc = (int)hipMalloc(&c->send, (size_t)10 * c->P * c->esz);
@@ -423,6 +454,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->bx_shift = ilog2(cfg->block_x);
             c->by = std::max(1, kThreads / cfg->block_x);
             c->xcd = cfg->xcd_remap > 0 ? 1 : 0;
+            rc = upload_args(c);
         } else {
             rc = autotune(c);
         }

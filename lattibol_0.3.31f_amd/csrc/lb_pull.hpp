@@ -45,6 +45,7 @@ template <class S> struct StepArgs {
     float force[3];
     float scG, scrho0, scpsi_solid;
     int bb_full, zou_he, halo;
+    int rest_closure;             // feq_0 = rho - sum_{i>0} feq_i (exact mass balance)
     float zh_ux, zh_rho;
     int bx_shift, by, ntx, nty, nblocks, xcd;
 };
@@ -201,6 +202,26 @@ template <int Q, class T> LB_DEV void feq(T *f, T rho, T ux, T uy, T uz) {
     else feq19<T>(f, rho, ux, uy, uz, weight<19, T>(0), weight<19, T>(1), weight<19, T>(2));
 }
 
+// BGK relaxation towards feq (the reference expressions, collision.cu:312-330
+// and :504-541, evaluated first as equilibria, which rounds identically).
+// With rest_closure the rest equilibrium is rho minus the moving ones, so
+// the collision conserves mass up to rounding noise (the float lattice
+// weights alone sum to 1 + 3e-8 for D3Q19).
+template <int Q, class T>
+LB_DEV void collide(T *f, T rho, T ux, T uy, T uz, T ti, bool rest_closure) {
+    T eq[Q];
+    feq<Q, T>(eq, rho, ux, uy, uz);
+    if (rest_closure) {
+        T s = eq[1];
+        static_for<2, Q>([&](auto I) { s = s + eq[decltype(I)::value]; });
+        eq[0] = rho - s;
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        f[i] = f[i] - ti * (f[i] - eq[i]);
+    });
+}
+
 // Shan-Chen psi tile in LDS: (BX+2) x (BY+2) x 3 planes around the block.
 template <int Q, class S>
 LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int bx, int by) {
@@ -302,8 +323,7 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
         ey = ey + (gp * IF[1]) / (ti * rho);
         if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
     }
-    if constexpr (Q == 9) bgk9<T>(f, rho, ex, ey, ti, weight<9, T>(0), weight<9, T>(1), weight<9, T>(2));
-    else bgk19<T>(f, rho, ex, ey, ez, ti, weight<19, T>(0), weight<19, T>(1), weight<19, T>(2));
+    collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
     store_node<Q, S>(a, p, f, false);
 }
 

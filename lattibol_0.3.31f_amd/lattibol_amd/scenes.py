@@ -119,8 +119,10 @@ def ensemble(n=512, seed=1234, size=None, start=0):
 
 
 def channel_config(spec, storage="f32", **kw):
-    """Single-lattice configuration equal to one member of the batch."""
+    """Single-lattice configuration equal to one member of the batch: the
+    reference's Zou-He velocity inlet (u_in, 0) at x = 0 and pressure outlet
+    rho = 1 at x = nx-1, half-way bounce-back walls on rows 0 and ny-1."""
     cfg = make_config(9, (spec["nx"], spec["ny"]), tau=spec["tau"], storage=storage,
-                      bc={"-x": "inlet", "+x": "outflow", "-y": "wall", "+y": "wall"},
-                      bc_u={"-x": (spec["u_in"], 0.0, 0.0)}, bc_rho={"-x": 1.0}, **kw)
+                      bc={"-x": "outflow", "+x": "outflow", "-y": "wall", "+y": "wall"},
+                      zou_he=True, zh_ux_in=spec["u_in"], zh_rho_out=1.0, rest_closure=True, **kw)
     return cfg

@@ -23,8 +23,16 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
-                zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0):
-    """Plain-dict description of a lattice; the oracle binding reads the same dict."""
+                zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
+                rest_closure=None):
+    """Plain-dict description of a lattice; the oracle binding reads the same dict.
+
+    rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
+    that collisions balance mass exactly up to rounding; off reproduces the
+    reference's equilibrium for every direction (needed for its full-way
+    bounce-back decks)."""
+    if rest_closure is None:
+        rest_closure = not bb_full
     nx, ny = shape[0], shape[1]
     nz = shape[2] if len(shape) > 2 else 1
     if tau_inv is None:
@@ -45,13 +53,13 @@ def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, 
                 sc_G=float(sc_G), sc_rho0=float(sc_rho0), sc_psi_solid=float(sc_psi_solid),
                 bb_full=int(bool(bb_full)), zou_he=int(bool(zou_he)), zh_ux_in=float(zh_ux_in),
                 zh_rho_out=float(zh_rho_out), halo_z=int(bool(halo_z)), block_x=int(block_x),
-                xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk))
+                xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)))
 
 
 def to_lbconfig(d):
     c = LbConfig()
     for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid", "bb_full",
-              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk"):
+              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure"):
         setattr(c, k, d[k])
     for f in range(6):
         c.bc[f] = d["bc"][f]

@@ -614,7 +614,17 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     ey = ey + (gp * IF[1]) / (ti * rho);
                     if (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
                 }
-                if (Q == 9) bgk9<T>(f, rho, ex, ey, ti, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
+                if (cfg->rest_closure) {
+                    /* equilibria first (same roundings as the BGK expressions),
+                       rest population closes the mass balance */
+                    T eq[19];
+                    if (Q == 9) feq9<T>(eq, rho, ex, ey, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
+                    else feq19<T>(eq, rho, ex, ey, ez, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
+                    T sum = eq[1];
+                    for (int i = 2; i < Q; i++) sum = sum + eq[i];
+                    eq[0] = rho - sum;
+                    for (int i = 0; i < Q; i++) f[i] = f[i] - ti * (f[i] - eq[i]);
+                } else if (Q == 9) bgk9<T>(f, rho, ex, ey, ti, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                 else bgk19<T>(f, rho, ex, ey, ez, ti, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
                 for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
                 wrote[x] = 1;

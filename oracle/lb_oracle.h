@@ -57,6 +57,7 @@ typedef struct {
     int bb_full;            /* 1: full-way bounce-back at solid nodes */
     int zou_he;             /* 1: reference Zou-He inlet (x=0) / outlet (x=nx-1) */
     float zh_ux_in, zh_rho_out;
+    int rest_closure;       /* 1: feq_0 = rho - sum_{i>0} feq_i */
 } orc_cfg;
 
 /* ---- shared numerics -------------------------------------------------- */

@@ -29,13 +29,13 @@ class OrcCfg(C.Structure):
                 ("bc_u", (C.c_float * 3) * 6), ("bc_rho", C.c_float * 6), ("force", C.c_float * 3),
                 ("sc_G", C.c_float), ("sc_rho0", C.c_float), ("sc_psi_solid", C.c_float),
                 ("bb_full", C.c_int), ("zou_he", C.c_int), ("zh_ux_in", C.c_float),
-                ("zh_rho_out", C.c_float)]
+                ("zh_rho_out", C.c_float), ("rest_closure", C.c_int)]
 
     @classmethod
     def from_dict(cls, d):
         c = cls()
         for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid",
-                  "bb_full", "zou_he", "zh_ux_in", "zh_rho_out"):
+                  "bb_full", "zou_he", "zh_ux_in", "zh_rho_out", "rest_closure"):
             setattr(c, k, d[k])
         for f in range(6):
             c.bc[f] = d["bc"][f]
