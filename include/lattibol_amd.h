@@ -177,7 +177,9 @@ int lb_refsim_data_d3q19(lb_refsim *s, int dev, DataD3Q19 *out);
 /* ---------------------------------------------------------------------- */
 enum { LB_BC_PERIODIC = 0, LB_BC_WALL = 1, LB_BC_MOVING_WALL = 2, LB_BC_INLET_EQ = 3,
        LB_BC_OUTFLOW = 4 };
-enum { LB_STORE_F32 = 0, LB_STORE_F16 = 1, LB_STORE_BF16 = 2, LB_STORE_F64 = 3 };
+/* F32 / F64 store f_i and follow the reference expressions; F32S / F16 / BF16
+ * store s_i = f_i - w_i and run the collision in deviation form (FP32). */
+enum { LB_STORE_F32 = 0, LB_STORE_F16 = 1, LB_STORE_BF16 = 2, LB_STORE_F64 = 3, LB_STORE_F32S = 4 };
 
 /* Faces are 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z in storage index space (D2Q9 rows
  * grow downward as in the reference, physical y = -row).  Populations use
@@ -185,7 +187,7 @@ enum { LB_STORE_F32 = 0, LB_STORE_F16 = 1, LB_STORE_BF16 = 2, LB_STORE_F64 = 3 }
 typedef struct {
     int lattice;            /* 9 or 19 */
     int nx, ny, nz;         /* nz = 1 for D2Q9 */
-    int storage;            /* LB_STORE_*; FP16/BF16 store f_i - w_i */
+    int storage;            /* LB_STORE_* */
     float tau_inv;
     int bc[6];
     float bc_u[6][3];       /* physical wall / inlet velocity */
@@ -204,6 +206,8 @@ typedef struct {
     int graph_chunk;        /* steps per captured hipGraph; 0 = default */
     int rest_closure;       /* 1: feq_0 = rho - sum_{i>0} feq_i (exact mass balance);
                                0: the reference's equilibrium for every direction */
+    int mem_mode;           /* population loads/stores: -1 autotune, 0 plain,
+                               1 non-temporal stores, 2 non-temporal loads + stores */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;
@@ -260,8 +264,9 @@ typedef struct {
 typedef struct lb_batch lb_batch;
 /* Walls (half-way bounce-back) at rows 0 and ny-1, Zou-He velocity inlet
  * u = (u_in, 0) at x = 0 and Zou-He pressure outlet rho = 1 at x = nx-1 (the
- * reference's inlet/outlet, collision.cu:211-262), rest closure on; solid
- * masks packed back to back. */
+ * reference's inlet/outlet, collision.cu:211-262), rest closure on, FP32
+ * deviation storage (LB_STORE_F32S); solid masks packed back to back.
+ * lb_batch_get_populations returns the stored deviations f_i - w_i. */
 int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t *solids, lb_batch **out);
 int lb_batch_destroy(lb_batch *b);
 /* Start from rest (rho 1, u 0) or from device fields packed like the masks
@@ -270,7 +275,8 @@ int lb_batch_init_rest(lb_batch *b);
 int lb_batch_init_fields_device(lb_batch *b, const float *d_fields);
 int lb_batch_step(lb_batch *b, int nsteps);
 /* Run to residual < tol (checked every k steps) or max_steps; per lattice
- * steps taken and final residual.  Converged lattices retire. */
+ * steps taken and final residual.  Converged lattices retire; a lattice whose
+ * |u| exceeds 0.5 or turns non-finite retires with residual +inf. */
 int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *steps_out,
                            double *res_out);
 /* Macros (rho, ux, uy) of lattice i to host. */

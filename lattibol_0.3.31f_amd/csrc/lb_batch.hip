@@ -9,6 +9,7 @@
 //   Zou-He velocity inlet u = (u_in, 0) at x = 0 and pressure outlet rho = 1
 //   at x = nx-1 (the reference's pair, collision.cu:211-262), rows 0 and ny-1
 //   half-way bounce-back walls, solids half-way bounce-back, rest closure,
+//   FP32 deviation storage (s_i = f_i - w_i, collision in deviation form),
 // so a lattice of the batch is bit-identical to lb_create() with that
 // configuration.
 #include <hip/hip_runtime.h>
@@ -40,7 +41,7 @@ struct BatchLat {
 inline lb_config lattice_config(int nx, int ny, float ti, float u_in) {
     lb_config c{};
     c.lattice = 9; c.nx = nx; c.ny = ny; c.nz = 1;
-    c.storage = LB_STORE_F32;
+    c.storage = LB_STORE_F32S;
     c.tau_inv = ti;
     c.bc[0] = LB_BC_OUTFLOW; c.bc[1] = LB_BC_OUTFLOW; c.bc[2] = LB_BC_WALL; c.bc[3] = LB_BC_WALL;
     c.zou_he = 1; c.zh_ux_in = u_in; c.zh_rho_out = 1.0f;
@@ -49,9 +50,9 @@ inline lb_config lattice_config(int nx, int ny, float ti, float u_in) {
 }
 
 // Argument block of lattice L reading buffer `src` (A = 0 / B = 1).
-StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags, float *macros) {
+StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags, float *macros) {
     const lb_config c = lattice_config(L.nx, L.ny, L.ti, L.u_in);
-    StepArgs<StF32> a{};
+    StepArgs<StF32S> a{};
     float *S0 = src == 0 ? A : B, *S1 = src == 0 ? B : A;
     a.A = S0 + L.pop_off;
     a.B = S1 + L.pop_off;
@@ -76,14 +77,16 @@ StepArgs<StF32> lattice_args(const BatchLat &L, float *A, float *B, int src, con
 }
 
 template <bool MACROS>
-__global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32> *__restrict__ args,
+__global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32S> *__restrict__ args,
                                                          const int2 *__restrict__ table, int src) {
     const int2 e = table[blockIdx.x];          // (lattice, local tile)
-    const StepArgs<StF32> &a = args[2 * e.x + src];
-    step_body<9, StF32, false, MACROS>(a, node_of(a, e.y), nullptr);
+    const StepArgs<StF32S> &a = args[2 * e.x + src];
+    step_body<9, StF32S, false, MACROS>(a, node_of(a, e.y), nullptr);
 }
 
-// Per-tile residual partials of the velocity change since the snapshot.
+// Per-tile residual partials of the velocity change since the snapshot, plus
+// the largest |u| (a diverging lattice can hold huge, slowly changing values
+// that the relative residual alone would call converged).
 __global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lats, const int2 *table,
                                                              const float *macros, float *snap,
                                                              const uint8_t *flags, double *partial) {
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lat
     const BatchLat L = lats[e.x];
     int tx = e.y % L.ntx, ty = e.y / L.ntx;
     int x = tx * kBX + (threadIdx.x & (kBX - 1)), y = ty * kBY + (threadIdx.x >> 6);
-    double num = 0, den = 0;
+    double num = 0, den = 0, umax = 0;
     if (x < L.nx && y < L.ny) {
         uint32_t id = (uint32_t)x + (uint32_t)L.nx * (uint32_t)y;
         if (!(flags[L.node_off + id] & F_SOLID)) {
@@ -102,30 +105,42 @@ __global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lat
                 double s = sn[(size_t)c * L.N + id];
                 num += (u - s) * (u - s);
                 den += u * u;
+                double au = fabs(u);
+                umax = (au <= 1e30) ? fmax(umax, au) : INFINITY;   // NaN / inf -> inf
                 sn[(size_t)c * L.N + id] = (float)u;
             }
         }
     }
-    __shared__ double sh[2][kThreads / 64];
+    __shared__ double sh[3][kThreads / 64];
     double s0 = wave_sum(num), s1 = wave_sum(den);
+    double s2 = umax;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s2 = fmax(s2, __shfl_xor(s2, off, 64));
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { sh[0][w] = s0; sh[1][w] = s1; }
+    if (lane == 0) { sh[0][w] = s0; sh[1][w] = s1; sh[2][w] = s2; }
     __syncthreads();
-    if (threadIdx.x < 2) {
+    if (threadIdx.x < 3) {
         double t = 0;
-        for (int k = 0; k < kThreads / 64; k++) t += sh[threadIdx.x][k];
-        partial[(size_t)blockIdx.x * 2 + threadIdx.x] = t;
+        for (int k = 0; k < kThreads / 64; k++) t = threadIdx.x < 2 ? t + sh[threadIdx.x][k] : fmax(t, sh[2][k]);
+        partial[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
     }
 }
 
-// One thread per active lattice: fixed-order sum over its tiles.
+// One thread per active lattice: fixed-order sum over its tiles.  A lattice
+// whose |u| exceeds kUDiverged (Mach ~0.9) or turns non-finite reports +inf.
+constexpr double kUDiverged = 0.5;
 __global__ void k_batch_final(const int2 *ranges, int nact, const double *partial, double *res) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nact) return;
     int2 r = ranges[k];   // (first block, count)
-    double num = 0, den = 0;
-    for (int b = r.x; b < r.x + r.y; b++) { num += partial[2 * b]; den += partial[2 * b + 1]; }
-    res[k] = den > 0 ? sqrt(num / den) : 0.0;
+    double num = 0, den = 0, umax = 0;
+    for (int b = r.x; b < r.x + r.y; b++) {
+        num += partial[3 * b];
+        den += partial[3 * b + 1];
+        umax = fmax(umax, partial[3 * b + 2]);
+    }
+    double v = den > 0 ? sqrt(num / den) : 0.0;
+    res[k] = (umax > kUDiverged || !(v <= 1e300)) ? INFINITY : v;
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, const int2 *table, float *A, float *B,
@@ -142,7 +157,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, c
         rho = F[id]; ux = F[L.N + id]; uy = F[2 * (size_t)L.N + id];
     }
     float f[9];
-    feq9<float>(f, rho, ux, uy, weight<9, float>(0), weight<9, float>(1), weight<9, float>(2));
+    feq_dev<9, float>(f, rho - 1.0f, ux, uy, 0.0f);
     for (int i = 0; i < 9; i++) {
         A[L.pop_off + (size_t)i * L.N + id] = f[i];
         B[L.pop_off + (size_t)i * L.N + id] = f[i];
@@ -161,7 +176,7 @@ struct lb_batch {
     uint64_t pop_total = 0, node_total = 0;
     hipStream_t stream = nullptr;
     BatchLat *d_lats = nullptr;
-    StepArgs<StF32> *d_args = nullptr;   // 2 per lattice (source buffer A / B)
+    StepArgs<StF32S> *d_args = nullptr;   // 2 per lattice (source buffer A / B)
     int2 *d_table = nullptr, *d_ranges = nullptr;
     float *A = nullptr, *B = nullptr, *macros = nullptr, *snap = nullptr;
     uint8_t *flags = nullptr;
@@ -256,7 +271,7 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     b->macros_valid.assign(n, 0);
     int rc = (int)hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (!rc) rc = (int)hipMalloc((void **)&b->d_lats, n * sizeof(BatchLat));
-    if (!rc) rc = (int)hipMalloc((void **)&b->d_args, 2 * n * sizeof(StepArgs<StF32>));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_args, 2 * n * sizeof(StepArgs<StF32S>));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_table, total_tiles * sizeof(int2));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_ranges, n * sizeof(int2));
     if (!rc) rc = (int)hipMalloc((void **)&b->A, pop * sizeof(float));
@@ -264,14 +279,14 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     if (!rc) rc = (int)hipMalloc((void **)&b->macros, 4 * node * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&b->snap, 2 * node * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&b->flags, node);
-    if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 2 * sizeof(double));
+    if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 3 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
     if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
-    std::vector<StepArgs<StF32>> hargs(2 * (size_t)n);
+    std::vector<StepArgs<StF32S>> hargs(2 * (size_t)n);
     if (!rc)
         for (int i = 0; i < n; i++)
             for (int src = 0; src < 2; src++) hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->macros);
-    if (!rc) rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32>), hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32S>), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemsetAsync(b->macros, 0, 4 * node * sizeof(float), b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);
@@ -363,8 +378,9 @@ extern "C" int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_st
             steps[i] += k;
             b->cur[i] = b->parity;
             b->macros_valid[i] = 1;
-            res[i] = b->snap_valid[i] ? hres[a] : 1.0;
-            if (b->snap_valid[i] && !(hres[a] >= tol)) {   // converged (or NaN: retire and report)
+            const bool diverged = std::isinf(hres[a]);
+            res[i] = (b->snap_valid[i] || diverged) ? hres[a] : 1.0;
+            if (diverged || (b->snap_valid[i] && hres[a] < tol)) {   // converged, or diverged (+inf)
                 b->active[i] = 0;
                 changed = true;
             }

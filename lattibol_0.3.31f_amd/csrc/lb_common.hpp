@@ -109,25 +109,40 @@ LB_DEV float bf16_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_
 LB_DEV uint16_t f32_to_f16(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
 LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// DEV = false: the population itself is stored and the arithmetic follows the
+// reference expressions.  DEV = true: the deviation s_i = f_i - w_i is stored
+// and the whole collision runs on deviations (moments, equilibria, boundary
+// closures), which keeps FP32 rounding noise relative to the deviation, not
+// to w_i (see the deviation-form functions below).
 struct StF32 {
     typedef float raw; typedef float T;
+    static constexpr bool DEV = false;
     static LB_DEV T ld(raw v, float) { return v; }
     static LB_DEV raw st(T f, float) { return f; }
 };
 struct StF64 {
     typedef double raw; typedef double T;
+    static constexpr bool DEV = false;
     static LB_DEV T ld(raw v, float) { return v; }
     static LB_DEV raw st(T f, float) { return f; }
 };
+struct StF32S {
+    typedef float raw; typedef float T;
+    static constexpr bool DEV = true;
+    static LB_DEV T ld(raw v, float) { return v; }
+    static LB_DEV raw st(T s, float) { return s; }
+};
 struct StF16 {
     typedef uint16_t raw; typedef float T;
-    static LB_DEV T ld(raw v, float w) { return f16_to_f32(v) + w; }
-    static LB_DEV raw st(T f, float w) { return f32_to_f16(f - w); }
+    static constexpr bool DEV = true;
+    static LB_DEV T ld(raw v, float) { return f16_to_f32(v); }
+    static LB_DEV raw st(T s, float) { return f32_to_f16(s); }
 };
 struct StBF16 {
     typedef uint16_t raw; typedef float T;
-    static LB_DEV T ld(raw v, float w) { return bf16_to_f32(v) + w; }
-    static LB_DEV raw st(T f, float w) { return f32_to_bf16(f - w); }
+    static constexpr bool DEV = true;
+    static LB_DEV T ld(raw v, float) { return bf16_to_f32(v); }
+    static LB_DEV raw st(T s, float) { return f32_to_bf16(s); }
 };
 
 // ---------------------------------------------------------------------------
@@ -322,6 +337,99 @@ template <class T> LB_DEV void zouhe19_out(T *f, float rho_out) {
     double nz = (double)(f[5] + f[11] + f[15] - f[6] - f[16] - f[18]);
     f[14] = (T)((double)f[9] - (1. / 6.) * (double)ru + 0.5 * nz);
     f[13] = (T)((double)f[10] - (1. / 6.) * (double)ru - 0.5 * nz);
+}
+
+// ---------------------------------------------------------------------------
+// Deviation form (s_i = f_i - w_i).  With drho = rho - 1:
+//   drho = sum s_i,  rho u = sum s_i c_i  (the w_i c_i cancel pairwise),
+//   seq_i = w_i (drho + rho g_i),  g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2,
+//   s_i <- s_i - (1/tau) (s_i - seq_i),
+// i.e. the same BGK as the reference written for deviations.  The summation
+// orders mirror the population form.
+// ---------------------------------------------------------------------------
+template <class T> LB_DEV void geq9(T *g, T ux, T uy) {
+    T uxs = ux * ux, uys = uy * uy;
+    T r15 = T(1.5) * (uxs + uys);
+    T s5 = +ux + uy, s6 = -ux + uy;
+    const T three = T(3), c45 = T(4.5);
+    g[0] = -r15;
+    g[1] = three * ux + c45 * uxs - r15;
+    g[2] = three * uy + c45 * uys - r15;
+    g[3] = -three * ux + c45 * uxs - r15;
+    g[4] = -three * uy + c45 * uys - r15;
+    g[5] = three * s5 + c45 * s5 * s5 - r15;
+    g[6] = three * s6 + c45 * s6 * s6 - r15;
+    g[7] = -three * s5 + c45 * s5 * s5 - r15;
+    g[8] = -three * s6 + c45 * s6 * s6 - r15;
+}
+template <class T> LB_DEV void geq19(T *g, T ux, T uy, T uz) {
+    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
+    T r15 = T(1.5) * (uxsq + uysq + uzsq);
+    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
+    const T three = T(3), c45 = T(4.5);
+    g[0] = -r15;
+    g[1] = three * ux + c45 * uxsq - r15;
+    g[2] = -three * ux + c45 * uxsq - r15;
+    g[3] = three * uy + c45 * uysq - r15;
+    g[4] = -three * uy + c45 * uysq - r15;
+    g[5] = three * uz + c45 * uzsq - r15;
+    g[6] = -three * uz + c45 * uzsq - r15;
+    g[7] = three * s7 + c45 * s7 * s7 - r15;
+    g[8] = three * s8 + c45 * s8 * s8 - r15;
+    g[9] = three * s9 + c45 * s9 * s9 - r15;
+    g[10] = three * s10 + c45 * s10 * s10 - r15;
+    g[11] = -three * s8 + c45 * s8 * s8 - r15;
+    g[12] = -three * s7 + c45 * s7 * s7 - r15;
+    g[13] = -three * s10 + c45 * s10 * s10 - r15;
+    g[14] = -three * s9 + c45 * s9 * s9 - r15;
+    g[15] = three * s15 + c45 * s15 * s15 - r15;
+    g[16] = three * s16 + c45 * s16 * s16 - r15;
+    g[17] = -three * s16 + c45 * s16 * s16 - r15;
+    g[18] = -three * s15 + c45 * s15 * s15 - r15;
+}
+
+// Zou-He in deviation form (the known-population weights sum to one, the
+// paired weights cancel); z correction with f17 (the reference's Q1 slip is
+// kept only in the population form).
+template <class T> LB_DEV void zouhe9_in_dev(T *s, T ux_in) {
+    T S = (s[0] + s[2] + s[4]) + T(2) * (s[3] + s[7] + s[6]);
+    T drho = (S + ux_in) / (T(1) - ux_in);
+    T ru = (T(1) + drho) * ux_in;
+    s[1] = s[3] + (T(2) / T(3)) * ru;
+    s[5] = s[7] + (T(1) / T(6)) * ru - T(0.5) * (s[2] - s[4]);
+    s[8] = s[6] + (T(1) / T(6)) * ru - T(0.5) * (s[4] - s[2]);
+}
+template <class T> LB_DEV void zouhe9_out_dev(T *s, T rho_out) {
+    T S = (s[0] + s[2] + s[4]) + T(2) * (s[1] + s[5] + s[8]);
+    T ru = (T(1) + S) - rho_out;
+    s[3] = s[1] - (T(2) / T(3)) * ru;
+    s[7] = s[5] - (T(1) / T(6)) * ru + T(0.5) * (s[2] - s[4]);
+    s[6] = s[8] - (T(1) / T(6)) * ru + T(0.5) * (s[4] - s[2]);
+}
+template <class T> LB_DEV void zouhe19_in_dev(T *s, T ux_in) {
+    T S = (s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18]) +
+          T(2) * (s[2] + s[11] + s[12] + s[13] + s[14]);
+    T drho = (S + ux_in) / (T(1) - ux_in);
+    T ru = (T(1) + drho) * ux_in;
+    T ny = s[3] + s[15] + s[16] - s[4] - s[17] - s[18];
+    T nz = s[5] + s[15] + s[17] - s[6] - s[16] - s[18];
+    s[1] = s[2] + (T(1) / T(3)) * ru;
+    s[8] = s[11] + (T(1) / T(6)) * ru + T(0.5) * ny;
+    s[7] = s[12] + (T(1) / T(6)) * ru - T(0.5) * ny;
+    s[9] = s[14] + (T(1) / T(6)) * ru - T(0.5) * nz;
+    s[10] = s[13] + (T(1) / T(6)) * ru + T(0.5) * nz;
+}
+template <class T> LB_DEV void zouhe19_out_dev(T *s, T rho_out) {
+    T S = (s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18]) +
+          T(2) * (s[1] + s[7] + s[8] + s[9] + s[10]);
+    T ru = (T(1) + S) - rho_out;
+    T ny = s[3] + s[15] + s[16] - s[4] - s[17] - s[18];
+    T nz = s[5] + s[15] + s[17] - s[6] - s[16] - s[18];
+    s[2] = s[1] - (T(1) / T(3)) * ru;
+    s[11] = s[8] - (T(1) / T(6)) * ru - T(0.5) * ny;
+    s[12] = s[7] - (T(1) / T(6)) * ru + T(0.5) * ny;
+    s[14] = s[9] - (T(1) / T(6)) * ru + T(0.5) * nz;
+    s[13] = s[10] - (T(1) / T(6)) * ru - T(0.5) * nz;
 }
 
 // ---------------------------------------------------------------------------

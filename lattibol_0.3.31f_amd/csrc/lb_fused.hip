@@ -24,7 +24,7 @@ namespace {
 // The argument block lives in device memory (uniform pointer -> scalar loads);
 // passed by value it would be copied to per-lane scratch because the device
 // functions index its arrays.
-template <int Q, class S, bool SC, bool MACROS>
+template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
 __global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const StepArgs<S> &a = *pa;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict
         load_psi_tile<Q, S>(a, b, (typename S::T *)lds_raw, 1 << a.bx_shift, a.by);
         __syncthreads();
     }
-    step_body<Q, S, SC, MACROS>(a, p, lds_raw);
+    step_body<Q, S, SC, MACROS, MEM>(a, p, lds_raw);
 }
 
 template <int Q, class S>
@@ -52,7 +52,8 @@ __global__ __launch_bounds__(kThreads) void k_init_eq(typename S::raw *A, typena
     uint32_t id = blockIdx.x * kThreads + threadIdx.x;
     if (id >= N) return;
     T f[Q];
-    feq<Q, T>(f, (T)rho[id], (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
+    if constexpr (S::DEV) feq_dev<Q, T>(f, (T)rho[id] - T(1), (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
+    else feq<Q, T>(f, (T)rho[id], (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         auto v = S::st(f[i], weight<Q, float>(Vel<Q>::wc(i)));
@@ -72,6 +73,7 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const typename S::raw *A, c
         static_for<0, Q>([&](auto I) {
             constexpr int i = decltype(I)::value;
             double v = (double)S::ld(A[(size_t)i * N + id], weight<Q, float>(L::wc(i)));
+            if constexpr (S::DEV) v += (double)weight<Q, float>(L::wc(i));
             m += v;
             if constexpr (L::cx(i) > 0) px += v; else if constexpr (L::cx(i) < 0) px -= v;
             if constexpr (L::pcy(i) > 0) py += v; else if constexpr (L::pcy(i) < 0) py -= v;
@@ -181,6 +183,7 @@ struct lb_ctx {
     int graph_steps = 0;
     int bx_shift = 6, by = 4;
     int xcd = 0;
+    int mem = 0;                      // population load/store hints (see ldm/stm)
     int corr_lo[3] = {0, 0, 0}, corr_hi[3] = {0, 0, 0};
     void *dargs = nullptr;            // two argument blocks (source buffer 0 / 1)
     static constexpr size_t kArgBytes = 512;
@@ -245,6 +248,7 @@ template <class S> int upload_args_t(lb_ctx *c) {
 int upload_args(lb_ctx *c) {
     switch (c->cfg.storage) {
     case LB_STORE_F32: return upload_args_t<StF32>(c);
+    case LB_STORE_F32S: return upload_args_t<StF32S>(c);
     case LB_STORE_F16: return upload_args_t<StF16>(c);
     case LB_STORE_BF16: return upload_args_t<StBF16>(c);
     default: return upload_args_t<StF64>(c);
@@ -266,6 +270,8 @@ int launch_step_t(lb_ctx *c, int src, bool macros) {
         else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa);
     } else {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, false, false, 1>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, false, false, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
         else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
     }
     return (int)hipGetLastError();
@@ -274,6 +280,7 @@ int launch_step_t(lb_ctx *c, int src, bool macros) {
 template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros) {
     switch (c->cfg.storage) {
     case LB_STORE_F32: return launch_step_t<Q, StF32>(c, src, macros);
+    case LB_STORE_F32S: return launch_step_t<Q, StF32S>(c, src, macros);
     case LB_STORE_F16: return launch_step_t<Q, StF16>(c, src, macros);
     case LB_STORE_BF16: return launch_step_t<Q, StBF16>(c, src, macros);
     default: return launch_step_t<Q, StF64>(c, src, macros);
@@ -301,6 +308,7 @@ template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
 template <int Q> int launch_reduce_q(lb_ctx *c) {
     switch (c->cfg.storage) {
     case LB_STORE_F32: return launch_reduce_t<Q, StF32>(c);
+    case LB_STORE_F32S: return launch_reduce_t<Q, StF32S>(c);
     case LB_STORE_F16: return launch_reduce_t<Q, StF16>(c);
     case LB_STORE_BF16: return launch_reduce_t<Q, StBF16>(c);
     default: return launch_reduce_t<Q, StF64>(c);
@@ -315,6 +323,7 @@ template <int Q, class S> int launch_init_t(lb_ctx *c, const float *r, const flo
 template <int Q> int launch_init_q(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
     switch (c->cfg.storage) {
     case LB_STORE_F32: return launch_init_t<Q, StF32>(c, r, ux, uy, uz);
+    case LB_STORE_F32S: return launch_init_t<Q, StF32S>(c, r, ux, uy, uz);
     case LB_STORE_F16: return launch_init_t<Q, StF16>(c, r, ux, uy, uz);
     case LB_STORE_BF16: return launch_init_t<Q, StBF16>(c, r, ux, uy, uz);
     default: return launch_init_t<Q, StF64>(c, r, ux, uy, uz);
@@ -364,24 +373,27 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
 // Launch-shape autotuner: times each candidate (block shape x XCD mapping)
 // over a few launches on the real lattice and keeps the fastest.
 int autotune(lb_ctx *c) {
-    struct Cand { int bx, by, xcd; };
+    struct Cand { int bx, by, xcd, mem; };
     std::vector<Cand> cands;
     const int bxs[4] = {32, 64, 128, 256};
     for (int bx : bxs) {
         if (bx > 2 * c->cfg.nx && bx != 32) continue;
         for (int xcd = 0; xcd < 2; xcd++) {
             if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
-            cands.push_back({bx, kThreads / bx, xcd});
+            for (int mem = 0; mem < 3; mem++) {
+                if (c->cfg.mem_mode >= 0 && mem != c->cfg.mem_mode) continue;
+                cands.push_back({bx, kThreads / bx, xcd, mem});
+            }
         }
     }
-    if (cands.empty()) cands.push_back({64, 4, 0});
+    if (cands.empty()) cands.push_back({64, 4, 0, 0});
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
     float best = 1e30f;
     Cand bestc = cands[0];
     const int reps = 6;
     for (const Cand &k : cands) {
-        c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd;
+        c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem;
         if (int rc = upload_args(c)) return rc;
         launch_step(c, 0, false);   // warm
         (void)hipEventRecord(e0, c->stream);
@@ -394,7 +406,7 @@ int autotune(lb_ctx *c) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd;
+    c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd; c->mem = bestc.mem;
     int rc = (int)hipGetLastError();
     if (!rc) rc = upload_args(c);
     return rc;
@@ -409,20 +421,21 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (cfg->lattice != 9 && cfg->lattice != 19) return LB_ERR_INVALID;
     if (cfg->nx < 2 || cfg->ny < 1 || cfg->nz < 1) return LB_ERR_INVALID;
     if (cfg->lattice == 9 && cfg->nz != 1) return LB_ERR_INVALID;
-    if (cfg->storage < 0 || cfg->storage > 3) return LB_ERR_INVALID;
+    if (cfg->storage < 0 || cfg->storage > 4) return LB_ERR_INVALID;
     for (int k = 0; k < 6; k++) if (cfg->bc[k] < 0 || cfg->bc[k] > 4) return LB_ERR_INVALID;
     uint64_t N64 = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (N64 * (uint64_t)cfg->lattice >= (1ull << 31)) return LB_ERR_UNSUPPORTED;   // 32-bit node indices
     if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
+    if (cfg->mem_mode > 2) return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
     c->cfg = *cfg;
     c->Q = cfg->lattice;
     c->N = (uint32_t)N64;
     c->P = (uint32_t)cfg->nx * cfg->ny;
-    c->esz = cfg->storage == LB_STORE_F32 ? 4 : (cfg->storage == LB_STORE_F64 ? 8 : 2);
+    c->esz = (cfg->storage == LB_STORE_F32 || cfg->storage == LB_STORE_F32S) ? 4 : (cfg->storage == LB_STORE_F64 ? 8 : 2);
     c->graph_steps = cfg->graph_chunk > 0 ? cfg->graph_chunk : 64;
     c->solid.assign(c->N, 0);
     face_corrections(*cfg, c->corr_lo, c->corr_hi);
@@ -454,6 +467,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->bx_shift = ilog2(cfg->block_x);
             c->by = std::max(1, kThreads / cfg->block_x);
             c->xcd = cfg->xcd_remap > 0 ? 1 : 0;
+            c->mem = cfg->mem_mode > 0 ? cfg->mem_mode : 0;
             rc = upload_args(c);
         } else {
             rc = autotune(c);
@@ -464,6 +478,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.block_x = 1 << c->bx_shift;
     c->cfg.block_y = c->by;
     c->cfg.xcd_remap = c->xcd;
+    c->cfg.mem_mode = c->mem;
     c->cfg.graph_chunk = c->graph_steps;
     *out = c;
     return 0;

@@ -83,8 +83,19 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     return p;
 }
 
+// Population loads / stores: MEM 0 plain, 1 non-temporal stores, 2 non-
+// temporal loads and stores (streaming hints; chosen by the autotuner).
+template <int MEM, class R> LB_DEV R ldm(const R *p) {
+    if constexpr (MEM >= 2) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <int MEM, class R> LB_DEV void stm(R *p, R v) {
+    if constexpr (MEM >= 1) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // Pull all Q populations of a fluid node into f (compute type).
-template <int Q, class S>
+template <int Q, class S, int MEM = 0>
 LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typename S::T *f) {
     typedef typename S::T T;
     typedef Vel<Q> L;
@@ -101,7 +112,7 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typena
                        (cz > 0 ? loz : (cz < 0 ? hiz : 0));
             int off = cx + cy * a.nx + cz * (int)a.nxy;
             uint32_t src = (uint32_t)((int)p.id - off + corr);
-            f[i] = S::ld(a.A[(size_t)i * a.N + src], weight<Q, float>(L::wc(i)));
+            f[i] = S::ld(ldm<MEM>(&a.A[(size_t)i * a.N + src]), weight<Q, float>(L::wc(i)));
         });
         return;
     }
@@ -158,7 +169,7 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typena
         } else if (Q == 19 && ghost == 2) {
             v = S::ld(a.recv[(size_t)zslot(i) * a.P + sxy], weight<Q, float>(L::wc(i)));
         } else {
-            v = S::ld(a.A[(size_t)i * a.N + sid], weight<Q, float>(L::wc(i)));
+            v = S::ld(ldm<MEM>(&a.A[(size_t)i * a.N + sid]), weight<Q, float>(L::wc(i)));
         }
         f[i] = v;
     });
@@ -169,14 +180,14 @@ template <int Q, class T> LB_DEV void moments(const T *f, T &rho, T &ux, T &uy, 
     else mom19(f, rho, ux, uy, uz);
 }
 
-template <int Q, class S>
+template <int Q, class S, int MEM = 0>
 LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S::T *f, bool reversed) {
     typedef Vel<Q> L;
     // reversed (full-way bounce-back at a solid node): slot opp(i) <- f_i
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         auto v = S::st(f[i], weight<Q, float>(L::wc(i)));
-        a.B[(size_t)(reversed ? L::opp(i) : i) * a.N + p.id] = v;
+        stm<MEM>(&a.B[(size_t)(reversed ? L::opp(i) : i) * a.N + p.id], v);
     });
     if constexpr (Q == 19) {
         if (a.halo && (p.z == a.nz - 1 || p.z == 0)) {
@@ -222,6 +233,51 @@ LB_DEV void collide(T *f, T rho, T ux, T uy, T uz, T ti, bool rest_closure) {
     });
 }
 
+// Deviation form: moments, equilibrium deviations and relaxation.
+template <int Q, class T> LB_DEV void moments_dev(const T *s, T &drho, T &ux, T &uy, T &uz) {
+    T rho;
+    if constexpr (Q == 9) {
+        drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8];
+        rho = T(1) + drho;
+        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) / rho;
+        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) / rho;
+        uz = T(0);
+    } else {
+        drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8] + s[9] + s[10] + s[11] +
+               s[12] + s[13] + s[14] + s[15] + s[16] + s[17] + s[18];
+        rho = T(1) + drho;
+        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) / rho;
+        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) / rho;
+        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) / rho;
+    }
+}
+
+template <int Q, class T> LB_DEV void feq_dev(T *seq, T drho, T ux, T uy, T uz) {
+    T g[Q];
+    if constexpr (Q == 9) geq9<T>(g, ux, uy);
+    else geq19<T>(g, ux, uy, uz);
+    const T rho = T(1) + drho;
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        seq[i] = weight<Q, T>(Vel<Q>::wc(i)) * (drho + rho * g[i]);
+    });
+}
+
+template <int Q, class T>
+LB_DEV void collide_dev(T *s, T drho, T ux, T uy, T uz, T ti, bool rest_closure) {
+    T eq[Q];
+    feq_dev<Q, T>(eq, drho, ux, uy, uz);
+    if (rest_closure) {
+        T sum = eq[1];
+        static_for<2, Q>([&](auto I) { sum = sum + eq[decltype(I)::value]; });
+        eq[0] = drho - sum;
+    }
+    static_for<0, Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        s[i] = s[i] - ti * (s[i] - eq[i]);
+    });
+}
+
 // Shan-Chen psi tile in LDS: (BX+2) x (BY+2) x 3 planes around the block.
 template <int Q, class S>
 LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int bx, int by) {
@@ -254,10 +310,23 @@ LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int b
 
 // ---------------------------------------------------------------------------
 // The fused step.
+// Zou-He inlet (x = 0) / outlet (x = nx-1) in the form matching the storage.
+template <int Q, class S> LB_DEV void apply_zou_he(const StepArgs<S> &a, const NodePos &p, typename S::T *f) {
+    typedef typename S::T T;
+    if (!a.zou_he) return;
+    if (p.x == 0) {
+        if constexpr (S::DEV) { if constexpr (Q == 9) zouhe9_in_dev<T>(f, (T)a.zh_ux); else zouhe19_in_dev<T>(f, (T)a.zh_ux); }
+        else { if constexpr (Q == 9) zouhe9_in(f, a.zh_ux); else zouhe19_in(f, a.zh_ux); }
+    } else if (p.x == a.nx - 1) {
+        if constexpr (S::DEV) { if constexpr (Q == 9) zouhe9_out_dev<T>(f, (T)a.zh_rho); else zouhe19_out_dev<T>(f, (T)a.zh_rho); }
+        else { if constexpr (Q == 9) zouhe9_out(f, a.zh_rho); else zouhe19_out(f, a.zh_rho); }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The body of the fused step for one node.  lds_raw holds the psi tile when
 // SC is set (loaded by the caller, all threads, before a barrier).
-template <int Q, class S, bool SC, bool MACROS>
+template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
 LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned char *lds_raw) {
     typedef typename S::T T;
     typedef Vel<Q> L;
@@ -267,8 +336,8 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
     T f[Q];
     if (fl & F_SOLID) {
         if (a.bb_full) {
-            pull_node<Q, S>(a, p, F_SLOW, f);
-            store_node<Q, S>(a, p, f, true);
+            pull_node<Q, S, MEM>(a, p, F_SLOW, f);
+            store_node<Q, S, MEM>(a, p, f, true);
         }
         if (MACROS) {
             a.macros[p.id] = 0.f; a.macros[a.N + p.id] = 0.f;
@@ -279,21 +348,24 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
     if (fl & F_INLET) {
         int face = fl >> 4;
         T rho = (T)a.bcrho[face], ux = (T)a.bcu[face][0], uy = (T)a.bcu[face][1], uz = (T)a.bcu[face][2];
-        feq<Q, T>(f, rho, ux, uy, uz);
-        store_node<Q, S>(a, p, f, false);
+        if constexpr (S::DEV) feq_dev<Q, T>(f, rho - T(1), ux, uy, uz);
+        else feq<Q, T>(f, rho, ux, uy, uz);
+        store_node<Q, S, MEM>(a, p, f, false);
         if (MACROS) {
             a.macros[p.id] = (float)rho; a.macros[a.N + p.id] = (float)ux;
             a.macros[2 * a.N + p.id] = (float)uy; a.macros[3 * a.N + p.id] = (float)uz;
         }
         return;
     }
-    pull_node<Q, S>(a, p, fl, f);
-    if (a.zou_he) {
-        if (p.x == 0) { if constexpr (Q == 9) zouhe9_in(f, a.zh_ux); else zouhe19_in(f, a.zh_ux); }
-        else if (p.x == a.nx - 1) { if constexpr (Q == 9) zouhe9_out(f, a.zh_rho); else zouhe19_out(f, a.zh_rho); }
+    pull_node<Q, S, MEM>(a, p, fl, f);
+    apply_zou_he<Q, S>(a, p, f);
+    T rho, ux, uy, uz, drho = T(0);
+    if constexpr (S::DEV) {
+        moments_dev<Q, T>(f, drho, ux, uy, uz);
+        rho = T(1) + drho;
+    } else {
+        moments<Q, T>(f, rho, ux, uy, uz);
     }
-    T rho, ux, uy, uz;
-    moments<Q, T>(f, rho, ux, uy, uz);
     if (MACROS) {
         a.macros[p.id] = (float)rho; a.macros[a.N + p.id] = (float)ux;
         a.macros[2 * a.N + p.id] = (float)uy; a.macros[3 * a.N + p.id] = (float)uz;
@@ -323,8 +395,9 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
         ey = ey + (gp * IF[1]) / (ti * rho);
         if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
     }
-    collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
-    store_node<Q, S>(a, p, f, false);
+    if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
+    else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
+    store_node<Q, S, MEM>(a, p, f, false);
 }
 
 // Shan-Chen density pre-pass: psi(rho) of the post-streaming density, with
@@ -341,12 +414,15 @@ LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
     } else {
         T f[Q];
         pull_node<Q, S>(a, p, fl, f);
-        if (a.zou_he) {
-            if (p.x == 0) { if constexpr (Q == 9) zouhe9_in(f, a.zh_ux); else zouhe19_in(f, a.zh_ux); }
-            else if (p.x == a.nx - 1) { if constexpr (Q == 9) zouhe9_out(f, a.zh_rho); else zouhe19_out(f, a.zh_rho); }
-        }
+        apply_zou_he<Q, S>(a, p, f);
         T ux, uy, uz;
-        moments<Q, T>(f, rho, ux, uy, uz);
+        if constexpr (S::DEV) {
+            T drho;
+            moments_dev<Q, T>(f, drho, ux, uy, uz);
+            rho = T(1) + drho;
+        } else {
+            moments<Q, T>(f, rho, ux, uy, uz);
+        }
     }
     a.psi[p.id] = sc_psi<T>(rho, a.scrho0);
 }

@@ -118,10 +118,11 @@ def ensemble(n=512, seed=1234, size=None, start=0):
     return [channel_spec(seed, start + i, size) for i in range(n)]
 
 
-def channel_config(spec, storage="f32", **kw):
+def channel_config(spec, storage="f32s", **kw):
     """Single-lattice configuration equal to one member of the batch: the
     reference's Zou-He velocity inlet (u_in, 0) at x = 0 and pressure outlet
-    rho = 1 at x = nx-1, half-way bounce-back walls on rows 0 and ny-1."""
+    rho = 1 at x = nx-1, half-way bounce-back walls on rows 0 and ny-1, FP32
+    deviation storage (s_i = f_i - w_i, collision in deviation form)."""
     cfg = make_config(9, (spec["nx"], spec["ny"]), tau=spec["tau"], storage=storage,
                       bc={"-x": "outflow", "+x": "outflow", "-y": "wall", "+y": "wall"},
                       zou_he=True, zh_ux_in=spec["u_in"], zh_rho_out=1.0, rest_closure=True, **kw)

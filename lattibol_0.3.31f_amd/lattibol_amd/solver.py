@@ -12,8 +12,8 @@ from . import _lib
 from ._lib import LbConfig, check, lib
 
 BC = {"periodic": 0, "wall": 1, "moving_wall": 2, "inlet": 3, "outflow": 4}
-STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3}
-RAW_DTYPE = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64}
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3, "f32s": 4}
+RAW_DTYPE = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64, 4: np.float32}
 FACES = ("-x", "+x", "-y", "+y", "-z", "+z")
 
 # Lattice weights as float32 (simulation_d2q9.cu:56-58, simulation_d3q19.cu:54-56).
@@ -24,7 +24,7 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
                 zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
-                rest_closure=None):
+                rest_closure=None, mem_mode=-1):
     """Plain-dict description of a lattice; the oracle binding reads the same dict.
 
     rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
@@ -53,13 +53,15 @@ def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, 
                 sc_G=float(sc_G), sc_rho0=float(sc_rho0), sc_psi_solid=float(sc_psi_solid),
                 bb_full=int(bool(bb_full)), zou_he=int(bool(zou_he)), zh_ux_in=float(zh_ux_in),
                 zh_rho_out=float(zh_rho_out), halo_z=int(bool(halo_z)), block_x=int(block_x),
-                xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)))
+                xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)),
+                mem_mode=int(mem_mode))
 
 
 def to_lbconfig(d):
     c = LbConfig()
     for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid", "bb_full",
-              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure"):
+              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure",
+              "mem_mode"):
         setattr(c, k, d[k])
     for f in range(6):
         c.bc[f] = d["bc"][f]
@@ -114,7 +116,8 @@ class Solver:
     def launch_config(self):
         c = LbConfig()
         check(lib.lb_get_config(self._h, C.byref(c)), "lb_get_config")
-        return dict(block_x=c.block_x, block_y=c.block_y, xcd_remap=c.xcd_remap, graph_chunk=c.graph_chunk)
+        return dict(block_x=c.block_x, block_y=c.block_y, xcd_remap=c.xcd_remap, mem_mode=c.mem_mode,
+                    graph_chunk=c.graph_chunk)
 
     def set_solid(self, solid):
         s = np.ascontiguousarray(np.asarray(solid, dtype=np.uint8).reshape(self.shape))
@@ -194,16 +197,18 @@ class Solver:
 
 
 def decode_populations(raw, storage, lattice):
-    """Stored populations -> float64 (adds w_i back for shifted 16-bit storage)."""
+    """Stored populations -> float64 (adds w_i back for the deviation storages)."""
     st = STORAGE[storage] if isinstance(storage, str) else storage
     w = (W9 if lattice == 9 else W19).astype(np.float32)
     if st == 0:
         return raw.astype(np.float64)
     if st == 3:
         return raw.astype(np.float64)
-    if st == 1:
+    if st == 4:
+        v = raw.astype(np.float32)
+    elif st == 1:
         v = raw.view(np.float16).astype(np.float32)
     else:
         v = (raw.astype(np.uint32) << 16).view(np.float32)
     shp = (-1,) + (1,) * (raw.ndim - 1)
-    return (v + w.reshape(shp)).astype(np.float64)
+    return v.astype(np.float64) + w.astype(np.float64).reshape(shp)

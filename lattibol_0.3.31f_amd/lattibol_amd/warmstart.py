@@ -68,8 +68,11 @@ def dataset(n, seed, size=(128, 128), tol=1e-6, k=100, max_steps=300000):
     rho = np.stack([m["rho"] for m in out["macros"]])
     ux = np.stack([m["ux"] for m in out["macros"]])
     uy = np.stack([m["uy"] for m in out["macros"]])
+    t = make_targets(rho, ux, uy, u_in)
+    fluid = (1 - solid).astype(bool)
+    ok = ok & np.array([np.isfinite(t[i][fluid[i]]).all() for i in range(len(specs))])
     return dict(specs=specs, converged=ok, steps=out["steps"], wall=out["wall"],
-                x=make_inputs(solid, u_in, tau), t=make_targets(rho, ux, uy, u_in),
+                x=make_inputs(solid, u_in, tau), t=np.where(np.isfinite(t), t, 0).astype(np.float32),
                 mask=(1 - solid).astype(np.float32), u_in=u_in, tau=tau)
 
 

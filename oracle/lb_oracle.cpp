@@ -169,14 +169,17 @@ float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
  * Storage policies for the pull specification.  T is the compute type.
  * Shifted storage keeps f_i - w_i (w_i the float lattice weight).
  * ---------------------------------------------------------------------- */
-struct StF32 { typedef float raw; typedef float T;
+struct StF32 { typedef float raw; typedef float T; static const bool DEV = false;
     static T ld(raw v, float) { return v; } static raw st(T f, float) { return f; } };
-struct StF64 { typedef double raw; typedef double T;
+struct StF64 { typedef double raw; typedef double T; static const bool DEV = false;
     static T ld(raw v, float) { return v; } static raw st(T f, float) { return f; } };
-struct StF16 { typedef uint16_t raw; typedef float T;
-    static T ld(raw v, float w) { return h2f(v) + w; } static raw st(T f, float w) { return f2h(f - w); } };
-struct StBF16 { typedef uint16_t raw; typedef float T;
-    static T ld(raw v, float w) { return bf2f(v) + w; } static raw st(T f, float w) { return f2bf(f - w); } };
+/* deviation storages: values are s_i = f_i - w_i and stay deviations */
+struct StF32S { typedef float raw; typedef float T; static const bool DEV = true;
+    static T ld(raw v, float) { return v; } static raw st(T s, float) { return s; } };
+struct StF16 { typedef uint16_t raw; typedef float T; static const bool DEV = true;
+    static T ld(raw v, float) { return h2f(v); } static raw st(T s, float) { return f2h(s); } };
+struct StBF16 { typedef uint16_t raw; typedef float T; static const bool DEV = true;
+    static T ld(raw v, float) { return bf2f(v); } static raw st(T s, float) { return f2bf(s); } };
 
 /* Lattice weights in the compute type: float weights for float compute
  * (as the reference), exact double quotients for the FP64 accuracy run. */
@@ -361,6 +364,90 @@ inline void zouhe19_out(T *f, float rho_out) {
     f[13] = (T)((double)f[10] - (1. / 6.) * (double)ru - 0.5 * nz);
 }
 
+/* ---- deviation form: s_i = f_i - w_i, drho = rho - 1 ---------------------
+ * drho = sum s_i, rho u = sum s_i c_i, seq_i = w_i (drho + rho g_i) with
+ * g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2, s_i <- s_i - (1/tau)(s_i - seq_i).
+ * Zou-He closures follow because the known weights sum to one and paired
+ * weights cancel; the D3Q19 z correction uses f17. */
+template <class T> void geq9(T *g, T ux, T uy) {
+    T uxs = ux * ux, uys = uy * uy;
+    T r15 = T(1.5) * (uxs + uys);
+    T s5 = +ux + uy, s6 = -ux + uy;
+    g[0] = -r15;
+    g[1] = T(3) * ux + T(4.5) * uxs - r15;
+    g[2] = T(3) * uy + T(4.5) * uys - r15;
+    g[3] = -T(3) * ux + T(4.5) * uxs - r15;
+    g[4] = -T(3) * uy + T(4.5) * uys - r15;
+    g[5] = T(3) * s5 + T(4.5) * s5 * s5 - r15;
+    g[6] = T(3) * s6 + T(4.5) * s6 * s6 - r15;
+    g[7] = -T(3) * s5 + T(4.5) * s5 * s5 - r15;
+    g[8] = -T(3) * s6 + T(4.5) * s6 * s6 - r15;
+}
+template <class T> void geq19(T *g, T ux, T uy, T uz) {
+    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
+    T r15 = T(1.5) * (uxsq + uysq + uzsq);
+    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
+    g[0] = -r15;
+    g[1] = T(3) * ux + T(4.5) * uxsq - r15;
+    g[2] = -T(3) * ux + T(4.5) * uxsq - r15;
+    g[3] = T(3) * uy + T(4.5) * uysq - r15;
+    g[4] = -T(3) * uy + T(4.5) * uysq - r15;
+    g[5] = T(3) * uz + T(4.5) * uzsq - r15;
+    g[6] = -T(3) * uz + T(4.5) * uzsq - r15;
+    g[7] = T(3) * s7 + T(4.5) * s7 * s7 - r15;
+    g[8] = T(3) * s8 + T(4.5) * s8 * s8 - r15;
+    g[9] = T(3) * s9 + T(4.5) * s9 * s9 - r15;
+    g[10] = T(3) * s10 + T(4.5) * s10 * s10 - r15;
+    g[11] = -T(3) * s8 + T(4.5) * s8 * s8 - r15;
+    g[12] = -T(3) * s7 + T(4.5) * s7 * s7 - r15;
+    g[13] = -T(3) * s10 + T(4.5) * s10 * s10 - r15;
+    g[14] = -T(3) * s9 + T(4.5) * s9 * s9 - r15;
+    g[15] = T(3) * s15 + T(4.5) * s15 * s15 - r15;
+    g[16] = T(3) * s16 + T(4.5) * s16 * s16 - r15;
+    g[17] = -T(3) * s16 + T(4.5) * s16 * s16 - r15;
+    g[18] = -T(3) * s15 + T(4.5) * s15 * s15 - r15;
+}
+template <class T> void zouhe9_in_dev(T *s, T u) {
+    T S = (s[0] + s[2] + s[4]) + T(2) * (s[3] + s[7] + s[6]);
+    T drho = (S + u) / (T(1) - u);
+    T ru = (T(1) + drho) * u;
+    s[1] = s[3] + (T(2) / T(3)) * ru;
+    s[5] = s[7] + (T(1) / T(6)) * ru - T(0.5) * (s[2] - s[4]);
+    s[8] = s[6] + (T(1) / T(6)) * ru - T(0.5) * (s[4] - s[2]);
+}
+template <class T> void zouhe9_out_dev(T *s, T rho_out) {
+    T S = (s[0] + s[2] + s[4]) + T(2) * (s[1] + s[5] + s[8]);
+    T ru = (T(1) + S) - rho_out;
+    s[3] = s[1] - (T(2) / T(3)) * ru;
+    s[7] = s[5] - (T(1) / T(6)) * ru + T(0.5) * (s[2] - s[4]);
+    s[6] = s[8] - (T(1) / T(6)) * ru + T(0.5) * (s[4] - s[2]);
+}
+template <class T> void zouhe19_in_dev(T *s, T u) {
+    T S = (s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18]) +
+          T(2) * (s[2] + s[11] + s[12] + s[13] + s[14]);
+    T drho = (S + u) / (T(1) - u);
+    T ru = (T(1) + drho) * u;
+    T ny = s[3] + s[15] + s[16] - s[4] - s[17] - s[18];
+    T nz = s[5] + s[15] + s[17] - s[6] - s[16] - s[18];
+    s[1] = s[2] + (T(1) / T(3)) * ru;
+    s[8] = s[11] + (T(1) / T(6)) * ru + T(0.5) * ny;
+    s[7] = s[12] + (T(1) / T(6)) * ru - T(0.5) * ny;
+    s[9] = s[14] + (T(1) / T(6)) * ru - T(0.5) * nz;
+    s[10] = s[13] + (T(1) / T(6)) * ru + T(0.5) * nz;
+}
+template <class T> void zouhe19_out_dev(T *s, T rho_out) {
+    T S = (s[0] + s[3] + s[4] + s[5] + s[6] + s[15] + s[16] + s[17] + s[18]) +
+          T(2) * (s[1] + s[7] + s[8] + s[9] + s[10]);
+    T ru = (T(1) + S) - rho_out;
+    T ny = s[3] + s[15] + s[16] - s[4] - s[17] - s[18];
+    T nz = s[5] + s[15] + s[17] - s[6] - s[16] - s[18];
+    s[2] = s[1] - (T(1) / T(3)) * ru;
+    s[11] = s[8] - (T(1) / T(6)) * ru - T(0.5) * ny;
+    s[12] = s[7] - (T(1) / T(6)) * ru + T(0.5) * ny;
+    s[14] = s[9] - (T(1) / T(6)) * ru + T(0.5) * nz;
+    s[13] = s[10] - (T(1) / T(6)) * ru - T(0.5) * nz;
+}
+
 inline int wrapi(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
 /* ========================================================================
@@ -452,6 +539,47 @@ inline void moments(const T *f, T &rho, T &ux, T &uy, T &uz) {
     else mom19(f, rho, ux, uy, uz);
 }
 
+/* Deviation-form moments and equilibrium (storages with S::DEV). */
+template <int Q, class T>
+inline void moments_dev(const T *s, T &drho, T &ux, T &uy, T &uz) {
+    if (Q == 9) {
+        drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8];
+        T rho = T(1) + drho;
+        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) / rho;
+        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) / rho;
+        uz = T(0);
+    } else {
+        drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8] + s[9] + s[10] + s[11] + s[12] +
+               s[13] + s[14] + s[15] + s[16] + s[17] + s[18];
+        T rho = T(1) + drho;
+        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) / rho;
+        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) / rho;
+        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) / rho;
+    }
+}
+template <int Q, class T>
+inline void feq_dev(T *seq, T drho, T ux, T uy, T uz) {
+    T g[19];
+    if (Q == 9) geq9<T>(g, ux, uy);
+    else geq19<T>(g, ux, uy, uz);
+    T rho = T(1) + drho;
+    for (int i = 0; i < Q; i++) seq[i] = wq<Q, T>(i) * (drho + rho * g[i]);
+}
+template <int Q, class S>
+inline void zou_he_any(const orc_cfg *cfg, int x, int nx, typename S::T *f) {
+    typedef typename S::T T;
+    if (!cfg->zou_he) return;
+    if (x == 0) {
+        if (S::DEV) { if (Q == 9) zouhe9_in_dev<T>(f, (T)cfg->zh_ux_in); else zouhe19_in_dev<T>(f, (T)cfg->zh_ux_in); }
+        else if (Q == 9) zouhe9_in(f, cfg->zh_ux_in);
+        else zouhe19_in(f, cfg->zh_ux_in);
+    } else if (x == nx - 1) {
+        if (S::DEV) { if (Q == 9) zouhe9_out_dev<T>(f, (T)cfg->zh_rho_out); else zouhe19_out_dev<T>(f, (T)cfg->zh_rho_out); }
+        else if (Q == 9) zouhe9_out(f, cfg->zh_rho_out);
+        else zouhe19_out(f, cfg->zh_rho_out);
+    }
+}
+
 /* Nodes away from every face whose neighbours are all fluid pull with plain
  * index offsets (the same values resolve_src would find). */
 template <int Q>
@@ -488,12 +616,15 @@ typename S::T node_density(const orc_cfg *cfg, const Geo &g, const typename S::r
     int inf = inlet_face(cfg, g, p, dims);
     if (inf >= 0) return (T)cfg->bc_rho[inf];
     pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
-    if (cfg->zou_he) {
-        if (p[0] == 0) { if (Q == 9) zouhe9_in(f, cfg->zh_ux_in); else zouhe19_in(f, cfg->zh_ux_in); }
-        else if (p[0] == g.n[0] - 1) { if (Q == 9) zouhe9_out(f, cfg->zh_rho_out); else zouhe19_out(f, cfg->zh_rho_out); }
-    }
+    zou_he_any<Q, S>(cfg, p[0], g.n[0], f);
     T rho, ux, uy, uz;
-    moments<Q, T>(f, rho, ux, uy, uz);
+    if (S::DEV) {
+        T drho;
+        moments_dev<Q, T>(f, drho, ux, uy, uz);
+        rho = T(1) + drho;
+    } else {
+        moments<Q, T>(f, rho, ux, uy, uz);
+    }
     return rho;
 }
 
@@ -568,7 +699,8 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                 if (inf >= 0) {
                     T rho = (T)cfg->bc_rho[inf], ux = (T)cfg->bc_u[inf][0], uy = (T)cfg->bc_u[inf][1],
                       uz = (T)cfg->bc_u[inf][2];
-                    if (Q == 9) feq9<T>(f, rho, ux, uy, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
+                    if (S::DEV) feq_dev<Q, T>(f, rho - T(1), ux, uy, uz);
+                    else if (Q == 9) feq9<T>(f, rho, ux, uy, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                     else feq19<T>(f, rho, ux, uy, uz, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
                     for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
                     wrote[x] = 1;
@@ -579,12 +711,14 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                 }
                 if (simple[id]) for (int i = 0; i < Q; i++) f[i] = rin[(size_t)i * nx + x];
                 else pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
-                if (cfg->zou_he) {
-                    if (x == 0) { if (Q == 9) zouhe9_in(f, cfg->zh_ux_in); else zouhe19_in(f, cfg->zh_ux_in); }
-                    else if (x == g.n[0] - 1) { if (Q == 9) zouhe9_out(f, cfg->zh_rho_out); else zouhe19_out(f, cfg->zh_rho_out); }
+                zou_he_any<Q, S>(cfg, x, g.n[0], f);
+                T rho, ux, uy, uz, drho = T(0);
+                if (S::DEV) {
+                    moments_dev<Q, T>(f, drho, ux, uy, uz);
+                    rho = T(1) + drho;
+                } else {
+                    moments<Q, T>(f, rho, ux, uy, uz);
                 }
-                T rho, ux, uy, uz;
-                moments<Q, T>(f, rho, ux, uy, uz);
                 if (macros) {
                     macros[id] = rho; macros[g.N + id] = ux; macros[2 * g.N + id] = uy; macros[3 * g.N + id] = uz;
                 }
@@ -614,7 +748,16 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     ey = ey + (gp * IF[1]) / (ti * rho);
                     if (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
                 }
-                if (cfg->rest_closure) {
+                if (S::DEV) {
+                    T eq[19];
+                    feq_dev<Q, T>(eq, drho, ex, ey, ez);
+                    if (cfg->rest_closure) {
+                        T sum = eq[1];
+                        for (int i = 2; i < Q; i++) sum = sum + eq[i];
+                        eq[0] = drho - sum;
+                    }
+                    for (int i = 0; i < Q; i++) f[i] = f[i] - ti * (f[i] - eq[i]);
+                } else if (cfg->rest_closure) {
                     /* equilibria first (same roundings as the BGK expressions),
                        rest population closes the mass balance */
                     T eq[19];
@@ -646,7 +789,8 @@ void init_eq_impl(const orc_cfg *cfg, const double *rho, const double *ux, const
     for (int64_t id = 0; id < g.N; id++) {
         T f[19];
         T r = (T)rho[id], a = (T)ux[id], b = (T)uy[id], c = uz ? (T)uz[id] : T(0);
-        if (Q == 9) feq9<T>(f, r, a, b, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
+        if (S::DEV) feq_dev<Q, T>(f, r - T(1), a, b, c);
+        else if (Q == 9) feq9<T>(f, r, a, b, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
         else feq19<T>(f, r, a, b, c, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
         for (int i = 0; i < Q; i++) A[(int64_t)i * g.N + id] = S::st(f[i], (float)wq<Q, float>(i));
     }
@@ -657,7 +801,8 @@ void decode_impl(const orc_cfg *cfg, const typename S::raw *A, double *out) {
     Geo g(cfg);
     for (int i = 0; i < Q; i++)
         for (int64_t id = 0; id < g.N; id++)
-            out[(int64_t)i * g.N + id] = (double)S::ld(A[(int64_t)i * g.N + id], (float)wq<Q, float>(i));
+            out[(int64_t)i * g.N + id] = (double)S::ld(A[(int64_t)i * g.N + id], (float)wq<Q, float>(i)) +
+                                         (S::DEV ? (double)wq<Q, float>(i) : 0.0);
 }
 
 template <int Q, class S>
@@ -667,7 +812,8 @@ void reduce_impl(const orc_cfg *cfg, const typename S::raw *A, const uint8_t *so
     for (int64_t id = 0; id < g.N; id++) {
         if (solid[id]) continue;
         for (int i = 0; i < Q; i++) {
-            long double v = (long double)S::ld(A[(int64_t)i * g.N + id], (float)wq<Q, float>(i));
+            long double v = (long double)S::ld(A[(int64_t)i * g.N + id], (float)wq<Q, float>(i)) +
+                            (S::DEV ? (long double)wq<Q, float>(i) : 0.0L);
             m += v;
             px += v * Lat<Q>::cphys(i, 0);
             py += v * Lat<Q>::cphys(i, 1);
@@ -685,6 +831,7 @@ void reduce_impl(const orc_cfg *cfg, const typename S::raw *A, const uint8_t *so
         if ((cfg)->lattice == 9) {                                                  \
             switch ((cfg)->storage) {                                               \
             case ORC_STORE_F32: { const int Q = 9; typedef StF32 S; CALL; } break;  \
+            case ORC_STORE_F32S: { const int Q = 9; typedef StF32S S; CALL; } break;\
             case ORC_STORE_F16: { const int Q = 9; typedef StF16 S; CALL; } break;  \
             case ORC_STORE_BF16: { const int Q = 9; typedef StBF16 S; CALL; } break;\
             default: { const int Q = 9; typedef StF64 S; CALL; } break;             \
@@ -692,6 +839,7 @@ void reduce_impl(const orc_cfg *cfg, const typename S::raw *A, const uint8_t *so
         } else {                                                                    \
             switch ((cfg)->storage) {                                               \
             case ORC_STORE_F32: { const int Q = 19; typedef StF32 S; CALL; } break; \
+            case ORC_STORE_F32S: { const int Q = 19; typedef StF32S S; CALL; } break;\
             case ORC_STORE_F16: { const int Q = 19; typedef StF16 S; CALL; } break; \
             case ORC_STORE_BF16: { const int Q = 19; typedef StBF16 S; CALL; } break;\
             default: { const int Q = 19; typedef StF64 S; CALL; } break;            \
@@ -1208,7 +1356,7 @@ void orc_pull_step(const orc_cfg *cfg, const void *A, void *B, const uint8_t *so
 }
 
 void orc_pull_run(const orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps, void *macros) {
-    size_t esz = cfg->storage == ORC_STORE_F32 ? 4 : (cfg->storage == ORC_STORE_F64 ? 8 : 2);
+    size_t esz = (cfg->storage == ORC_STORE_F32 || cfg->storage == ORC_STORE_F32S) ? 4 : (cfg->storage == ORC_STORE_F64 ? 8 : 2);
     size_t bytes = esz * (size_t)cfg->lattice * cfg->nx * cfg->ny * cfg->nz;
     void *a = A, *b = B;
     std::vector<uint8_t> simple;

@@ -40,7 +40,9 @@ extern "C" {
  * index space (D2Q9 rows grow downward, physical y = -row). */
 enum { ORC_BC_PERIODIC = 0, ORC_BC_WALL = 1, ORC_BC_MOVING_WALL = 2,
        ORC_BC_INLET_EQ = 3, ORC_BC_OUTFLOW = 4 };
-enum { ORC_STORE_F32 = 0, ORC_STORE_F16 = 1, ORC_STORE_BF16 = 2, ORC_STORE_F64 = 3 };
+/* F32/F64 store f_i (reference expressions); F16/BF16/F32S store
+ * s_i = f_i - w_i and collide in deviation form. */
+enum { ORC_STORE_F32 = 0, ORC_STORE_F16 = 1, ORC_STORE_BF16 = 2, ORC_STORE_F64 = 3, ORC_STORE_F32S = 4 };
 
 typedef struct {
     int lattice;            /* 9 (D2Q9) or 19 (D3Q19) */

@@ -89,7 +89,7 @@ for _n, (_r, _a) in _PROTOS.items():
     _fn.restype = _r
     _fn.argtypes = _a
 
-RAW = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64}
+RAW = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64, 4: np.float32}
 W9F = np.array([16 / 36] + [4 / 36] * 4 + [1 / 36] * 4, dtype=np.float32)
 W19F = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float32)
 

@@ -27,7 +27,7 @@ def _assert_pop_parity(gpu_raw, orc_raw, storage, fluid=None, rel=1e-5):
     g, o = gpu_raw, orc_raw
     if fluid is not None:
         g, o = g[:, fluid], o[:, fluid]
-    if storage in (1, 2):                      # 16-bit stored values
+    if storage in (1, 2):                      # 16-bit stored deviations
         np.testing.assert_array_equal(g, o)
         return
     nbad = int(np.count_nonzero(g != o))
@@ -138,12 +138,23 @@ FUSED_CASES = {
     "spinodal24_f16": ("spinodal", dict(n=24, storage="f16"), 300),
     "spinodal24_bf16": ("spinodal", dict(n=24, storage="bf16"), 300),
     "spinodal20_f64": ("spinodal", dict(n=20, storage="f64"), 300),
+    "cavity48_f32s": ("cavity", dict(n=48, storage="f32s"), 500),
+    "sphere_halo_f32s": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f32s"), 300),
+    "spinodal24_f32s": ("spinodal", dict(n=24, storage="f32s"), 300),
+    "channel_zouhe_f32s": ("channel", dict(seed=3, index=2, size=(128, 64)), 1000),
 }
+
+
+def _channel_scene(seed, index, size):
+    from lattibol_amd import scenes
+    spec = scenes.channel_spec(seed, index, size)
+    return scenes.channel_config(spec), spec["solid"][None], dict(rho=1.0, ux=0.0, uy=0.0, uz=0.0)
 
 
 def _scene(kind, kw):
     from lattibol_amd import scenes
-    return {"cavity": scenes.cavity, "sphere": scenes.sphere_channel, "spinodal": scenes.spinodal}[kind](**kw)
+    return {"cavity": scenes.cavity, "sphere": scenes.sphere_channel, "spinodal": scenes.spinodal,
+            "channel": _channel_scene}[kind](**kw)
 
 
 @pytest.mark.parametrize("case", sorted(FUSED_CASES))

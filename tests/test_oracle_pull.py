@@ -175,3 +175,72 @@ def test_shifted_storage_tracks_fp64(storage):
     tol = 2e-3 if storage == "f16" else 2e-2
     err = np.abs(o.macros()["ux"] - r.macros()["ux"]).max()
     assert err < tol * 0.1, err
+
+
+# ---------------------------------------------------------------------------
+# Deviation form (storages f32s / f16 / bf16): s_i = f_i - w_i
+# ---------------------------------------------------------------------------
+def _channel(size=(64, 64), u_in=0.03, tau=0.8, storage="f32s", seed=5):
+    spec = scenes.channel_spec(seed, 1, size)
+    spec["u_in"], spec["tau"] = u_in, tau
+    return spec, scenes.channel_config(spec, storage=storage)
+
+
+@pytest.mark.parametrize("Q", [9, 19])
+def test_deviation_zou_he_prescribes_inlet_velocity_and_outlet_density(Q):
+    if Q == 9:
+        spec, cfg = _channel((48, 32))
+        solid = spec["solid"]
+    else:
+        cfg = make_config(19, (24, 8, 6), tau=0.8, storage="f32s", bc={"-x": "outflow", "+x": "outflow"},
+                          zou_he=True, zh_ux_in=0.03, zh_rho_out=1.0)
+        solid = None
+    o = O.PullOracle(cfg, solid)
+    o.init_equilibrium(1.0, 0.0, 0.0)
+    o.run(400)
+    m = o.macros()
+    fl = o.solid == 0
+    inlet = fl[..., 0]
+    np.testing.assert_allclose(m["ux"][..., 0][inlet], cfg["zh_ux_in"], atol=2e-7)
+    outlet = fl[..., -1]
+    np.testing.assert_allclose(m["rho"][..., -1][outlet], 1.0, atol=2e-7)
+
+
+def test_deviation_form_matches_fp64_population_form():
+    spec, c32 = _channel(storage="f32s")
+    _, c64 = _channel(storage="f64")
+    a, b = O.PullOracle(c32, spec["solid"]), O.PullOracle(c64, spec["solid"])
+    for o in (a, b):
+        o.init_equilibrium(1.0, 0.0, 0.0)
+        o.run(3000)
+    fl = spec["solid"] == 0
+    du = np.abs(a.macros()["ux"][0][fl] - b.macros()["ux"][0][fl]).max()
+    umax = np.abs(b.macros()["ux"][0][fl]).max()
+    assert du < 2e-4 * umax, (du, umax)
+
+
+def test_deviation_form_conserves_mass_fp32():
+    cfg, solid, init = scenes.spinodal(n=16, storage="f32s")
+    o = O.PullOracle(cfg, solid)
+    o.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+    m0 = o.reduce()[0]
+    o.run(1000)
+    assert abs(o.reduce()[0] - m0) / m0 < 1e-6
+
+
+def test_deviation_form_lowers_fp32_residual_floor():
+    """The population form's rounding noise keeps ||du||/||u|| near 1e-6 at
+    steady state for small u_in; the deviation form sits well below."""
+    floors = {}
+    for st in ("f32", "f32s"):
+        spec, cfg = _channel(u_in=0.02, tau=0.9, storage=st)
+        o = O.PullOracle(cfg, spec["solid"])
+        o.init_equilibrium(1.0, 0.0, 0.0)
+        o.run(8000)
+        fl = spec["solid"] == 0
+        u0 = np.stack([o.macros()["ux"][0][fl], o.macros()["uy"][0][fl]]).astype(np.float64)
+        o.run(100)
+        u1 = np.stack([o.macros()["ux"][0][fl], o.macros()["uy"][0][fl]]).astype(np.float64)
+        floors[st] = np.linalg.norm(u1 - u0) / np.linalg.norm(u1)
+    assert floors["f32s"] < 5e-7, floors
+    assert floors["f32s"] < 0.5 * floors["f32"], floors
