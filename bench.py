@@ -28,7 +28,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16", "f64"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f32s", "f16", "bf16", "f64"])
+    ap.add_argument("--block-x", type=int, default=0, help="fix the sweep's block width (0 = autotune)")
+    ap.add_argument("--xcd-remap", type=int, default=-1, help="fix the XCD tile remap (0/1, -1 = autotune)")
+    ap.add_argument("--mem-mode", type=int, default=-1,
+                    help="fix the load/store mode (0 plain, 1 non-temporal stores, 2 non-temporal loads+stores)")
+    ap.add_argument("--layout", type=int, default=-1,
+                    help="fix the population layout (0 [i][z][y][x], 1 [z][y][i][x]; -1 = autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
@@ -236,15 +242,17 @@ def main():
     import lattibol_amd as lb
     from lattibol_amd import scenes
 
-    cfg, solid, init = scenes.sphere_channel(storage=args.storage)
+    cfg, solid, init = scenes.sphere_channel(storage=args.storage, block_x=args.block_x, xcd_remap=args.xcd_remap,
+                                             mem_mode=args.mem_mode, layout=args.layout)
     nodes = cfg["nx"] * cfg["ny"] * cfg["nz"]
-    esz = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[args.storage]
+    esz = {"f32": 4, "f32s": 4, "f16": 2, "bf16": 2, "f64": 8}[args.storage]
     bytes_per_update = 2 * 19 * esz
     print("bench: creating solver %dx%dx%d %s" % (cfg["nx"], cfg["ny"], cfg["nz"], args.storage), file=sys.stderr,
           flush=True)
     sol = lb.Solver(cfg, solid)
     sol.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
     launch = sol.launch_config
+    print("bench: launch %s" % json.dumps(launch), file=sys.stderr, flush=True)
     sol.step(args.warmup)
     sol.sync()
     t0 = time.perf_counter()
@@ -285,7 +293,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f32": "fp32", "f16": "fp16-storage/fp32-compute", "bf16": "bf16-storage/fp32-compute",
+        "dtype": {"f32": "fp32", "f32s": "fp32", "f16": "fp16-storage/fp32-compute", "bf16": "bf16-storage/fp32-compute",
                   "f64": "fp64"}[args.storage],
         "data": "synthetic",
         "config": {"workload": "D3Q19 BGK channel past a sphere 256x128x128 (r=16 at (64,64,64)), inlet u=0.05, "

@@ -180,6 +180,10 @@ enum { LB_BC_PERIODIC = 0, LB_BC_WALL = 1, LB_BC_MOVING_WALL = 2, LB_BC_INLET_EQ
 /* F32 / F64 store f_i and follow the reference expressions; F32S / F16 / BF16
  * store s_i = f_i - w_i and run the collision in deviation form (FP32). */
 enum { LB_STORE_F32 = 0, LB_STORE_F16 = 1, LB_STORE_BF16 = 2, LB_STORE_F64 = 3, LB_STORE_F32S = 4 };
+/* Device layout of the populations.  SOA: [i][z][y][x]; ROWS: [z][y][i][x]
+ * (the Q populations of one x-row back to back).  The host-side population
+ * API (lb_set_populations / lb_get_populations) always uses SOA order. */
+enum { LB_LAYOUT_SOA = 0, LB_LAYOUT_ROWS = 1 };
 
 /* Faces are 0:-x 1:+x 2:-y 3:+y 4:-z 5:+z in storage index space (D2Q9 rows
  * grow downward as in the reference, physical y = -row).  Populations use
@@ -208,6 +212,7 @@ typedef struct {
                                0: the reference's equilibrium for every direction */
     int mem_mode;           /* population loads/stores: -1 autotune, 0 plain,
                                1 non-temporal stores, 2 non-temporal loads + stores */
+    int layout;             /* -1 autotune, LB_LAYOUT_SOA, LB_LAYOUT_ROWS */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;
@@ -224,7 +229,8 @@ int lb_init_equilibrium(lb_ctx *ctx, const float *rho, const float *ux, const fl
 int lb_init_equilibrium_device(lb_ctx *ctx, const float *d_rho, const float *d_ux,
                                const float *d_uy, const float *d_uz);
 int lb_init_uniform(lb_ctx *ctx, float rho, float ux, float uy, float uz);
-/* Raw populations in the storage format (Q*N elements), host side. */
+/* Raw populations in the storage format (Q*N elements, [i][z][y][x] order
+ * whatever the device layout), host side. */
 int lb_set_populations(lb_ctx *ctx, const void *host);
 int lb_get_populations(lb_ctx *ctx, void *host);
 /* nsteps time steps; the last one also writes rho,u (the macros). */

@@ -50,13 +50,15 @@ inline lb_config lattice_config(int nx, int ny, float ti, float u_in) {
 }
 
 // Argument block of lattice L reading buffer `src` (A = 0 / B = 1).
-StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags, float *macros) {
+StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags,
+                              const uint32_t *lmask, float *macros) {
     const lb_config c = lattice_config(L.nx, L.ny, L.ti, L.u_in);
     StepArgs<StF32S> a{};
     float *S0 = src == 0 ? A : B, *S1 = src == 0 ? B : A;
     a.A = S0 + L.pop_off;
     a.B = S1 + L.pop_off;
     a.flags = flags + L.node_off;
+    a.lmask = lmask + L.node_off;
     a.macros = macros + 4 * L.node_off;
     a.N = L.N;
     a.nxy = L.N;
@@ -67,7 +69,9 @@ StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, co
         a.bc[k] = c.bc[k];
         a.bcrho[k] = c.bc_rho[k];
     }
-    face_corrections(c, a.corr_lo, a.corr_hi);
+    const PopLayout pl = pop_layout(LB_LAYOUT_SOA, 9, L.nx, L.ny, 1);
+    a.ps = pl.ps; a.sy = pl.sy; a.sz = pl.sz;
+    face_corrections(c, pl, a.corr_lo, a.corr_hi);
     a.zou_he = 1; a.zh_ux = c.zh_ux_in; a.zh_rho = c.zh_rho_out;
     a.rest_closure = 1;
     a.bx_shift = 6; a.by = kBY;
@@ -139,8 +143,10 @@ __global__ void k_batch_final(const int2 *ranges, int nact, const double *partia
         den += partial[3 * b + 1];
         umax = fmax(umax, partial[3 * b + 2]);
     }
+    // divergence is judged on the velocities alone (umax is +inf when any
+    // velocity is NaN or inf), never on the residual value itself
     double v = den > 0 ? sqrt(num / den) : 0.0;
-    res[k] = (umax > kUDiverged || !(v <= 1e300)) ? INFINITY : v;
+    res[k] = umax > kUDiverged ? INFINITY : v;
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, const int2 *table, float *A, float *B,
@@ -180,6 +186,7 @@ struct lb_batch {
     int2 *d_table = nullptr, *d_ranges = nullptr;
     float *A = nullptr, *B = nullptr, *macros = nullptr, *snap = nullptr;
     uint8_t *flags = nullptr;
+    uint32_t *lmask = nullptr;
     double *partial = nullptr, *d_res = nullptr;
     std::vector<int2> table, ranges;
     std::vector<int> act_list;
@@ -190,7 +197,7 @@ struct lb_batch {
 
     ~lb_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        void *ps[] = {d_lats, d_args, d_table, d_ranges, A, B, macros, snap, flags, partial, d_res};
+        void *ps[] = {d_lats, d_args, d_table, d_ranges, A, B, macros, snap, flags, lmask, partial, d_res};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -245,6 +252,7 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     uint64_t pop = 0, node = 0;
     size_t total_tiles = 0;
     std::vector<uint8_t> flags_all;
+    std::vector<uint32_t> lmask_all;
     for (int i = 0; i < n; i++) {
         const lb_batch_lattice &s = lat[i];
         if (s.nx < 2 || s.ny < 3 || (uint64_t)s.nx * s.ny * 9 >= (1ull << 31)) { delete b; return LB_ERR_INVALID; }
@@ -258,8 +266,10 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
         const lb_config cfg = lattice_config(s.nx, s.ny, s.tau_inv, s.u_in);
         std::vector<uint8_t> sol(solids + s.solid_offset, solids + s.solid_offset + L.N), fl;
         for (auto &v : sol) v = v ? 1 : 0;
-        build_flags(cfg, sol, fl);
+        std::vector<uint32_t> lm;
+        build_flags(cfg, sol, fl, lm);
         flags_all.insert(flags_all.end(), fl.begin(), fl.end());
+        lmask_all.insert(lmask_all.end(), lm.begin(), lm.end());
         pop += 9ull * L.N;
         node += L.N;
     }
@@ -279,15 +289,17 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     if (!rc) rc = (int)hipMalloc((void **)&b->macros, 4 * node * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&b->snap, 2 * node * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&b->flags, node);
+    if (!rc) rc = (int)hipMalloc((void **)&b->lmask, node * 4);
     if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 3 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
     if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
     std::vector<StepArgs<StF32S>> hargs(2 * (size_t)n);
     if (!rc)
         for (int i = 0; i < n; i++)
-            for (int src = 0; src < 2; src++) hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->macros);
+            for (int src = 0; src < 2; src++) hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->lmask, b->macros);
     if (!rc) rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32S>), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(b->lmask, lmask_all.data(), node * 4, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemsetAsync(b->macros, 0, 4 * node * sizeof(float), b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);
     if (!rc) rc = rebuild_table(b);
@@ -314,6 +326,9 @@ static int batch_init(lb_batch *b, const float *fields) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_init, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->A, b->B, fields);
     rc = (int)hipGetLastError();
+    // the first residual check only primes the snapshot, but its value must
+    // still be finite: start the snapshot from zero velocities
+    if (!rc) rc = (int)hipMemsetAsync(b->snap, 0, 2 * b->node_total * sizeof(float), b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);
     return rc;
 }

@@ -106,7 +106,14 @@ LB_DEV uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 LB_DEV float bf16_to_f32(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
-LB_DEV uint16_t f32_to_f16(float f) { return __builtin_bit_cast(uint16_t, (_Float16)f); }
+// The empty asm pins f as a rounded FP32 register value.  Without it the
+// compiler folds a producing multiply or FMA and this conversion into one
+// v_fma_mixlo_f16 (a single rounding straight to FP16), which differs from
+// FP32-then-FP16 rounding whenever the FP32 result sits on an FP16 tie.
+LB_DEV uint16_t f32_to_f16(float f) {
+    __asm__("" : "+v"(f));
+    return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
 LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
 // DEV = false: the population itself is stored and the arithmetic follows the

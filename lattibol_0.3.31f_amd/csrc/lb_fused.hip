@@ -1,9 +1,11 @@
 // lb_fused.hip -- the lb_* solver handle around the fused single-sweep pull
 // kernels of lb_pull.hpp (D2Q9 / D3Q19, A -> B double buffering).
 //
-// Data layout in HBM: population i of node n = x + nx (y + ny z) at
-// element i*N + n of one buffer per time level; 1-byte node flags; macro
-// planes rho,ux,uy,uz; Shan-Chen psi plane; z halo planes (send/recv).
+// Data layout in HBM: one population buffer per time level, either
+// [i][z][y][x] (SOA, element i*N + n for node n = x + nx (y + ny z)) or
+// [z][y][i][x] (ROWS, the Q populations of an x-row back to back), chosen by
+// the autotuner; 1-byte node flags; macro planes rho,ux,uy,uz; Shan-Chen psi
+// plane; z halo planes (send/recv).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,33 +48,36 @@ __global__ __launch_bounds__(kThreads) void k_density(const StepArgs<S> *__restr
 // Equilibrium initialisation from device fields.
 template <int Q, class S>
 __global__ __launch_bounds__(kThreads) void k_init_eq(typename S::raw *A, typename S::raw *B, uint32_t N,
+                                                      uint32_t nx, uint32_t ny, PopLayout pl,
                                                       const float *rho, const float *ux, const float *uy,
                                                       const float *uz) {
     typedef typename S::T T;
     uint32_t id = blockIdx.x * kThreads + threadIdx.x;
     if (id >= N) return;
+    const uint32_t e = elem_of(id, nx, ny, pl.sy, pl.sz);
     T f[Q];
     if constexpr (S::DEV) feq_dev<Q, T>(f, (T)rho[id] - T(1), (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
     else feq<Q, T>(f, (T)rho[id], (T)ux[id], (T)uy[id], uz ? (T)uz[id] : T(0));
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         auto v = S::st(f[i], weight<Q, float>(Vel<Q>::wc(i)));
-        A[(size_t)i * N + id] = v;
-        B[(size_t)i * N + id] = v;
+        A[(size_t)i * pl.ps + e] = v;
+        B[(size_t)i * pl.ps + e] = v;
     });
 }
 
 // FP64 totals of the populations over fluid nodes: per-block partials.
 template <int Q, class S>
 __global__ __launch_bounds__(kThreads) void k_reduce(const typename S::raw *A, const uint8_t *flags, uint32_t N,
-                                                     double *partial) {
+                                                     uint32_t nx, uint32_t ny, PopLayout pl, double *partial) {
     typedef Vel<Q> L;
     double m = 0, px = 0, py = 0, pz = 0;
     for (uint32_t id = blockIdx.x * kThreads + threadIdx.x; id < N; id += gridDim.x * kThreads) {
         if (flags[id] & F_SOLID) continue;
+        const uint32_t e = elem_of(id, nx, ny, pl.sy, pl.sz);
         static_for<0, Q>([&](auto I) {
             constexpr int i = decltype(I)::value;
-            double v = (double)S::ld(A[(size_t)i * N + id], weight<Q, float>(L::wc(i)));
+            double v = (double)S::ld(A[(size_t)i * pl.ps + e], weight<Q, float>(L::wc(i)));
             if constexpr (S::DEV) v += (double)weight<Q, float>(L::wc(i));
             m += v;
             if constexpr (L::cx(i) > 0) px += v; else if constexpr (L::cx(i) < 0) px -= v;
@@ -90,6 +95,37 @@ __global__ __launch_bounds__(kThreads) void k_reduce(const typename S::raw *A, c
         for (int k = 0; k < kThreads / 64; k++) s += sh[threadIdx.x][k];
         partial[blockIdx.x * 4 + threadIdx.x] = s;
     }
+}
+
+// Between the host order [i][z][y][x] and the device layout pl (raw copy).
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_relayout(const R *src, R *dst, uint32_t N, int Q, uint32_t nx,
+                                                       uint32_t ny, PopLayout pl, int to_device) {
+    uint32_t id = blockIdx.x * kThreads + threadIdx.x;
+    if (id >= N) return;
+    const uint32_t e = elem_of(id, nx, ny, pl.sy, pl.sz);
+    for (int i = 0; i < Q; i++) {
+        if (to_device) dst[(size_t)i * pl.ps + e] = src[(size_t)i * N + id];
+        else dst[(size_t)i * N + id] = src[(size_t)i * pl.ps + e];
+    }
+}
+
+// Fresh z halo from the populations in A: recv slots 0-4 <- the -z
+// populations of plane 0, slots 5-9 <- the +z populations of plane nz-1
+// (what the pack + copy after a sweep would leave there).
+template <class R>
+__global__ __launch_bounds__(kThreads) void k_fill_recv(const R *A, R *recv, uint32_t P, uint32_t nx, int nz,
+                                                        PopLayout pl) {
+    typedef Vel<19> L;
+    uint32_t xy = blockIdx.x * kThreads + threadIdx.x;
+    if (xy >= P) return;
+    const uint32_t y = xy / nx, x = xy - y * nx;
+    const uint32_t e0 = x + y * pl.sy, e1 = e0 + (uint32_t)(nz - 1) * pl.sz;
+    static_for<0, 19>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (L::cz(i) < 0) recv[(size_t)zslot(i) * P + xy] = A[(size_t)i * pl.ps + e0];
+        else if constexpr (L::cz(i) > 0) recv[(size_t)(5 + zslot(i)) * P + xy] = A[(size_t)i * pl.ps + e1];
+    });
 }
 
 // Residual accumulators: sum |u - u_snap|^2 and |u|^2 over fluid nodes,
@@ -170,6 +206,7 @@ struct lb_ctx {
     void *buf[2] = {nullptr, nullptr};
     int cur = 0;
     uint8_t *flags = nullptr;
+    uint32_t *lmask = nullptr;
     float *macros = nullptr;
     float *snap = nullptr;
     void *psi = nullptr;
@@ -184,16 +221,17 @@ struct lb_ctx {
     int bx_shift = 6, by = 4;
     int xcd = 0;
     int mem = 0;                      // population load/store hints (see ldm/stm)
-    int corr_lo[3] = {0, 0, 0}, corr_hi[3] = {0, 0, 0};
+    int layout = LB_LAYOUT_SOA;       // population layout (LB_LAYOUT_*)
     void *dargs = nullptr;            // two argument blocks (source buffer 0 / 1)
     static constexpr size_t kArgBytes = 512;
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
-        void *ps[] = {buf[0], buf[1], flags, macros, snap, psi, send, recv, partial, dout, tmp_fields, dargs};
+        void *ps[] = {buf[0], buf[1], flags, lmask, macros, snap, psi, send, recv, partial, dout, tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    PopLayout pl() const { return pop_layout(layout, Q, cfg.nx, cfg.ny, cfg.nz); }
     void drop_graphs() {
         for (auto &g : graph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
     }
@@ -208,11 +246,14 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     a.A = (const typename S::raw *)c->buf[src];
     a.B = (typename S::raw *)c->buf[1 - src];
     a.flags = c->flags;
+    a.lmask = c->lmask;
     a.macros = c->macros;
     a.psi = (typename S::T *)c->psi;
     a.send = (typename S::raw *)c->send;
     a.recv = (const typename S::raw *)c->recv;
     a.N = c->N;
+    const PopLayout pl = c->pl();
+    a.ps = pl.ps; a.sy = pl.sy; a.sz = pl.sz;
     a.nxy = (uint32_t)c->cfg.nx * (uint32_t)c->cfg.ny;
     a.P = c->P;
     a.nx = c->cfg.nx; a.ny = c->cfg.ny; a.nz = c->cfg.nz;
@@ -222,7 +263,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
         a.bcrho[k] = c->cfg.bc_rho[k];
         for (int d = 0; d < 3; d++) a.bcu[k][d] = c->cfg.bc_u[k][d];
     }
-    for (int d = 0; d < 3; d++) { a.corr_lo[d] = c->corr_lo[d]; a.corr_hi[d] = c->corr_hi[d]; a.force[d] = c->cfg.force[d]; }
+    face_corrections(c->cfg, pl, a.corr_lo, a.corr_hi);
+    for (int d = 0; d < 3; d++) a.force[d] = c->cfg.force[d];
     a.scG = c->cfg.sc_G; a.scrho0 = c->cfg.sc_rho0; a.scpsi_solid = c->cfg.sc_psi_solid;
     a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he; a.rest_closure = c->cfg.rest_closure;
     a.halo = c->cfg.halo_z && c->Q == 19;
@@ -301,7 +343,8 @@ int launch_step(lb_ctx *c, int src, bool macros) {
 
 template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
     hipLaunchKernelGGL((k_reduce<Q, S>), dim3(kReduceBlocks), dim3(kThreads), 0, c->stream,
-                       (const typename S::raw *)c->buf[c->cur], c->flags, c->N, c->partial);
+                       (const typename S::raw *)c->buf[c->cur], c->flags, c->N, (uint32_t)c->cfg.nx,
+                       (uint32_t)c->cfg.ny, c->pl(), c->partial);
     hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(64), 0, c->stream, c->partial, kReduceBlocks, 4, c->dout);
     return (int)hipGetLastError();
 }
@@ -317,7 +360,8 @@ template <int Q> int launch_reduce_q(lb_ctx *c) {
 
 template <int Q, class S> int launch_init_t(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
     hipLaunchKernelGGL((k_init_eq<Q, S>), dim3((c->N + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
-                       (typename S::raw *)c->buf[0], (typename S::raw *)c->buf[1], c->N, r, ux, uy, uz);
+                       (typename S::raw *)c->buf[0], (typename S::raw *)c->buf[1], c->N, (uint32_t)c->cfg.nx,
+                       (uint32_t)c->cfg.ny, c->pl(), r, ux, uy, uz);
     return (int)hipGetLastError();
 }
 template <int Q> int launch_init_q(lb_ctx *c, const float *r, const float *ux, const float *uy, const float *uz) {
@@ -330,10 +374,41 @@ template <int Q> int launch_init_q(lb_ctx *c, const float *r, const float *ux, c
     }
 }
 
+// Raw-element dispatch for the layout kernels (2, 4 or 8 byte elements).
+template <class F> int by_esz(int esz, F &&f) {
+    if (esz == 2) return f(uint16_t{});
+    if (esz == 4) return f(uint32_t{});
+    return f(uint64_t{});
+}
+
+// recv planes from buf[0] (after an initialisation or a population upload).
+int refill_recv(lb_ctx *c) {
+    if (!c->recv) return 0;
+    return by_esz(c->esz, [&](auto t) {
+        typedef decltype(t) R;
+        hipLaunchKernelGGL((k_fill_recv<R>), dim3((c->P + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
+                           (const R *)c->buf[0], (R *)c->recv, c->P, (uint32_t)c->cfg.nx, c->cfg.nz, c->pl());
+        return (int)hipGetLastError();
+    });
+}
+
+// Host-order populations <-> device layout through a temporary device buffer.
+int relayout(lb_ctx *c, const void *src, void *dst, bool to_device) {
+    return by_esz(c->esz, [&](auto t) {
+        typedef decltype(t) R;
+        hipLaunchKernelGGL((k_relayout<R>), dim3((c->N + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
+                           (const R *)src, (R *)dst, c->N, c->Q, (uint32_t)c->cfg.nx, (uint32_t)c->cfg.ny, c->pl(),
+                           to_device ? 1 : 0);
+        return (int)hipGetLastError();
+    });
+}
+
 int upload_flags(lb_ctx *c) {
     std::vector<uint8_t> fl;
-    build_flags(c->cfg, c->solid, fl);
+    std::vector<uint32_t> lm;
+    build_flags(c->cfg, c->solid, fl, lm);
     int rc = (int)hipMemcpyAsync(c->flags, fl.data(), c->N, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(c->lmask, lm.data(), (size_t)c->N * 4, hipMemcpyHostToDevice, c->stream);
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     return rc;
 }
@@ -370,30 +445,36 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
     return 0;
 }
 
-// Launch-shape autotuner: times each candidate (block shape x XCD mapping)
-// over a few launches on the real lattice and keeps the fastest.
+// Launch autotuner: times each candidate (block shape x XCD mapping x
+// load/store hints x population layout) over a few launches on the real
+// lattice and keeps the fastest.  Runs at creation, before any populations
+// are set, so switching the layout moves no data.
 int autotune(lb_ctx *c) {
-    struct Cand { int bx, by, xcd, mem; };
+    struct Cand { int bx, by, xcd, mem, layout; };
     std::vector<Cand> cands;
     const int bxs[4] = {32, 64, 128, 256};
     for (int bx : bxs) {
         if (bx > 2 * c->cfg.nx && bx != 32) continue;
+        if (c->cfg.block_x > 0 && bx != c->cfg.block_x) continue;
         for (int xcd = 0; xcd < 2; xcd++) {
             if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
             for (int mem = 0; mem < 3; mem++) {
                 if (c->cfg.mem_mode >= 0 && mem != c->cfg.mem_mode) continue;
-                cands.push_back({bx, kThreads / bx, xcd, mem});
+                for (int lay = 0; lay < 2; lay++) {
+                    if (c->cfg.layout >= 0 && lay != c->cfg.layout) continue;
+                    cands.push_back({bx, kThreads / bx, xcd, mem, lay});
+                }
             }
         }
     }
-    if (cands.empty()) cands.push_back({64, 4, 0, 0});
+    if (cands.empty()) cands.push_back({64, 4, 0, 0, 0});
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
     float best = 1e30f;
     Cand bestc = cands[0];
     const int reps = 6;
     for (const Cand &k : cands) {
-        c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem;
+        c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
         if (int rc = upload_args(c)) return rc;
         launch_step(c, 0, false);   // warm
         (void)hipEventRecord(e0, c->stream);
@@ -407,6 +488,7 @@ int autotune(lb_ctx *c) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd; c->mem = bestc.mem;
+    c->layout = bestc.layout;
     int rc = (int)hipGetLastError();
     if (!rc) rc = upload_args(c);
     return rc;
@@ -428,7 +510,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
-    if (cfg->mem_mode > 2) return LB_ERR_INVALID;
+    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS) return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
     c->cfg = *cfg;
@@ -438,12 +520,12 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->esz = (cfg->storage == LB_STORE_F32 || cfg->storage == LB_STORE_F32S) ? 4 : (cfg->storage == LB_STORE_F64 ? 8 : 2);
     c->graph_steps = cfg->graph_chunk > 0 ? cfg->graph_chunk : 64;
     c->solid.assign(c->N, 0);
-    face_corrections(*cfg, c->corr_lo, c->corr_hi);
     int rc = (int)hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     const size_t pop_bytes = (size_t)c->Q * c->N * c->esz;
     if (!rc) rc = (int)hipMalloc(&c->buf[0], pop_bytes);
     if (!rc) rc = (int)hipMalloc(&c->buf[1], pop_bytes);
     if (!rc) rc = (int)hipMalloc((void **)&c->flags, c->N);
+    if (!rc) rc = (int)hipMalloc((void **)&c->lmask, (size_t)c->N * 4);
     if (!rc) rc = (int)hipMalloc((void **)&c->macros, (size_t)4 * c->N * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&c->snap, (size_t)3 * c->N * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&c->partial, (size_t)kReduceBlocks * 4 * sizeof(double));
@@ -463,11 +545,12 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc && c->psi) rc = (int)hipMemsetAsync(c->psi, 0, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4), c->stream);
     if (!rc) rc = upload_flags(c);
     if (!rc) {
-        if (cfg->block_x > 0) {
+        if (cfg->block_x > 0 && cfg->xcd_remap >= 0 && cfg->mem_mode >= 0 && cfg->layout >= 0) {
             c->bx_shift = ilog2(cfg->block_x);
             c->by = std::max(1, kThreads / cfg->block_x);
-            c->xcd = cfg->xcd_remap > 0 ? 1 : 0;
-            c->mem = cfg->mem_mode > 0 ? cfg->mem_mode : 0;
+            c->xcd = cfg->xcd_remap;
+            c->mem = cfg->mem_mode;
+            c->layout = cfg->layout;
             rc = upload_args(c);
         } else {
             rc = autotune(c);
@@ -479,6 +562,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.block_y = c->by;
     c->cfg.xcd_remap = c->xcd;
     c->cfg.mem_mode = c->mem;
+    c->cfg.layout = c->layout;
     c->cfg.graph_chunk = c->graph_steps;
     *out = c;
     return 0;
@@ -518,21 +602,8 @@ extern "C" int lb_init_equilibrium_device(lb_ctx *c, const float *r, const float
         if (rc) return rc;
     }
     // A fresh halo: the planes the first sweep pulls across z come from the
-    // initial state, so pack them now (recv_hi <- plane 0 -z pops, recv_lo <-
-    // plane nz-1 +z pops), as the copy after a sweep would.
-    if (c->recv) {
-        const int minus[5] = {6, 10, 14, 16, 18}, plus[5] = {5, 9, 13, 15, 17};
-        const size_t pb = (size_t)c->P * c->esz;
-        for (int k = 0; k < 5 && !rc; k++) {
-            const char *A = (const char *)c->buf[0];
-            rc = (int)hipMemcpyAsync((char *)c->recv + k * pb, A + ((size_t)minus[k] * c->N) * c->esz, pb,
-                                     hipMemcpyDeviceToDevice, c->stream);
-            if (!rc)
-                rc = (int)hipMemcpyAsync((char *)c->recv + (5 + k) * pb,
-                                         A + ((size_t)plus[k] * c->N + (size_t)(c->cfg.nz - 1) * c->P) * c->esz, pb,
-                                         hipMemcpyDeviceToDevice, c->stream);
-        }
-    }
+    // initial state (as the copy after a sweep would leave them).
+    rc = refill_recv(c);
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     return rc;
 }
@@ -562,33 +633,41 @@ extern "C" int lb_init_uniform(lb_ctx *c, float rho, float ux, float uy, float u
 extern "C" int lb_set_populations(lb_ctx *c, const void *host) {
     if (!c || !host) return LB_ERR_INVALID;
     size_t bytes = (size_t)c->Q * c->N * c->esz;
-    int rc = (int)hipMemcpyAsync(c->buf[0], host, bytes, hipMemcpyHostToDevice, c->stream);
-    if (!rc) rc = (int)hipMemcpyAsync(c->buf[1], host, bytes, hipMemcpyHostToDevice, c->stream);
+    int rc = 0;
+    if (c->layout == LB_LAYOUT_SOA) {
+        rc = (int)hipMemcpyAsync(c->buf[0], host, bytes, hipMemcpyHostToDevice, c->stream);
+    } else {
+        void *tmp = nullptr;
+        rc = (int)hipMalloc(&tmp, bytes);
+        if (!rc) rc = (int)hipMemcpyAsync(tmp, host, bytes, hipMemcpyHostToDevice, c->stream);
+        if (!rc) rc = relayout(c, tmp, c->buf[0], true);
+        if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+        if (tmp) (void)hipFree(tmp);
+    }
+    if (!rc) rc = (int)hipMemcpyAsync(c->buf[1], c->buf[0], bytes, hipMemcpyDeviceToDevice, c->stream);
     c->cur = 0;
     c->steps = 0;
     c->macros_valid = false;
     c->snap_valid = false;
-    if (!rc && c->recv) {
-        const int minus[5] = {6, 10, 14, 16, 18}, plus[5] = {5, 9, 13, 15, 17};
-        const size_t pb = (size_t)c->P * c->esz;
-        for (int k = 0; k < 5 && !rc; k++) {
-            const char *A = (const char *)c->buf[0];
-            rc = (int)hipMemcpyAsync((char *)c->recv + k * pb, A + ((size_t)minus[k] * c->N) * c->esz, pb,
-                                     hipMemcpyDeviceToDevice, c->stream);
-            if (!rc)
-                rc = (int)hipMemcpyAsync((char *)c->recv + (5 + k) * pb,
-                                         A + ((size_t)plus[k] * c->N + (size_t)(c->cfg.nz - 1) * c->P) * c->esz, pb,
-                                         hipMemcpyDeviceToDevice, c->stream);
-        }
-    }
+    if (!rc) rc = refill_recv(c);
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     return rc;
 }
 
 extern "C" int lb_get_populations(lb_ctx *c, void *host) {
     if (!c || !host) return LB_ERR_INVALID;
-    int rc = (int)hipMemcpyAsync(host, c->buf[c->cur], (size_t)c->Q * c->N * c->esz, hipMemcpyDeviceToHost, c->stream);
+    size_t bytes = (size_t)c->Q * c->N * c->esz;
+    if (c->layout == LB_LAYOUT_SOA) {
+        int rc = (int)hipMemcpyAsync(host, c->buf[c->cur], bytes, hipMemcpyDeviceToHost, c->stream);
+        if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+        return rc;
+    }
+    void *tmp = nullptr;
+    int rc = (int)hipMalloc(&tmp, bytes);
+    if (!rc) rc = relayout(c, c->buf[c->cur], tmp, false);
+    if (!rc) rc = (int)hipMemcpyAsync(host, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
     return rc;
 }
 

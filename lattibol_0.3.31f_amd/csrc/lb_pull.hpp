@@ -31,11 +31,13 @@ template <class S> struct StepArgs {
     const raw *A;
     raw *B;
     const uint8_t *flags;
+    const uint32_t *lmask;  // link masks of F_SLOW nodes (see pull_node)
     float *macros;          // 4*N: rho, ux, uy, uz
     T *psi;                 // Shan-Chen pseudo-potential
     raw *send;              // [send_lo (5 planes, -z pops of z=0) | send_hi (+z pops of z=nz-1)]
     const raw *recv;        // [recv_hi (ghost z=nz) | recv_lo (ghost z=-1)]
     uint32_t N, nxy, P;     // nodes, nodes per z-plane (= halo plane size)
+    uint32_t ps, sy, sz;    // population layout: element of (i, x, y, z) = i*ps + x + y*sy + z*sz
     int nx, ny, nz;
     float ti;
     int bc[6];
@@ -58,7 +60,27 @@ LB_DEV int xcd_remap(int b, int nb) {
     return xcd * q + min(xcd, r) + local;
 }
 
-struct NodePos { int x, y, z; uint32_t id; bool valid; };
+// id: node index x + nx (y + ny z) (flags, macros, psi); e: element offset of
+// the node's population 0 in the population buffers.
+struct NodePos { int x, y, z; uint32_t id, e; bool valid; };
+
+// Population layouts.  LB_LAYOUT_SOA: [i][z][y][x], element i*N + id.
+// LB_LAYOUT_ROWS: [z][y][i][x], the Q populations of one x-row stored back to
+// back (Q*nx contiguous elements per row), so a 256-node row tile writes one
+// contiguous run and reads Q short runs from at most 9 rows: far fewer DRAM
+// pages open at once than 2Q independent streams.
+struct PopLayout { uint32_t ps, sy, sz; };
+inline PopLayout pop_layout(int layout, int Q, int nx, int ny, int nz) {
+    const uint32_t N = (uint32_t)nx * ny * nz;
+    if (layout == LB_LAYOUT_ROWS) return {(uint32_t)nx, (uint32_t)Q * nx, (uint32_t)Q * nx * ny};
+    return {N, (uint32_t)nx, (uint32_t)nx * ny};
+}
+// Element offset of node id (1-D kernels: init, reduce, relayout).
+LB_DEV uint32_t elem_of(uint32_t id, uint32_t nx, uint32_t ny, uint32_t sy, uint32_t sz) {
+    uint32_t r = id / nx, x = id - r * nx;
+    uint32_t z = r / ny, y = r - z * ny;
+    return x + y * sy + z * sz;
+}
 
 template <class S> LB_DEV int tile_of_block(const StepArgs<S> &a) {
     int b = blockIdx.x;
@@ -80,99 +102,111 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     p.z = z;
     p.valid = p.x < a.nx && p.y < a.ny;
     p.id = (uint32_t)p.x + (uint32_t)a.nx * ((uint32_t)p.y + (uint32_t)a.ny * (uint32_t)z);
+    p.e = (uint32_t)p.x + a.sy * (uint32_t)p.y + a.sz * (uint32_t)z;
     return p;
 }
 
-// Population loads / stores: MEM 0 plain, 1 non-temporal stores, 2 non-
-// temporal loads and stores (streaming hints; chosen by the autotuner).
-template <int MEM, class R> LB_DEV R ldm(const R *p) {
-    if constexpr (MEM >= 2) return __builtin_nontemporal_load(p);
-    else return *p;
+// Global-memory accesses.  The pointers come from the argument block in
+// device memory, so the compiler cannot infer their address space and would
+// emit FLAT instructions, which also count against the scalar-load counter:
+// every s_waitcnt on an argument load would then wait for the population
+// traffic in flight.  The casts make them global loads / stores.
+// MEM: 0 plain, 1 non-temporal stores, 2 non-temporal loads and stores
+// (streaming hints; chosen by the autotuner).
+#define LB_GLOBAL __attribute__((address_space(1)))
+template <int MEM = 0, class R> LB_DEV R ldm(const R *p) {
+    const LB_GLOBAL R *g = (const LB_GLOBAL R *)p;
+    if constexpr (MEM >= 2) return __builtin_nontemporal_load(g);
+    else return *g;
 }
-template <int MEM, class R> LB_DEV void stm(R *p, R v) {
-    if constexpr (MEM >= 1) __builtin_nontemporal_store(v, p);
-    else *p = v;
+template <int MEM = 0, class R> LB_DEV void stm(R *p, R v) {
+    LB_GLOBAL R *g = (LB_GLOBAL R *)p;
+    if constexpr (MEM >= 1) __builtin_nontemporal_store(v, g);
+    else *g = v;
 }
 
-// Pull all Q populations of a fluid node into f (compute type).
+// Source coordinate of a link along one axis: periodic wrap, clamp at
+// outflow / equilibrium-inlet faces; false where the link leaves through a
+// wall face.
+LB_DEV bool axis_src(int &s, int n, int bc_lo, int bc_hi) {
+    if (s >= 0 && s < n) return true;
+    const int k = s < 0 ? bc_lo : bc_hi;
+    if (k == LB_BC_PERIODIC) { s = s < 0 ? s + n : s - n; return true; }
+    if (k == LB_BC_OUTFLOW || k == LB_BC_INLET_EQ) { s = s < 0 ? 0 : n - 1; return true; }
+    return false;
+}
+
+// Wall face a link leaves through (axes checked x, y, z), -1 if none.
+template <int Q, class S> LB_DEV int wall_face(const StepArgs<S> &a, const NodePos &p, int cx, int cy, int cz) {
+    int sx = p.x - cx, sy = p.y - cy, sz = p.z - cz;
+    if (!axis_src(sx, a.nx, a.bc[0], a.bc[1])) return sx < 0 ? 0 : 1;
+    if (!axis_src(sy, a.ny, a.bc[2], a.bc[3])) return sy < 0 ? 2 : 3;
+    if (Q == 19 && !a.halo && !axis_src(sz, a.nz, a.bc[4], a.bc[5])) return sz < 0 ? 4 : 5;
+    return -1;
+}
+
+// Pull all Q populations of a node into f (compute type).  Every link is
+// one independent load: from the upstream neighbour (with periodic / clamp
+// index corrections at the faces), from the node itself (opposite slot) for
+// a half-way bounce-back link, or from the received halo plane.  Nodes next
+// to walls or solids (F_SLOW) carry a link mask: bit i set = link i bounces,
+// bit 31 = one of those links bounces off a moving wall.  No per-link
+// branches on loaded data, so the loads of a node are all in flight at once.
 template <int Q, class S, int MEM = 0>
 LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typename S::T *f) {
     typedef typename S::T T;
+    typedef typename S::raw R;
     typedef Vel<Q> L;
-    if (!(fl & F_SLOW)) {
-        // Interior or periodic/clamped faces: one index correction per axis.
-        int lox = p.x == 0 ? a.corr_lo[0] : 0, hix = p.x == a.nx - 1 ? a.corr_hi[0] : 0;
-        int loy = p.y == 0 ? a.corr_lo[1] : 0, hiy = p.y == a.ny - 1 ? a.corr_hi[1] : 0;
-        int loz = 0, hiz = 0;
-        if (Q == 19) { loz = p.z == 0 ? a.corr_lo[2] : 0; hiz = p.z == a.nz - 1 ? a.corr_hi[2] : 0; }
-        static_for<0, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-            int corr = (cx > 0 ? lox : (cx < 0 ? hix : 0)) + (cy > 0 ? loy : (cy < 0 ? hiy : 0)) +
-                       (cz > 0 ? loz : (cz < 0 ? hiz : 0));
-            int off = cx + cy * a.nx + cz * (int)a.nxy;
-            uint32_t src = (uint32_t)((int)p.id - off + corr);
-            f[i] = S::ld(ldm<MEM>(&a.A[(size_t)i * a.N + src]), weight<Q, float>(L::wc(i)));
-        });
-        return;
-    }
+    const int lox = p.x == 0 ? a.corr_lo[0] : 0, hix = p.x == a.nx - 1 ? a.corr_hi[0] : 0;
+    const int loy = p.y == 0 ? a.corr_lo[1] : 0, hiy = p.y == a.ny - 1 ? a.corr_hi[1] : 0;
+    int loz = 0, hiz = 0;
+    if (Q == 19) { loz = p.z == 0 ? a.corr_lo[2] : 0; hiz = p.z == a.nz - 1 ? a.corr_hi[2] : 0; }
+    const uint32_t bm = (fl & F_SLOW) ? ldm(&a.lmask[p.id]) : 0u;
+    // z-slab halo: links crossing z = -1 / z = nz read the received planes
+    // (a tile lies in one z-plane, so these tests are uniform per workgroup)
+    const bool glo = Q == 19 && a.halo && p.z == 0;
+    const bool ghi = Q == 19 && a.halo && p.z == a.nz - 1;
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-        int sx = p.x - cx, sy = p.y - cy, sz = p.z - cz;
-        bool bnc = false;
-        int wf = -1, ghost = 0;
-        if (cx != 0) {
-            if (sx < 0 || sx >= a.nx) {
-                int face = sx < 0 ? 0 : 1;
-                int k = a.bc[face];
-                if (k == LB_BC_PERIODIC) sx = sx < 0 ? sx + a.nx : sx - a.nx;
-                else if (k == LB_BC_OUTFLOW || k == LB_BC_INLET_EQ) sx = sx < 0 ? 0 : a.nx - 1;
-                else { bnc = true; wf = face; }
+        constexpr int j = L::opp(i);
+        const bool bounce = (bm >> i) & 1u;
+        if constexpr (Q == 19 && cz != 0) {
+            if ((cz > 0 && glo) || (cz < 0 && ghi)) {
+                if (bounce) {
+                    f[i] = S::ld(ldm(&a.A[(size_t)j * a.ps + p.e]), weight<Q, float>(L::wc(i)));
+                } else {
+                    int sx = p.x - cx, sy = p.y - cy;
+                    axis_src(sx, a.nx, a.bc[0], a.bc[1]);
+                    axis_src(sy, a.ny, a.bc[2], a.bc[3]);
+                    const uint32_t sxy = (uint32_t)sx + (uint32_t)a.nx * (uint32_t)sy;
+                    f[i] = S::ld(ldm(&a.recv[(size_t)((cz > 0 ? 5 : 0) + zslot(i)) * a.P + sxy]),
+                                 weight<Q, float>(L::wc(i)));
+                }
+                return;
             }
         }
-        if (!bnc && cy != 0) {
-            if (sy < 0 || sy >= a.ny) {
-                int face = sy < 0 ? 2 : 3;
-                int k = a.bc[face];
-                if (k == LB_BC_PERIODIC) sy = sy < 0 ? sy + a.ny : sy - a.ny;
-                else if (k == LB_BC_OUTFLOW || k == LB_BC_INLET_EQ) sy = sy < 0 ? 0 : a.ny - 1;
-                else { bnc = true; wf = face; }
-            }
-        }
-        if (Q == 19 && !bnc && cz != 0) {
-            if (sz < 0 || sz >= a.nz) {
-                int face = sz < 0 ? 4 : 5;
-                if (a.halo) { ghost = sz < 0 ? 1 : 2; sz = sz < 0 ? sz + a.nz : sz - a.nz; }
-                else {
-                    int k = a.bc[face];
-                    if (k == LB_BC_PERIODIC) sz = sz < 0 ? sz + a.nz : sz - a.nz;
-                    else if (k == LB_BC_OUTFLOW || k == LB_BC_INLET_EQ) sz = sz < 0 ? 0 : a.nz - 1;
-                    else { bnc = true; wf = face; }
+        const int corr = (cx > 0 ? lox : (cx < 0 ? hix : 0)) + (cy > 0 ? loy : (cy < 0 ? hiy : 0)) +
+                         (cz > 0 ? loz : (cz < 0 ? hiz : 0));
+        const int off = cx + cy * (int)a.sy + cz * (int)a.sz;
+        const uint32_t src = (uint32_t)((int)p.e - off + corr);
+        const R *ptr = bounce ? &a.A[(size_t)j * a.ps + p.e] : &a.A[(size_t)i * a.ps + src];
+        f[i] = S::ld(ldm<MEM>(ptr), weight<Q, float>(L::wc(i)));
+    });
+    if (bm >> 31) {
+        // bounced links that left through a moving wall gain 6 w_i (c_i . u_w)
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            if ((bm >> i) & 1u) {
+                const int wf = wall_face<Q>(a, p, L::cx(i), L::cy(i), L::cz(i));
+                if (wf >= 0 && a.bc[wf] == LB_BC_MOVING_WALL) {
+                    T cu = (T)L::cx(i) * (T)a.bcu[wf][0] + (T)L::pcy(i) * (T)a.bcu[wf][1] +
+                           (T)L::cz(i) * (T)a.bcu[wf][2];
+                    f[i] = f[i] + (T(6) * weight<Q, T>(L::wc(i))) * cu;
                 }
             }
-        }
-        uint32_t sxy = (uint32_t)sx + (uint32_t)a.nx * (uint32_t)sy;
-        uint32_t sid = sxy + a.nxy * (uint32_t)sz;
-        if (!bnc && !a.bb_full && (a.flags[sid] & F_SOLID)) bnc = true;
-        T v;
-        if (bnc) {
-            constexpr int j = L::opp(i);
-            v = S::ld(a.A[(size_t)j * a.N + p.id], weight<Q, float>(L::wc(j)));
-            if (wf >= 0 && a.bc[wf] == LB_BC_MOVING_WALL) {
-                T cu = (T)L::cx(i) * (T)a.bcu[wf][0] + (T)L::pcy(i) * (T)a.bcu[wf][1] +
-                       (T)L::cz(i) * (T)a.bcu[wf][2];
-                v = v + (T(6) * weight<Q, T>(L::wc(i))) * cu;
-            }
-        } else if (Q == 19 && ghost == 1) {
-            v = S::ld(a.recv[(size_t)(5 + zslot(i)) * a.P + sxy], weight<Q, float>(L::wc(i)));
-        } else if (Q == 19 && ghost == 2) {
-            v = S::ld(a.recv[(size_t)zslot(i) * a.P + sxy], weight<Q, float>(L::wc(i)));
-        } else {
-            v = S::ld(ldm<MEM>(&a.A[(size_t)i * a.N + sid]), weight<Q, float>(L::wc(i)));
-        }
-        f[i] = v;
-    });
+        });
+    }
 }
 
 template <int Q, class T> LB_DEV void moments(const T *f, T &rho, T &ux, T &uy, T &uz) {
@@ -180,14 +214,16 @@ template <int Q, class T> LB_DEV void moments(const T *f, T &rho, T &ux, T &uy, 
     else mom19(f, rho, ux, uy, uz);
 }
 
-template <int Q, class S, int MEM = 0>
-LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S::T *f, bool reversed) {
+// REV (full-way bounce-back at a solid node): slot opp(i) <- f_i.  A
+// compile-time flag: a runtime choice between f[i] and f[opp(i)] becomes a
+// dynamically indexed private array, i.e. per-lane scratch memory.
+template <int Q, class S, int MEM = 0, bool REV = false>
+LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S::T *f) {
     typedef Vel<Q> L;
-    // reversed (full-way bounce-back at a solid node): slot opp(i) <- f_i
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         auto v = S::st(f[i], weight<Q, float>(L::wc(i)));
-        stm<MEM>(&a.B[(size_t)(reversed ? L::opp(i) : i) * a.N + p.id], v);
+        stm<MEM>(&a.B[(size_t)(REV ? L::opp(i) : i) * a.ps + p.e], v);
     });
     if constexpr (Q == 19) {
         if (a.halo && (p.z == a.nz - 1 || p.z == 0)) {
@@ -199,8 +235,8 @@ LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S:
                 constexpr int cz = L::cz(i);
                 if constexpr (cz != 0) {
                     if ((cz > 0 && p.z == a.nz - 1) || (cz < 0 && p.z == 0)) {
-                        auto v = S::st(reversed ? f[L::opp(i)] : f[i], weight<Q, float>(L::wc(i)));
-                        a.send[(size_t)((cz > 0 ? 5 : 0) + zslot(i)) * a.P + xy] = v;
+                        auto v = S::st(f[REV ? L::opp(i) : i], weight<Q, float>(L::wc(i)));
+                        stm(&a.send[(size_t)((cz > 0 ? 5 : 0) + zslot(i)) * a.P + xy], v);
                     }
                 }
             });
@@ -304,7 +340,7 @@ LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int b
             if (gz < 0) gz = (a.halo || a.bc[4] == LB_BC_PERIODIC) ? gz + a.nz : 0;
             if (gz >= a.nz) gz = (a.halo || a.bc[5] == LB_BC_PERIODIC) ? gz - a.nz : a.nz - 1;
         }
-        lds[k] = a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)];
+        lds[k] = ldm(&a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)]);
     }
 }
 
@@ -314,13 +350,24 @@ LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int b
 template <int Q, class S> LB_DEV void apply_zou_he(const StepArgs<S> &a, const NodePos &p, typename S::T *f) {
     typedef typename S::T T;
     if (!a.zou_he) return;
+    // Two independent ifs, not if / else: with if / else the compiler sinks
+    // the two branches' stores (to different f[] slots) into one store with a
+    // selected address, which puts f in per-lane scratch memory.
     if (p.x == 0) {
         if constexpr (S::DEV) { if constexpr (Q == 9) zouhe9_in_dev<T>(f, (T)a.zh_ux); else zouhe19_in_dev<T>(f, (T)a.zh_ux); }
         else { if constexpr (Q == 9) zouhe9_in(f, a.zh_ux); else zouhe19_in(f, a.zh_ux); }
-    } else if (p.x == a.nx - 1) {
+    }
+    if (p.x == a.nx - 1) {
         if constexpr (S::DEV) { if constexpr (Q == 9) zouhe9_out_dev<T>(f, (T)a.zh_rho); else zouhe19_out_dev<T>(f, (T)a.zh_rho); }
         else { if constexpr (Q == 9) zouhe9_out(f, a.zh_rho); else zouhe19_out(f, a.zh_rho); }
     }
+}
+
+template <class S> LB_DEV void store_macros(const StepArgs<S> &a, uint32_t id, float r, float ux, float uy, float uz) {
+    stm(&a.macros[id], r);
+    stm(&a.macros[a.N + id], ux);
+    stm(&a.macros[2 * a.N + id], uy);
+    stm(&a.macros[3 * a.N + id], uz);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,72 +379,66 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
     typedef Vel<Q> L;
     const int bx = 1 << a.bx_shift;
     if (!p.valid) return;
-    const uint8_t fl = a.flags[p.id];
+    const uint8_t fl = ldm(&a.flags[p.id]);
     T f[Q];
     if (fl & F_SOLID) {
         if (a.bb_full) {
-            pull_node<Q, S, MEM>(a, p, F_SLOW, f);
-            store_node<Q, S, MEM>(a, p, f, true);
+            pull_node<Q, S, MEM>(a, p, fl, f);
+            store_node<Q, S, MEM, true>(a, p, f);
         }
         if (MACROS) {
-            a.macros[p.id] = 0.f; a.macros[a.N + p.id] = 0.f;
-            a.macros[2 * a.N + p.id] = 0.f; a.macros[3 * a.N + p.id] = 0.f;
+            store_macros(a, p.id, 0.f, 0.f, 0.f, 0.f);
         }
         return;
     }
+    // Inlet and fluid nodes end in the same store, so a workgroup that holds
+    // both issues each population store once.
+    T rho, ux, uy, uz;
     if (fl & F_INLET) {
-        int face = fl >> 4;
-        T rho = (T)a.bcrho[face], ux = (T)a.bcu[face][0], uy = (T)a.bcu[face][1], uz = (T)a.bcu[face][2];
+        const int face = fl >> 4;
+        rho = (T)a.bcrho[face]; ux = (T)a.bcu[face][0]; uy = (T)a.bcu[face][1]; uz = (T)a.bcu[face][2];
         if constexpr (S::DEV) feq_dev<Q, T>(f, rho - T(1), ux, uy, uz);
         else feq<Q, T>(f, rho, ux, uy, uz);
-        store_node<Q, S, MEM>(a, p, f, false);
-        if (MACROS) {
-            a.macros[p.id] = (float)rho; a.macros[a.N + p.id] = (float)ux;
-            a.macros[2 * a.N + p.id] = (float)uy; a.macros[3 * a.N + p.id] = (float)uz;
-        }
-        return;
-    }
-    pull_node<Q, S, MEM>(a, p, fl, f);
-    apply_zou_he<Q, S>(a, p, f);
-    T rho, ux, uy, uz, drho = T(0);
-    if constexpr (S::DEV) {
-        moments_dev<Q, T>(f, drho, ux, uy, uz);
-        rho = T(1) + drho;
     } else {
-        moments<Q, T>(f, rho, ux, uy, uz);
+        pull_node<Q, S, MEM>(a, p, fl, f);
+        apply_zou_he<Q, S>(a, p, f);
+        T drho = T(0);
+        if constexpr (S::DEV) {
+            moments_dev<Q, T>(f, drho, ux, uy, uz);
+            rho = T(1) + drho;
+        } else {
+            moments<Q, T>(f, rho, ux, uy, uz);
+        }
+        const T ti = (T)a.ti;
+        T ex = ux, ey = uy, ez = uz;
+        if (a.force[0] != 0.0f) ex = ux + (T)a.force[0] / (rho * ti);
+        if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
+        if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
+        if constexpr (SC) {
+            const T *lds = (const T *)lds_raw;
+            const int W = bx + 2, Hh = a.by + 2;
+            const int lx = (threadIdx.x & (bx - 1)) + 1, ly = (threadIdx.x >> a.bx_shift) + 1;
+            T IF[3] = {T(0), T(0), T(0)};
+            static_for<1, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+                int k = (lx + cx) + W * ((ly + cy) + Hh * (Q == 19 ? 1 + cz : 0));
+                T wp = weight<Q, T>(L::wc(i)) * lds[k];
+                if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
+                if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
+                if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
+            });
+            const int kc = lx + W * (ly + Hh * (Q == 19 ? 1 : 0));
+            T gp = (T)(-a.scG) * lds[kc];
+            ex = ex + (gp * IF[0]) / (ti * rho);
+            ey = ey + (gp * IF[1]) / (ti * rho);
+            if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+        }
+        if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
+        else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
     }
-    if (MACROS) {
-        a.macros[p.id] = (float)rho; a.macros[a.N + p.id] = (float)ux;
-        a.macros[2 * a.N + p.id] = (float)uy; a.macros[3 * a.N + p.id] = (float)uz;
-    }
-    const T ti = (T)a.ti;
-    T ex = ux, ey = uy, ez = uz;
-    if (a.force[0] != 0.0f) ex = ux + (T)a.force[0] / (rho * ti);
-    if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
-    if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
-    if constexpr (SC) {
-        const T *lds = (const T *)lds_raw;
-        const int W = bx + 2, Hh = a.by + 2;
-        const int lx = (threadIdx.x & (bx - 1)) + 1, ly = (threadIdx.x >> a.bx_shift) + 1;
-        T IF[3] = {T(0), T(0), T(0)};
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-            int k = (lx + cx) + W * ((ly + cy) + Hh * (Q == 19 ? 1 + cz : 0));
-            T wp = weight<Q, T>(L::wc(i)) * lds[k];
-            if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
-            if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
-            if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
-        });
-        const int kc = lx + W * (ly + Hh * (Q == 19 ? 1 : 0));
-        T gp = (T)(-a.scG) * lds[kc];
-        ex = ex + (gp * IF[0]) / (ti * rho);
-        ey = ey + (gp * IF[1]) / (ti * rho);
-        if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
-    }
-    if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
-    else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
-    store_node<Q, S, MEM>(a, p, f, false);
+    if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
+    store_node<Q, S, MEM>(a, p, f);
 }
 
 // Shan-Chen density pre-pass: psi(rho) of the post-streaming density, with
@@ -406,8 +447,8 @@ template <int Q, class S>
 LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
     typedef typename S::T T;
     if (!p.valid) return;
-    const uint8_t fl = a.flags[p.id];
-    if (fl & F_SOLID) { a.psi[p.id] = (T)a.scpsi_solid; return; }
+    const uint8_t fl = ldm(&a.flags[p.id]);
+    if (fl & F_SOLID) { stm(&a.psi[p.id], (T)a.scpsi_solid); return; }
     T rho;
     if (fl & F_INLET) {
         rho = (T)a.bcrho[fl >> 4];
@@ -424,18 +465,23 @@ LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
             moments<Q, T>(f, rho, ux, uy, uz);
         }
     }
-    a.psi[p.id] = sc_psi<T>(rho, a.scrho0);
+    stm(&a.psi[p.id], sc_psi<T>(rho, a.scrho0));
 }
 
 // ---------------------------------------------------------------------------
-// Node flags on the host: solid, inlet plane (+ face in the high nibble),
-// and "slow" when a link crosses a wall / halo face or meets a solid node
-// (those nodes take the general per-link path of pull_node).
-inline void build_flags(const lb_config &g, const std::vector<uint8_t> &solid, std::vector<uint8_t> &fl) {
+// Node flags on the host: solid, inlet plane (+ face in the high nibble), and
+// F_SLOW with a link mask (lm) for nodes with a link that bounces back: bit i
+// when link i leaves through a wall face (axes checked x, y, z; periodic
+// faces wrap, outflow / inlet faces clamp, halo z faces pass to the received
+// planes) or, without full-way bounce-back, when its upstream node is solid;
+// bit 31 when one of those links leaves through a moving wall.
+inline void build_flags(const lb_config &g, const std::vector<uint8_t> &solid, std::vector<uint8_t> &fl,
+                        std::vector<uint32_t> &lm) {
     const int Q = g.lattice;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     const bool halo = g.halo_z && Q == 19;
     fl.assign(solid.size(), 0);
+    lm.assign(solid.size(), 0u);
     static const int C9[9][3] = {{0, 0, 0}, {1, 0, 0}, {0, -1, 0}, {-1, 0, 0}, {0, 1, 0},
                                  {1, -1, 0}, {-1, -1, 0}, {-1, 1, 0}, {1, 1, 0}};
     static const int C19[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0},
@@ -459,33 +505,38 @@ inline void build_flags(const lb_config &g, const std::vector<uint8_t> &solid, s
                 }
                 bool on_face = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || (Q == 19 && (z == 0 || z == nz - 1));
                 if (!on_face && !any_solid) { fl[id] = f; continue; }
-                bool slow = false;
-                for (int i = 1; i < Q && !slow; i++) {
+                uint32_t m = 0;
+                for (int i = 1; i < Q; i++) {
                     int s[3] = {x - C[i][0], y - C[i][1], z - C[i][2]};
-                    for (int a = 0; a < 3 && !slow; a++) {
+                    int wf = -1;
+                    for (int a = 0; a < 3 && wf < 0; a++) {
                         if (a == 2 && Q == 9) break;
                         if (s[a] >= 0 && s[a] < n[a]) continue;
-                        int face = 2 * a + (s[a] >= n[a] ? 1 : 0);
-                        if (a == 2 && halo) { slow = true; break; }
-                        int k = g.bc[face];
+                        const int face = 2 * a + (s[a] >= n[a] ? 1 : 0);
+                        const int k = (a == 2 && halo) ? LB_BC_PERIODIC : g.bc[face];
                         if (k == LB_BC_PERIODIC) s[a] = s[a] < 0 ? s[a] + n[a] : s[a] - n[a];
                         else if (k == LB_BC_OUTFLOW || k == LB_BC_INLET_EQ) s[a] = s[a] < 0 ? 0 : n[a] - 1;
-                        else slow = true;
+                        else wf = face;
                     }
-                    if (!slow && !g.bb_full && solid[s[0] + (size_t)nx * (s[1] + (size_t)ny * s[2])]) slow = true;
+                    bool bnc = wf >= 0;
+                    if (!bnc && !g.bb_full && solid[s[0] + (size_t)nx * (s[1] + (size_t)ny * s[2])]) bnc = true;
+                    if (bnc) {
+                        m |= 1u << i;
+                        if (wf >= 0 && g.bc[wf] == LB_BC_MOVING_WALL) m |= 1u << 31;
+                    }
                 }
-                if ((f & F_SOLID) && g.bb_full) slow = true;
-                if (slow) f |= F_SLOW;
+                if (m) { f |= F_SLOW; lm[id] = m; }
                 fl[id] = f;
             }
 }
 
 
-// Fast-path index corrections across each face (element units):
-// periodic wrap or clamp; 0 where the face needs the per-link path.
-inline void face_corrections(const lb_config &g, int lo[3], int hi[3]) {
+// Fast-path index corrections across each face (element units of the
+// population layout): periodic wrap or clamp; 0 where the face needs the
+// per-link path.
+inline void face_corrections(const lb_config &g, const PopLayout &L, int lo[3], int hi[3]) {
     const int n[3] = {g.nx, g.ny, g.nz};
-    const int stride[3] = {1, g.nx, g.nx * g.ny};
+    const int stride[3] = {1, (int)L.sy, (int)L.sz};
     for (int a = 0; a < 3; a++) {
         auto corr = [&](int face, int sign) {
             int k = g.bc[face];

@@ -93,21 +93,40 @@ def channel_geometry(nx, ny, circles):
     return solid
 
 
-def channel_spec(seed, index, size=None):
+def _place_circle(nx, ny, r, u_x, u_y):
+    xlo, xhi = r + 4, max(r + 4, nx - r - 8)
+    ylo, yhi = r + 2, max(r + 2, ny - r - 3)
+    return (xlo + int(u_x * (xhi - xlo + 1)), ylo + int(u_y * (yhi - ylo + 1)), r)
+
+
+def channel_spec(seed, index, size=None, min_open=0.5):
     """Lattice `index` of ensemble `seed`: size, 1-4 circles (r 4..16),
-    u_in in [0.02, 0.08], tau in [0.52, 0.9]; 16 uniforms per lattice."""
+    u_in in [0.02, 0.08], tau in [0.52, 0.9]; 16 uniforms per lattice.
+
+    A circle whose addition would leave any column with less than `min_open`
+    of its interior rows open is shrunk (r - 1, same uniforms for its centre)
+    until it fits; one that does not fit at r = 4 is dropped (the first
+    circle is always kept at r = 4 or more).  Without this, overlapping
+    circles can close the channel and the velocity inlet then drives the gap
+    flow past Ma = 1."""
     u = uniform01(seed, 16, start=16 * index)
     nx, ny = size if size is not None else SIZES[int(u[0] * 4)]
     ncirc = 1 + int(u[1] * 4)
     circles = []
+    interior = ny - 2
     for c in range(ncirc):
-        r = 4 + int(u[2 + 3 * c] * 13)
-        r = min(r, (ny - 6) // 2)
-        xlo, xhi = r + 4, max(r + 4, nx - r - 8)
-        ylo, yhi = r + 2, max(r + 2, ny - r - 3)
-        cx = xlo + int(u[3 + 3 * c] * (xhi - xlo + 1))
-        cy = ylo + int(u[4 + 3 * c] * (yhi - ylo + 1))
-        circles.append((cx, cy, r))
+        r = min(4 + int(u[2 + 3 * c] * 13), (ny - 6) // 2)
+        while True:
+            cand = circles + [_place_circle(nx, ny, r, u[3 + 3 * c], u[4 + 3 * c])]
+            blocked = channel_geometry(nx, ny, cand)[1:-1].sum(axis=0)
+            if (interior - blocked).min() >= min_open * interior:
+                circles = cand
+                break
+            if r <= 4:
+                if not circles:
+                    circles = cand
+                break
+            r -= 1
     u_in = 0.02 + 0.06 * u[14]
     tau = 0.52 + 0.38 * u[15]
     return dict(nx=nx, ny=ny, tau=float(tau), u_in=float(np.float32(u_in)),

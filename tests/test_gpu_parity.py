@@ -141,7 +141,7 @@ FUSED_CASES = {
     "cavity48_f32s": ("cavity", dict(n=48, storage="f32s"), 500),
     "sphere_halo_f32s": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f32s"), 300),
     "spinodal24_f32s": ("spinodal", dict(n=24, storage="f32s"), 300),
-    "channel_zouhe_f32s": ("channel", dict(seed=3, index=2, size=(128, 64)), 1000),
+    "channel_zouhe_f32s": ("channel", dict(seed=3, index=0, size=(128, 64)), 1000),
 }
 
 
@@ -157,12 +157,16 @@ def _scene(kind, kw):
             "channel": _channel_scene}[kind](**kw)
 
 
+@pytest.mark.parametrize("layout", ["soa", "rows"])
 @pytest.mark.parametrize("case", sorted(FUSED_CASES))
-def test_fused_solver_matches_oracle(gpu, case):
+def test_fused_solver_matches_oracle(gpu, case, layout):
+    """Both device population layouts; populations come back in [i][z][y][x]."""
     kind, kw, steps = FUSED_CASES[case]
     cfg, solid, init = _scene(kind, kw)
-    print("case", case, cfg["nx"], cfg["ny"], cfg["nz"], "storage", cfg["storage"], flush=True)
+    cfg["layout"] = {"soa": 0, "rows": 1}[layout]
+    print("case", case, layout, cfg["nx"], cfg["ny"], cfg["nz"], "storage", cfg["storage"], flush=True)
     sol, orc = _run_pair(gpu, cfg, solid, init, steps)
+    assert sol.launch_config["layout"] == cfg["layout"]
     fluid = solid.reshape(orc.shape) == 0
     _assert_pop_parity(sol.populations(), orc.A, cfg["storage"], fluid)
     m, mo = sol.macros(), orc.macros()
@@ -173,18 +177,21 @@ def test_fused_solver_matches_oracle(gpu, case):
     sol.close()
 
 
+@pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("Q", [9, 19])
-def test_fused_reference_physics_mode_matches_reference_driver(gpu, Q):
-    """bb_full + zou_he: the fused sweep reproduces the reference step."""
+def test_fused_reference_physics_mode_matches_reference_driver(gpu, Q, layout):
+    """bb_full + zou_he: the fused sweep reproduces the reference step
+    (populations uploaded in [i][z][y][x] order into either device layout)."""
     from lattibol_amd import make_config
     shape = (16, 48) if Q == 9 else (6, 10, 24)
     s = _deck_solids(shape)
     ref = O.RefOracle(s, ux=0.04, rho=1.0, viscosity=0.05)
     dims = tuple(reversed(shape))
     cfg = make_config(Q, dims, tau_inv=float(ref.tau_inv), bb_full=True, zou_he=True, zh_ux_in=0.04,
-                      zh_rho_out=1.0)
+                      zh_rho_out=1.0, layout=layout)
     sol = gpu.Solver(cfg, s)
     sol.set_populations(ref.f[0])
+    np.testing.assert_array_equal(sol.populations().reshape(ref.f[0].shape), ref.f[0])
     ref.runLBM(150)
     sol.step(150)
     fluid = s.reshape((1,) + shape if Q == 9 else shape) == 0
@@ -193,11 +200,13 @@ def test_fused_reference_physics_mode_matches_reference_driver(gpu, Q):
     sol.close()
 
 
-def test_halo_planes_equal_periodic_z(gpu):
+@pytest.mark.parametrize("layout", [0, 1])
+def test_halo_planes_equal_periodic_z(gpu, layout):
     from lattibol_amd import scenes
     outs = []
     for halo in (True, False):
-        cfg, solid, init = scenes.sphere_channel(64, 32, 24, radius=6, center=(20, 16, 12), halo_z=halo)
+        cfg, solid, init = scenes.sphere_channel(64, 32, 24, radius=6, center=(20, 16, 12), halo_z=halo,
+                                                 layout=layout)
         sol = gpu.Solver(cfg, solid)
         sol.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
         sol.step(150)
