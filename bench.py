@@ -4,7 +4,8 @@
 Workload (default): D3Q19 BGK channel flow past a sphere, 256x128x128 FP32,
 sphere r=16 at (64,64,64), equilibrium inlet u=0.05, zero-gradient outflow,
 periodic y, z periodic through the z-slab halo planes (pack fused into the
-sweep, device copy, unpack fused into the next sweep), tau=0.56.  A step is
+sweep, unpack fused into the next sweep, two alternating halo buffers),
+tau=0.56.  A step is
 one time step of the whole lattice.  Inputs are resident in HBM before the
 timed region; the timed region is exactly --steps steps between two device
 synchronisations.

@@ -252,9 +252,14 @@ int lb_steps_done(lb_ctx *ctx, long long *out);
 /* Timing helper: average duration (ms) of one step kernel, measured with HIP
  * events over n back-to-back launches on the handle's stream (no macros). */
 int lb_time_step_kernel(lb_ctx *ctx, int n, float *ms_per_launch);
-/* Halo buffers of the z-slab (device pointers): send/recv planes of the five
- * +z and five -z populations (5*nx*ny each).  For a multi-GPU ring the
- * caller moves send_hi -> neighbour's recv_lo and send_lo -> recv_hi. */
+/* Halo buffers of the z-slab for the next step (device pointers): the send
+ * planes that sweep packs (the five +z / five -z populations of the top /
+ * bottom plane, 5*nx*ny elements each) and the recv planes it unpacks.  Two
+ * buffers alternate step by step: the sweep packs into one and unpacks the
+ * other, which the previous sweep packed.  On this one-slab ring that is the
+ * exchange itself (recv_hi <- send_lo, recv_lo <- send_hi is the identity);
+ * a ring of several slabs would move send_hi -> neighbour's recv_lo and
+ * send_lo -> the other neighbour's recv_hi between single steps. */
 int lb_halo_buffers(lb_ctx *ctx, void **send_lo, void **send_hi, void **recv_lo,
                     void **recv_hi);
 

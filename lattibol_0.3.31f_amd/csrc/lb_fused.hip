@@ -210,7 +210,8 @@ struct lb_ctx {
     float *macros = nullptr;
     float *snap = nullptr;
     void *psi = nullptr;
-    void *send = nullptr, *recv = nullptr;
+    void *halo[2] = {nullptr, nullptr};   // z halo planes; the sweep from buffer s packs into halo[s]
+                                          // and unpacks halo[1-s]
     double *partial = nullptr, *dout = nullptr;
     float *tmp_fields = nullptr;
     bool macros_valid = false, snap_valid = false;
@@ -227,7 +228,8 @@ struct lb_ctx {
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
-        void *ps[] = {buf[0], buf[1], flags, lmask, macros, snap, psi, send, recv, partial, dout, tmp_fields, dargs};
+        void *ps[] = {buf[0], buf[1], flags, lmask, macros, snap, psi, halo[0], halo[1], partial, dout,
+                      tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -249,8 +251,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     a.lmask = c->lmask;
     a.macros = c->macros;
     a.psi = (typename S::T *)c->psi;
-    a.send = (typename S::raw *)c->send;
-    a.recv = (const typename S::raw *)c->recv;
+    a.send = (typename S::raw *)c->halo[src];
+    a.recv = (const typename S::raw *)c->halo[1 - src];
     a.N = c->N;
     const PopLayout pl = c->pl();
     a.ps = pl.ps; a.sy = pl.sy; a.sz = pl.sz;
@@ -329,16 +331,15 @@ template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros) {
     }
 }
 
-// One time step from buffer src: sweep (+ halo copy).
+// One time step from buffer src.  With the z-slab halo the sweep packs the
+// outgoing planes into halo[src] and unpacks halo[1 - src], written by the
+// previous sweep: for one subdomain on the ring its -z and +z neighbour is
+// the slab itself (recv_hi <- send_lo, recv_lo <- send_hi, slot for slot), so
+// the exchange between the two is the identity and alternating the two
+// buffers replaces the copy.  A ring of several slabs moves the planes
+// between sweeps instead (see lb_halo_buffers).
 int launch_step(lb_ctx *c, int src, bool macros) {
-    int rc = c->Q == 9 ? launch_step_q<9>(c, src, macros) : launch_step_q<19>(c, src, macros);
-    if (rc) return rc;
-    if (c->cfg.halo_z && c->Q == 19) {
-        // copy: with one subdomain on the ring the -z and +z neighbour is this
-        // slab itself, so recv_hi <- send_lo and recv_lo <- send_hi, one copy.
-        rc = (int)hipMemcpyAsync(c->recv, c->send, (size_t)10 * c->P * c->esz, hipMemcpyDeviceToDevice, c->stream);
-    }
-    return rc;
+    return c->Q == 9 ? launch_step_q<9>(c, src, macros) : launch_step_q<19>(c, src, macros);
 }
 
 template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
@@ -383,11 +384,11 @@ template <class F> int by_esz(int esz, F &&f) {
 
 // recv planes from buf[0] (after an initialisation or a population upload).
 int refill_recv(lb_ctx *c) {
-    if (!c->recv) return 0;
+    if (!c->halo[1]) return 0;
     return by_esz(c->esz, [&](auto t) {
         typedef decltype(t) R;
         hipLaunchKernelGGL((k_fill_recv<R>), dim3((c->P + kThreads - 1) / kThreads), dim3(kThreads), 0, c->stream,
-                           (const R *)c->buf[0], (R *)c->recv, c->P, (uint32_t)c->cfg.nx, c->cfg.nz, c->pl());
+                           (const R *)c->buf[0], (R *)c->halo[1], c->P, (uint32_t)c->cfg.nx, c->cfg.nz, c->pl());
         return (int)hipGetLastError();
     });
 }
@@ -472,7 +473,7 @@ int autotune(lb_ctx *c) {
     if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
     float best = 1e30f;
     Cand bestc = cands[0];
-    const int reps = 6;
+    const int reps = 10;
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
         if (int rc = upload_args(c)) return rc;
@@ -534,10 +535,10 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipMalloc(&c->dargs, 2 * lb_ctx::kArgBytes);
     if (!rc && cfg->sc_G != 0.0f) rc = (int)hipMalloc(&c->psi, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4));
     if (!rc && cfg->halo_z && c->Q == 19) {
-        rc = (int)hipMalloc(&c->send, (size_t)10 * c->P * c->esz);
-        if (!rc) rc = (int)hipMalloc(&c->recv, (size_t)10 * c->P * c->esz);
-        if (!rc) rc = (int)hipMemsetAsync(c->send, 0, (size_t)10 * c->P * c->esz, c->stream);
-        if (!rc) rc = (int)hipMemsetAsync(c->recv, 0, (size_t)10 * c->P * c->esz, c->stream);
+        for (int h = 0; h < 2 && !rc; h++) {
+            rc = (int)hipMalloc(&c->halo[h], (size_t)10 * c->P * c->esz);
+            if (!rc) rc = (int)hipMemsetAsync(c->halo[h], 0, (size_t)10 * c->P * c->esz, c->stream);
+        }
     }
     if (!rc) rc = (int)hipMemsetAsync(c->buf[0], 0, pop_bytes, c->stream);
     if (!rc) rc = (int)hipMemsetAsync(c->buf[1], 0, pop_bytes, c->stream);
@@ -597,8 +598,8 @@ extern "C" int lb_init_equilibrium_device(lb_ctx *c, const float *r, const float
     c->steps = 0;
     c->macros_valid = false;
     c->snap_valid = false;
-    if (c->send) {
-        rc = (int)hipMemsetAsync(c->send, 0, (size_t)10 * c->P * c->esz, c->stream);
+    if (c->halo[0]) {
+        rc = (int)hipMemsetAsync(c->halo[0], 0, (size_t)10 * c->P * c->esz, c->stream);
         if (rc) return rc;
     }
     // A fresh halo: the planes the first sweep pulls across z come from the
@@ -792,11 +793,12 @@ extern "C" int lb_time_step_kernel(lb_ctx *c, int n, float *ms) {
 
 extern "C" int lb_halo_buffers(lb_ctx *c, void **send_lo, void **send_hi, void **recv_lo, void **recv_hi) {
     if (!c) return LB_ERR_INVALID;
-    if (!c->send) return LB_ERR_UNSUPPORTED;
+    if (!c->halo[0]) return LB_ERR_UNSUPPORTED;
     const size_t pb5 = (size_t)5 * c->P * c->esz;
-    if (send_lo) *send_lo = c->send;
-    if (send_hi) *send_hi = (char *)c->send + pb5;
-    if (recv_hi) *recv_hi = c->recv;
-    if (recv_lo) *recv_lo = (char *)c->recv + pb5;
+    void *send = c->halo[c->cur], *recv = c->halo[1 - c->cur];   // the next sweep's pair
+    if (send_lo) *send_lo = send;
+    if (send_hi) *send_hi = (char *)send + pb5;
+    if (recv_hi) *recv_hi = recv;
+    if (recv_lo) *recv_lo = (char *)recv + pb5;
     return 0;
 }
