@@ -284,6 +284,11 @@ int lb_batch_destroy(lb_batch *b);
  * (rho, ux, uy planes per lattice: 3*N_b floats per lattice, back to back). */
 int lb_batch_init_rest(lb_batch *b);
 int lb_batch_init_fields_device(lb_batch *b, const float *d_fields);
+/* Same with a choice of initial populations: mode 0 equilibrium (as above),
+ * mode 1 equilibrium plus the first-order Chapman-Enskog non-equilibrium
+ * part from the fields' velocity gradients, in its post-collision form
+ * -(w_i rho (tau - 1) / cs^2) Q_i:S. */
+int lb_batch_init_fields_device_ex(lb_batch *b, const float *d_fields, int mode);
 int lb_batch_step(lb_batch *b, int nsteps);
 /* Run to residual < tol (checked every k steps) or max_steps; per lattice
  * steps taken and final residual.  Converged lattices retire; a lattice whose
@@ -315,7 +320,10 @@ int lb_unet_destroy(lb_unet *u);
 int64_t lb_unet_param_count(lb_unet *u);
 int lb_unet_get_params(lb_unet *u, float *host_params, float *host_grads);
 int lb_unet_set_params(lb_unet *u, const float *host_params);
-/* x: device (B, H, W, in_ch) FP32; y: device (B, H, W, out_ch) FP32 (may be NULL). */
+/* x: device (B, H, W, in_ch) FP32; y: device (B, H, W, out_ch) FP32 (may be
+ * NULL: the output then stays inside the handle for lb_unet_backward and the
+ * call is asynchronous on the handle's stream).  With y the call returns
+ * once y is written. */
 int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y);
 /* Masked MSE of the last forward vs d_t (B,H,W,out_ch) with weights d_mask
  * (B,H,W); fills every parameter gradient; *loss = mean squared error. */

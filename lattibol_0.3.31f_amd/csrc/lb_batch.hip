@@ -149,8 +149,23 @@ __global__ void k_batch_final(const int2 *ranges, int nact, const double *partia
     res[k] = umax > kUDiverged ? INFINITY : v;
 }
 
+// Initial populations from (rho, ux, uy) fields (NULL: rest).  neq = 0:
+// equilibrium.  neq = 1: equilibrium plus the first-order Chapman-Enskog
+// non-equilibrium part implied by the velocity field.  Before collision it
+// is f_i^neq = -(w_i rho tau / cs^2) Q_i:S (Q_i = c_i c_i - cs^2 I,
+// S = (grad u + grad u^T) / 2); the buffers hold post-collision values,
+// (1 - 1/tau) times that, i.e. -(w_i rho (tau - 1) / cs^2) Q_i:S.  Central
+// differences, one-sided next to solids and lattice edges.  Physical axes:
+// y = -row.
+__device__ float grad_1d(const float *u, uint32_t id, int stride, bool has_m, bool has_p) {
+    if (has_m && has_p) return 0.5f * (u[id + stride] - u[id - stride]);
+    if (has_p) return u[id + stride] - u[id];
+    if (has_m) return u[id] - u[id - stride];
+    return 0.0f;
+}
+
 __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, const int2 *table, float *A, float *B,
-                                                         const float *fields) {
+                                                         const float *fields, const uint8_t *flags, int neq) {
     const int2 e = table[blockIdx.x];
     const BatchLat L = lats[e.x];
     int tx = e.y % L.ntx, ty = e.y / L.ntx;
@@ -158,12 +173,29 @@ __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, c
     if (x >= L.nx || y >= L.ny) return;
     uint32_t id = (uint32_t)x + (uint32_t)L.nx * (uint32_t)y;
     float rho = 1.0f, ux = 0.0f, uy = 0.0f;
+    float f[9];
     if (fields) {
         const float *F = fields + 3 * L.node_off;
         rho = F[id]; ux = F[L.N + id]; uy = F[2 * (size_t)L.N + id];
     }
-    float f[9];
     feq_dev<9, float>(f, rho - 1.0f, ux, uy, 0.0f);
+    const uint8_t *fl = flags + L.node_off;
+    if (fields && neq && !(fl[id] & F_SOLID)) {
+        const float *Ux = fields + 3 * L.node_off + L.N, *Uy = Ux + L.N;
+        const bool xm = x > 0 && !(fl[id - 1] & F_SOLID), xp = x < L.nx - 1 && !(fl[id + 1] & F_SOLID);
+        const bool rm = y > 0 && !(fl[id - L.nx] & F_SOLID), rp = y < L.ny - 1 && !(fl[id + L.nx] & F_SOLID);
+        const int nx = L.nx;
+        const float dxux = grad_1d(Ux, id, 1, xm, xp), dxuy = grad_1d(Uy, id, 1, xm, xp);
+        const float dyux = -grad_1d(Ux, id, nx, rm, rp), dyuy = -grad_1d(Uy, id, nx, rm, rp);
+        const float sxx = dxux, syy = dyuy, sxy = 0.5f * (dxuy + dyux);
+        const float tau_m1 = 1.0f / L.ti - 1.0f;
+        static_for<0, 9>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr float cx = (float)Vel<9>::cx(i), cy = (float)Vel<9>::pcy(i);
+            const float qs = cx * cx * sxx + 2.0f * cx * cy * sxy + cy * cy * syy - (sxx + syy) * (1.0f / 3.0f);
+            f[i] = f[i] - 3.0f * weight<9, float>(Vel<9>::wc(i)) * rho * tau_m1 * qs;
+        });
+    }
     for (int i = 0; i < 9; i++) {
         A[L.pop_off + (size_t)i * L.N + id] = f[i];
         B[L.pop_off + (size_t)i * L.N + id] = f[i];
@@ -316,7 +348,7 @@ extern "C" int lb_batch_destroy(lb_batch *b) {
     return 0;
 }
 
-static int batch_init(lb_batch *b, const float *fields) {
+static int batch_init(lb_batch *b, const float *fields, int neq) {
     std::fill(b->active.begin(), b->active.end(), 1);
     std::fill(b->snap_valid.begin(), b->snap_valid.end(), 0);
     std::fill(b->macros_valid.begin(), b->macros_valid.end(), 0);
@@ -324,7 +356,8 @@ static int batch_init(lb_batch *b, const float *fields) {
     b->parity = 0;
     int rc = rebuild_table(b);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_batch_init, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->A, b->B, fields);
+    hipLaunchKernelGGL(k_batch_init, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->A, b->B, fields,
+                       b->flags, neq);
     rc = (int)hipGetLastError();
     // the first residual check only primes the snapshot, but its value must
     // still be finite: start the snapshot from zero velocities
@@ -335,12 +368,17 @@ static int batch_init(lb_batch *b, const float *fields) {
 
 extern "C" int lb_batch_init_rest(lb_batch *b) {
     if (!b) return LB_ERR_INVALID;
-    return batch_init(b, nullptr);
+    return batch_init(b, nullptr, 0);
 }
 
 extern "C" int lb_batch_init_fields_device(lb_batch *b, const float *d_fields) {
     if (!b || !d_fields) return LB_ERR_INVALID;
-    return batch_init(b, d_fields);
+    return batch_init(b, d_fields, 0);
+}
+
+extern "C" int lb_batch_init_fields_device_ex(lb_batch *b, const float *d_fields, int mode) {
+    if (!b || !d_fields || mode < 0 || mode > 1) return LB_ERR_INVALID;
+    return batch_init(b, d_fields, mode);
 }
 
 extern "C" int lb_batch_step(lb_batch *b, int nsteps) {

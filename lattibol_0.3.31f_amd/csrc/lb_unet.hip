@@ -663,7 +663,11 @@ extern "C" int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y) 
                        u->w + u->head_boff, y, (int)npix0, L[0].C, u->d.out_ch);
     if (y != u->y)
         (void)hipMemcpyAsync(u->y, y, npix0 * u->d.out_ch * 4, hipMemcpyDeviceToDevice, u->stream);
-    return (int)hipGetLastError();
+    int rc = (int)hipGetLastError();
+    // an output for the caller is complete on return (any stream or the host
+    // may read it); the training path (d_y NULL) stays asynchronous
+    if (!rc && d_y) rc = (int)hipStreamSynchronize(u->stream);
+    return rc;
 }
 
 // Masked MSE of the last forward against d_t (B, H, W, out_ch) with d_mask

@@ -154,6 +154,7 @@ _PROTOS = {
     "lb_batch_destroy": (_i, [_vp]),
     "lb_batch_init_rest": (_i, [_vp]),
     "lb_batch_init_fields_device": (_i, [_vp, _vp]),
+    "lb_batch_init_fields_device_ex": (_i, [_vp, _vp, _i]),
     "lb_batch_step": (_i, [_vp, _i]),
     "lb_batch_run_to_steady": (_i, [_vp, C.c_double, _i, _i, _vp, _vp]),
     "lb_batch_get_macros": (_i, [_vp, _i, _vp, _vp, _vp]),

@@ -52,8 +52,14 @@ class Batch:
     def init_rest(self):
         check(lib.lb_batch_init_rest(self._h), "lb_batch_init_rest")
 
-    def init_fields_device(self, ptr):
-        check(lib.lb_batch_init_fields_device(self._h, C.c_void_p(ptr)), "lb_batch_init_fields_device")
+    def init_fields_device(self, ptr, mode=0):
+        """Populations from packed device fields (rho, ux, uy per lattice).
+        mode 0: equilibrium; 1: equilibrium + first-order non-equilibrium part."""
+        if mode == 0:
+            check(lib.lb_batch_init_fields_device(self._h, C.c_void_p(ptr)), "lb_batch_init_fields_device")
+        else:
+            check(lib.lb_batch_init_fields_device_ex(self._h, C.c_void_p(ptr), int(mode)),
+                  "lb_batch_init_fields_device_ex")
 
     def step(self, n):
         check(lib.lb_batch_step(self._h, int(n)), "lb_batch_step")
