@@ -40,10 +40,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--workload", default="sphere",
-                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "copy"],
+                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "unet", "copy"],
                     help="sphere = D3Q19 channel past a sphere (the headline); the others print their own line")
-    ap.add_argument("--train-samples", type=int, default=1024)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--train-samples", type=int, default=4096)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--heldout", type=int, default=256)
     ap.add_argument("--max-steady-steps", type=int, default=300000)
     return ap.parse_args()
@@ -198,7 +198,8 @@ def bench_warmstart(args):
     log("training set: %d/%d converged in %.1f s" % (data["converged"].sum(), args.train_samples, gen))
     net = UNet(levels=4, base=32, in_ch=3, out_ch=3, height=128, width=128, max_batch=16, seed=1)
     hist = warmstart.train(net, data, epochs=args.epochs, batch=16, lr=1e-3, log=log)
-    ev = warmstart.evaluate(net, n=args.heldout, seed=777, max_steps=args.max_steady_steps)
+    ev = warmstart.evaluate(net, n=args.heldout, seed=777, max_steps=args.max_steady_steps, init_mode=0,
+                            extra_modes=(1,))
     ev.pop("per_lattice")
     return common("time-to-steady-state, warm-start / plain LBM (wall-time ratio)", round(ev["wall_ratio"], 4),
                   "ratio", args, 0, None, "fp32 LBM, bf16 U-Net",
@@ -208,6 +209,65 @@ def bench_warmstart(args):
                   higher=False, train={"samples": int(data["converged"].sum()), "generation_s": round(gen, 1),
                                        "epochs": args.epochs, "train_s": round(hist["seconds"], 1),
                                        "final_loss": hist["loss"][-1]}, evaluation=ev)
+
+
+def unet_flops(levels=4, base=32, H=128, W=128, B=16, out_ch=3):
+    """Dense MFMA work of one training iteration: forward, input gradients
+    (every conv but the first) and weight gradients, 2 flop per MAC, the first
+    conv counted with its padded 16 input channels."""
+    from lattibol_amd.unet import conv_layout
+    layout, _ = conv_layout(levels, base, 3, out_ch)
+    convs = [c for c in layout if c[0] == "conv"]
+    res = []
+    for l in range(levels - 1):
+        res += [l, l]
+    res += [levels - 1, levels - 1]
+    for l in range(levels - 2, -1, -1):
+        res += [l, l]
+    fwd = dgrad = 0
+    for k, (kind, co, ci, _, _) in enumerate(convs):
+        npix = B * (H >> res[k]) * (W >> res[k])
+        f = 2.0 * npix * 9 * ci * co
+        fwd += f
+        if k > 0:
+            dgrad += f
+    head = 2.0 * B * H * W * base * out_ch
+    return fwd + head, fwd + dgrad + fwd + 3 * head
+
+
+def bench_unet(args):
+    """U-Net training iterations (batch 16 of 128x128, synthetic inputs):
+    forward + backward + Adam, all on the BF16 MFMA kernels."""
+    import numpy as np
+    from lattibol_amd.unet import UNet
+    from lattibol_amd._lib import DeviceArray
+    net = UNet(levels=4, base=32, in_ch=3, out_ch=3, height=128, width=128, max_batch=16, seed=1)
+    rng = np.random.default_rng(0)
+    xd = DeviceArray.from_numpy(rng.standard_normal((16, 128, 128, 3)).astype(np.float32))
+    td = DeviceArray.from_numpy(rng.standard_normal((16, 128, 128, 3)).astype(np.float32))
+    md = DeviceArray.from_numpy(np.ones((16, 128, 128), np.float32))
+    yd = DeviceArray((16, 128, 128, 3), np.float32)
+    for _ in range(max(1, args.warmup // 50)):
+        net.forward(xd, 16); net.backward(td, md, 16); net.adam(1e-4)
+    net.sync()
+    steps = max(1, args.steps // 100)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward(xd, 16); net.backward(td, md, 16); net.adam(1e-4)
+    net.sync()
+    dt = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward(xd, 16, yd)
+    dtf = (time.perf_counter() - t0) / steps
+    ffwd, ftrain = unet_flops()
+    net.close()
+    return common("U-Net training iteration (batch 16 x 128x128)", round(dt * 1e3, 3), "ms", args, steps,
+                  round(dt * 1e3, 3), "bf16", {"workload": "4-level 32-channel U-Net, 3x3 convs, batch 16 of 128x128, "
+                                                           "synthetic inputs; forward+backward+Adam"},
+                  higher=False, train_tflops=round(ftrain / dt / 1e12, 2), train_gflop=round(ftrain / 1e9, 1),
+                  forward_ms=round(dtf * 1e3, 3), forward_tflops=round(ffwd / dtf / 1e12, 2),
+                  mfma_peak_tflops=2500.0)
 
 
 def bench_copy(args):
@@ -236,7 +296,7 @@ def main():
         sys.exit(2)
     if args.workload != "sphere":
         fn = {"cavity": bench_cavity, "spinodal": bench_spinodal, "ensemble": bench_ensemble,
-              "warmstart": bench_warmstart, "copy": bench_copy}[args.workload]
+              "warmstart": bench_warmstart, "unet": bench_unet, "copy": bench_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     import numpy as np

@@ -172,20 +172,23 @@ __global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *
     out[e] = s;
 }
 
-// db[co] = sum_m dout[m, co]   (one block of 256 per 32 channels, fixed order)
-__global__ __launch_bounds__(256) void k_bias_grad(const float *dout, int npix, int Co, float *db) {
+// Bias-gradient partials: db[co] = sum_m dout[m, co], block (32-channel
+// group, pixel chunk); 8 pixel lanes x 32 channels, fixed order, then
+// k_sum_chunks over the chunks.
+__global__ __launch_bounds__(256) void k_bias_grad(const float *dout, int npix, int Co, int chunk, float *partial) {
     const int co = blockIdx.x * 32 + (threadIdx.x & 31);
     const int part = threadIdx.x >> 5;   // 8 pixel strides
+    const int pbeg = blockIdx.y * chunk, pend = min(npix, pbeg + chunk);
     float s = 0.f;
     if (co < Co)
-        for (int m = part; m < npix; m += 8) s += dout[(size_t)m * Co + co];
+        for (int m = pbeg + part; m < pend; m += 8) s += dout[(size_t)m * Co + co];
     __shared__ float sh[8][32];
     sh[part][threadIdx.x & 31] = s;
     __syncthreads();
     if (threadIdx.x < 32 && co < Co) {
         float t = 0.f;
         for (int p = 0; p < 8; p++) t += sh[p][threadIdx.x];
-        db[co] = t;
+        partial[(size_t)blockIdx.y * Co + co] = t;
     }
 }
 
@@ -274,12 +277,15 @@ __global__ void k_head_bwd_data(const float *dy, const bf16 *Wh, float *dx, int 
     for (int o = 0; o < O; o++) s += dy[m * O + o] * from_bf16(Wh[o * C + c]);
     dx[e] = s;
 }
-// dWh[o, c] and dbh[o]: one block per (o, c) pair plus bias, fixed-order tree.
-__global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 *x, int npix, int C, int O, float *dW,
-                                                    float *db) {
+// dWh[o, c] and dbh[o] partials: block (pair (o, c) or bias, pixel chunk),
+// fixed-order tree; k_sum_chunks adds the chunks.
+__global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 *x, int npix, int C, int O, int chunk,
+                                                    float *partial) {
     const int o = blockIdx.x / (C + 1), c = blockIdx.x % (C + 1);
+    const int pbeg = blockIdx.y * chunk, pend = min(npix, pbeg + chunk);
     float s = 0.f;
-    for (int m = threadIdx.x; m < npix; m += 256) s += dy[(size_t)m * O + o] * (c < C ? from_bf16(x[(size_t)m * C + c]) : 1.f);
+    for (int m = pbeg + threadIdx.x; m < pend; m += 256)
+        s += dy[(size_t)m * O + o] * (c < C ? from_bf16(x[(size_t)m * C + c]) : 1.f);
     __shared__ float sh[256];
     sh[threadIdx.x] = s;
     __syncthreads();
@@ -287,10 +293,15 @@ __global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 
         if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        if (c < C) dW[o * C + c] = sh[0];
-        else db[o] = sh[0];
-    }
+    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
+}
+// partial (o, c) index -> dW [o][c], bias (o, C) -> db[o]
+__global__ void k_head_scatter(const float *sum, int C, int O, float *dW, float *db) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= O * (C + 1)) return;
+    const int o = e / (C + 1), c = e % (C + 1);
+    if (c < C) dW[o * C + c] = sum[e];
+    else db[o] = sum[e];
 }
 
 // Masked mean-squared error: dy = 2 (y - t) mask / count, loss partials.
@@ -313,11 +324,19 @@ __global__ __launch_bounds__(256) void k_mse(const float *y, const float *t, con
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
-__global__ void k_sum_d(const double *p, int n, double *out) {
-    if (threadIdx.x) return;
+// Sum of n doubles by one 256-thread block: strided per-thread sums, then a
+// fixed tree (deterministic).
+__global__ __launch_bounds__(256) void k_sum_d(const double *p, int n, double *out) {
     double s = 0;
-    for (int i = 0; i < n; i++) s += p[i];
-    *out = s;
+    for (int i = threadIdx.x; i < n; i += 256) s += p[i];
+    __shared__ double sh[256];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
 }
 
 // Adam on FP32 masters, then refresh the BF16 copy.
@@ -690,12 +709,20 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     if (cnt <= 0) return LB_ERR_INVALID;
     const unsigned nbl = nb256((size_t)npix0 * O);
     hipLaunchKernelGGL(k_mse, dim3(nbl), dim3(256), 0, u->stream, u->y, d_t, d_mask, u->dy, npix0, O, (float)(1.0 / cnt), u->lpart);
-    hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(64), 0, u->stream, u->lpart, (int)nbl, u->lsum);
+    hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(256), 0, u->stream, u->lpart, (int)nbl, u->lsum);
     // head
     const int C0 = L[0].C;
     const int last = (int)u->convs.size() - 1;
-    hipLaunchKernelGGL(k_head_bwd_w, dim3(O * (C0 + 1)), dim3(256), 0, u->stream, u->dy, u->act[last], npix0, C0, O,
-                       u->g + u->head_woff, u->g + u->head_boff);
+    {
+        // partials in the weight-gradient buffer (free until the first wgrad_launch)
+        const int chunk = 4096, nchunk = (npix0 + chunk - 1) / chunk, np = O * (C0 + 1);
+        hipLaunchKernelGGL(k_head_bwd_w, dim3(np, nchunk), dim3(256), 0, u->stream, u->dy, u->act[last], npix0, C0, O, chunk,
+                           u->partial);
+        hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial, nchunk, (size_t)np,
+                           u->partial + (size_t)nchunk * np);
+        hipLaunchKernelGGL(k_head_scatter, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial + (size_t)nchunk * np, C0, O,
+                           u->g + u->head_woff, u->g + u->head_boff);
+    }
     hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0)), dim3(256), 0, u->stream, u->dy, u->wb + u->head_woff,
                        u->gA, npix0, C0, O);
     // gA = d(out of current conv), post-activation gradient
@@ -706,7 +733,14 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const int npix = B * L[l].H * L[l].W;
         size_t n = (size_t)npix * c.co;
         hipLaunchKernelGGL(k_relu_bwd, dim3(nb256(n)), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, n);
-        hipLaunchKernelGGL(k_bias_grad, dim3((c.co + 31) / 32), dim3(256), 0, u->stream, gpre, npix, c.co, u->g + c.boff);
+        {
+            // the weight-gradient partial buffer is free until wgrad_launch below
+            const int chunk = 1024, nchunk = (npix + chunk - 1) / chunk;
+            hipLaunchKernelGGL(k_bias_grad, dim3((c.co + 31) / 32, nchunk), dim3(256), 0, u->stream, gpre, npix, c.co, chunk,
+                               u->partial);
+            hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(c.co)), dim3(256), 0, u->stream, u->partial, nchunk, (size_t)c.co,
+                               u->g + c.boff);
+        }
         wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co);
         if (need_din)
             conv_launch<float, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);

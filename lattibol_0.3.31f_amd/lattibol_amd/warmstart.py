@@ -134,7 +134,7 @@ def velocity_l2(a, b, mask):
     return float(np.linalg.norm(da) / max(np.linalg.norm(nb), 1e-30))
 
 
-def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128, 128), init_mode=0):
+def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128, 128), init_mode=0, extra_modes=()):
     """Warm start vs plain start from rest on n held-out channels.
 
     Time to steady state exists only for channels whose flow has a steady
@@ -160,7 +160,17 @@ def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128,
         pr = dict(ux=u * (y[i, ..., 0] * net.stats[1] + net.stats[0]), uy=u * (y[i, ..., 1] * net.stats[3] + net.stats[2]))
         pred_err.append(velocity_l2(pr, plain["macros"][i], 1 - specs[i]["solid"]))
     pred_err = np.array(pred_err)
-    return dict(
+    extra = {}
+    for mode in extra_modes:
+        w2 = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=mode)
+        b2 = (plain["res"] < tol) & (w2["res"] < tol)
+        l2b = np.array([velocity_l2(w2["macros"][i], plain["macros"][i], 1 - specs[i]["solid"]) for i in range(m)])
+        extra["init_mode_%d" % mode] = dict(
+            converged_warm=int((w2["res"] < tol).sum()), converged_both=int(b2.sum()),
+            step_ratio=float(w2["steps"][b2].sum() / max(plain["steps"][b2].sum(), 1)),
+            wall_warm=w2["wall"], wall_ratio=float(w2["wall"] / plain["wall"]),
+            l2_max=float(l2b[b2].max()) if b2.any() else None)
+    return dict(extra=extra,
         n=n, steady=m, unsteady_or_diverged=n - m, init_mode=init_mode,
         converged_plain=int((plain["res"] < tol).sum()), converged_warm=int((warm["res"] < tol).sum()),
         converged_both=int(both.sum()),
