@@ -342,6 +342,40 @@ int lb_unet_to_fields(lb_unet *u, const float *d_y, int B, const float *d_uin, c
 int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B, int H,
                     int W, int Ci, int Co, lb_stream_t stream);
 
+/* ---------------------------------------------------------------------- */
+/* 5. Input decks and VTK output (host only)                                */
+/* ---------------------------------------------------------------------- */
+/* A parameter file (.dat) as Input::loadParameters reads it (input.cpp:51-97):
+ * device count, maxDim, ux, uy, uz, rho, viscosity, updateSteps, Fx, Fy, Fz,
+ * G, Gads, one per line; lines past the end read as 0.  max_dim <= 1 means a
+ * 2D deck whose solids come from the BMP of the same name, > 1 a 3D deck
+ * voxelised from the STL of the same name.  tau_inv = 1/(3 nu + 1/2)
+ * (input.cpp:168-171); the colour-map bounds follow input.cpp:90-93. */
+typedef struct {
+    int device_count, max_dim, update_steps;
+    float ux, uy, uz, rho, viscosity, tau_inv;
+    float fx, fy, fz, G, Gads;
+    float min_u, max_u, min_rho, max_rho;
+} lb_deck_params;
+int lb_deck_read_params(const char *dat_path, lb_deck_params *out);
+/* Solids of a 2D deck (replaces Input::loadSolids, input.cpp:99-116): black
+ * pixels are solid, solids[x + width*y] with y = 0 the top row.  With
+ * solids == NULL only the size is returned.  Uncompressed BMP, 1/4/8/24/32
+ * bits per pixel. */
+int lb_deck_read_bmp(const char *bmp_path, int *width, int *height, uint8_t *solids);
+/* Solids of a 3D deck (replaces Input::loadSolidsSTL, input.cpp:118-142): the
+ * mesh (binary or ASCII STL) is scaled so its longest extent spans max_dim
+ * voxels; a voxel is solid when its centre is inside the closed surface.
+ * solids[x + width*(y + height*z)]; NULL returns the size only. */
+int lb_deck_voxelize_stl(const char *stl_path, int max_dim, int *width, int *height, int *depth,
+                         uint8_t *solids);
+/* VTK image data in the reference's layout (Output::saveParVTK / saveVTK,
+ * output.cpp:52-155): dir/r<id>.pvti indexing device_count pieces
+ * dir/r<id>p<dev>.vti, each with rho ("Pressure") and the velocity. */
+int lb_write_pvti(const char *dir, int vtk_id, int device_count, int width, int height, int length);
+int lb_write_vti(const char *dir, int vtk_id, int device_id, int width, int height, int length,
+                 const float *ux, const float *uy, const float *uz, const float *rho);
+
 #ifdef __cplusplus
 }
 #endif

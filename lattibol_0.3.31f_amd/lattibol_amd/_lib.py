@@ -78,6 +78,14 @@ class LbUnetDesc(C.Structure):
                 ("height", C.c_int), ("width", C.c_int), ("max_batch", C.c_int)]
 
 
+class LbDeckParams(C.Structure):
+    _fields_ = [("device_count", C.c_int), ("max_dim", C.c_int), ("update_steps", C.c_int),
+                ("ux", C.c_float), ("uy", C.c_float), ("uz", C.c_float), ("rho", C.c_float),
+                ("viscosity", C.c_float), ("tau_inv", C.c_float), ("fx", C.c_float), ("fy", C.c_float),
+                ("fz", C.c_float), ("G", C.c_float), ("Gads", C.c_float), ("min_u", C.c_float),
+                ("max_u", C.c_float), ("min_rho", C.c_float), ("max_rho", C.c_float)]
+
+
 # ---------------------------------------------------------------------------
 # Loading and prototypes
 # ---------------------------------------------------------------------------
@@ -172,6 +180,11 @@ _PROTOS = {
     "lb_unet_sync": (_i, [_vp]),
     "lb_unet_to_fields": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "lb_conv3x3_bf16": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "lb_deck_read_params": (_i, [C.c_char_p, C.POINTER(LbDeckParams)]),
+    "lb_deck_read_bmp": (_i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+    "lb_deck_voxelize_stl": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
+    "lb_write_pvti": (_i, [C.c_char_p, _i, _i, _i, _i, _i]),
+    "lb_write_vti": (_i, [C.c_char_p, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
 }
 
 EXPORTED = sorted(_PROTOS)

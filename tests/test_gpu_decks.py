@@ -1,0 +1,70 @@
+"""The reference's sample decks as regression scenes (tests/golden/decks):
+loaded with the deck readers, run by the reference driver on the GPU, checked
+bit for bit against the oracle over a short run and, for the body-force
+channel, against the physics of a plane Poiseuille channel at steady state."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+DECKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "decks")
+
+
+def ref_oracle(d):
+    p = d["params"]
+    return O.RefOracle(d["solids"], deviceCount=max(1, p["device_count"]), ux=p["ux"], uy=p["uy"], uz=p["uz"],
+                       rho=p["rho"], viscosity=p["viscosity"], Fx=p["fx"], Fy=p["fy"], G=p["G"], Gads=p["Gads"])
+
+
+@pytest.mark.parametrize("name,steps", [("fig31_force", 40), ("fig52_angle90", 60)])
+def test_sample_deck_matches_oracle(gpu, name, steps):
+    from lattibol_amd import deck
+    d = deck.load(os.path.join(DECKS, name + ".dat"))
+    print("deck", name, d["solids"].shape, flush=True)
+    sim = deck.simulation(d)
+    ref = ref_oracle(d)
+    sim.runLBM(steps)
+    ref.runLBM(steps)
+    got = sim.download(0)
+    np.testing.assert_array_equal(got["f"].reshape(ref.Q, -1), ref.f[0])
+    np.testing.assert_array_equal(got["rho"].ravel(), ref.rho[0])
+    np.testing.assert_array_equal(got["ux"].ravel(), ref.ux[0])
+    sim.close()
+
+
+def test_fig31_channel_reaches_poiseuille(gpu, tmp_path):
+    """Velocity inlet u = 0.01 plus body force in a 1024 x 256 channel with
+    full-way bounce-back walls on rows 0 and 255: far from the ends the steady
+    profile is a parabola vanishing half-way between the wall rows and the
+    first fluid rows, and it carries the inlet's flux."""
+    import time
+    from lattibol_amd import deck
+    d = deck.load(os.path.join(DECKS, "fig31_force.dat"))
+    sim = deck.simulation(d)
+    t0 = time.perf_counter()
+    sim.runLBM(1000)
+    _ = sim.download(0)
+    per_step = (time.perf_counter() - t0) / 1000
+    steps = 200000
+    print("reference driver %.1f us/step on 1024x256, running %d steps" % (per_step * 1e6, steps), flush=True)
+    sim.runLBM(steps - 1000)
+    m = sim.download(0)
+    ux = m["ux"][0][:, 512].astype(np.float64)          # rows 0..255 at mid-channel
+    rows = np.arange(1, 255)
+    u = ux[1:255]
+    # least-squares parabola through the fluid rows
+    A = np.stack([rows ** 2, rows, np.ones_like(rows)], 1).astype(np.float64)
+    coef, *_ = np.linalg.lstsq(A, u, rcond=None)
+    fit = A @ coef
+    assert np.abs(u - fit).max() < 0.01 * u.max(), np.abs(u - fit).max()
+    roots = np.sort(np.roots(coef).real)
+    assert abs(roots[0] - 0.5) < 0.6 and abs(roots[1] - 254.5) < 0.6, roots
+    flux = u.mean()
+    assert abs(flux - d["params"]["ux"]) < 0.02 * d["params"]["ux"], flux
+    # and the VTK writer on the result
+    deck.write_vtk(str(tmp_path), 0, {k: m[k] for k in ("rho", "ux", "uy", "uz")})
+    assert (tmp_path / "r0000.pvti").exists() and (tmp_path / "r0000p00.vti").exists()
+    sim.close()
