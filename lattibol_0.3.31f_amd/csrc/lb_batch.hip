@@ -51,7 +51,7 @@ inline lb_config lattice_config(int nx, int ny, float ti, float u_in) {
 
 // Argument block of lattice L reading buffer `src` (A = 0 / B = 1).
 StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, const uint8_t *flags,
-                              const uint32_t *lmask, float *macros) {
+                              const uint32_t *lmask, float *macros, int layout) {
     const lb_config c = lattice_config(L.nx, L.ny, L.ti, L.u_in);
     StepArgs<StF32S> a{};
     float *S0 = src == 0 ? A : B, *S1 = src == 0 ? B : A;
@@ -69,7 +69,7 @@ StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, co
         a.bc[k] = c.bc[k];
         a.bcrho[k] = c.bc_rho[k];
     }
-    const PopLayout pl = pop_layout(LB_LAYOUT_SOA, 9, L.nx, L.ny, 1);
+    const PopLayout pl = pop_layout(layout, 9, L.nx, L.ny, 1);
     a.ps = pl.ps; a.sy = pl.sy; a.sz = pl.sz;
     face_corrections(c, pl, a.corr_lo, a.corr_hi);
     a.zou_he = 1; a.zh_ux = c.zh_ux_in; a.zh_rho = c.zh_rho_out;
@@ -80,12 +80,12 @@ StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, co
     return a;
 }
 
-template <bool MACROS>
+template <bool MACROS, int MEM>
 __global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32S> *__restrict__ args,
                                                          const int2 *__restrict__ table, int src) {
     const int2 e = table[blockIdx.x];          // (lattice, local tile)
     const StepArgs<StF32S> &a = args[2 * e.x + src];
-    step_body<9, StF32S, false, MACROS>(a, node_of(a, e.y), nullptr);
+    step_body<9, StF32S, false, MACROS, MEM>(a, node_of(a, e.y), nullptr);
 }
 
 // Per-tile residual partials of the velocity change since the snapshot, plus
@@ -165,7 +165,8 @@ __device__ float grad_1d(const float *u, uint32_t id, int stride, bool has_m, bo
 }
 
 __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, const int2 *table, float *A, float *B,
-                                                         const float *fields, const uint8_t *flags, int neq) {
+                                                         const float *fields, const uint8_t *flags, int neq,
+                                                         int layout) {
     const int2 e = table[blockIdx.x];
     const BatchLat L = lats[e.x];
     int tx = e.y % L.ntx, ty = e.y / L.ntx;
@@ -196,9 +197,11 @@ __global__ __launch_bounds__(kThreads) void k_batch_init(const BatchLat *lats, c
             f[i] = f[i] - 3.0f * weight<9, float>(Vel<9>::wc(i)) * rho * tau_m1 * qs;
         });
     }
+    const PopLayout pl = pop_layout(layout, 9, L.nx, L.ny, 1);
+    const uint32_t e0 = (uint32_t)x + pl.sy * (uint32_t)y;
     for (int i = 0; i < 9; i++) {
-        A[L.pop_off + (size_t)i * L.N + id] = f[i];
-        B[L.pop_off + (size_t)i * L.N + id] = f[i];
+        A[L.pop_off + (size_t)i * pl.ps + e0] = f[i];
+        B[L.pop_off + (size_t)i * pl.ps + e0] = f[i];
     }
 }
 
@@ -226,6 +229,8 @@ struct lb_batch {
     hipGraphExec_t graph = nullptr;
     int graph_k = 0, graph_parity = -1;
     std::vector<char> macros_valid;
+    int layout = LB_LAYOUT_SOA;    // population layout of every lattice (autotuned)
+    int mem = 0;                   // load / store hints (autotuned, see ldm / stm)
 
     ~lb_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -260,8 +265,53 @@ int rebuild_table(lb_batch *b, bool all_lattices = false) {
 }
 
 void launch_batch_step(lb_batch *b, int parity, bool macros) {
-    if (macros) hipLaunchKernelGGL(k_batch_step<true>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_args, b->d_table, parity);
-    else hipLaunchKernelGGL(k_batch_step<false>, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_args, b->d_table, parity);
+    const dim3 g(b->nblocks), t(kThreads);
+    if (macros) hipLaunchKernelGGL((k_batch_step<true, 0>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
+    else if (b->mem == 1) hipLaunchKernelGGL((k_batch_step<false, 1>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
+    else if (b->mem == 2) hipLaunchKernelGGL((k_batch_step<false, 2>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
+    else hipLaunchKernelGGL((k_batch_step<false, 0>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
+}
+
+// Both argument blocks of every lattice for the current layout.
+int upload_batch_args(lb_batch *b) {
+    std::vector<StepArgs<StF32S>> hargs(2 * (size_t)b->n);
+    for (int i = 0; i < b->n; i++)
+        for (int src = 0; src < 2; src++)
+            hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->lmask, b->macros, b->layout);
+    int rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32S>),
+                                 hipMemcpyHostToDevice, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    return rc;
+}
+
+// Layout x load/store-hint autotuner over a few steps of the whole batch
+// (at creation, before any populations are set: nothing to relayout).
+int autotune_batch(lb_batch *b) {
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
+    float best = 1e30f;
+    int best_layout = 0, best_mem = 0;
+    for (int lay = 0; lay < 2; lay++)
+        for (int mem = 0; mem < 3; mem++) {
+            b->layout = lay;
+            b->mem = mem;
+            if (int rc = upload_batch_args(b)) return rc;
+            launch_batch_step(b, 0, false);
+            (void)hipEventRecord(e0, b->stream);
+            for (int r = 0; r < 6; r++) launch_batch_step(b, r & 1, false);
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; best_layout = lay; best_mem = mem; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->layout = best_layout;
+    b->mem = best_mem;
+    int rc = (int)hipGetLastError();
+    if (!rc) rc = upload_batch_args(b);
+    return rc;
 }
 
 void launch_batch_residual(lb_batch *b) {
@@ -325,16 +375,14 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 3 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
     if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
-    std::vector<StepArgs<StF32S>> hargs(2 * (size_t)n);
-    if (!rc)
-        for (int i = 0; i < n; i++)
-            for (int src = 0; src < 2; src++) hargs[2 * i + src] = lattice_args(b->lats[i], b->A, b->B, src, b->flags, b->lmask, b->macros);
-    if (!rc) rc = (int)hipMemcpyAsync(b->d_args, hargs.data(), hargs.size() * sizeof(StepArgs<StF32S>), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->lmask, lmask_all.data(), node * 4, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemsetAsync(b->macros, 0, 4 * node * sizeof(float), b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    if (!rc) rc = (int)hipMemsetAsync(b->A, 0, pop * sizeof(float), b->stream);
+    if (!rc) rc = (int)hipMemsetAsync(b->B, 0, pop * sizeof(float), b->stream);
     if (!rc) rc = rebuild_table(b);
+    if (!rc) rc = autotune_batch(b);
     if (!rc) rc = lb_batch_init_rest(b);
     if (rc) { delete b; return rc; }
     *out = b;
@@ -357,7 +405,7 @@ static int batch_init(lb_batch *b, const float *fields, int neq) {
     int rc = rebuild_table(b);
     if (rc) return rc;
     hipLaunchKernelGGL(k_batch_init, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->A, b->B, fields,
-                       b->flags, neq);
+                       b->flags, neq, b->layout);
     rc = (int)hipGetLastError();
     // the first residual check only primes the snapshot, but its value must
     // still be finite: start the snapshot from zero velocities
@@ -466,8 +514,20 @@ extern "C" int lb_batch_get_populations(lb_batch *b, int i, float *host) {
     if (!b || !host || i < 0 || i >= b->n) return LB_ERR_INVALID;
     const BatchLat &L = b->lats[i];
     const float *src = (b->cur[i] == 0 ? b->A : b->B) + L.pop_off;
-    int rc = (int)hipMemcpyAsync(host, src, 9ull * L.N * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+    if (b->layout == LB_LAYOUT_SOA) {
+        int rc = (int)hipMemcpyAsync(host, src, 9ull * L.N * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+        if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+        return rc;
+    }
+    // rows layout: [y][i][x] on the device, [i][y][x] for the caller
+    std::vector<float> tmp(9ull * L.N);
+    int rc = (int)hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, b->stream);
     if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    if (!rc)
+        for (int y = 0; y < L.ny; y++)
+            for (int q = 0; q < 9; q++)
+                std::memcpy(host + (size_t)q * L.N + (size_t)y * L.nx, &tmp[((size_t)y * 9 + q) * L.nx],
+                            L.nx * sizeof(float));
     return rc;
 }
 

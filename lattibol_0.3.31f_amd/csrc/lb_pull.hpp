@@ -70,7 +70,7 @@ struct NodePos { int x, y, z; uint32_t id, e; bool valid; };
 // contiguous run and reads Q short runs from at most 9 rows: far fewer DRAM
 // pages open at once than 2Q independent streams.
 struct PopLayout { uint32_t ps, sy, sz; };
-inline PopLayout pop_layout(int layout, int Q, int nx, int ny, int nz) {
+__host__ __device__ inline PopLayout pop_layout(int layout, int Q, int nx, int ny, int nz) {
     const uint32_t N = (uint32_t)nx * ny * nz;
     if (layout == LB_LAYOUT_ROWS) return {(uint32_t)nx, (uint32_t)Q * nx, (uint32_t)Q * nx * ny};
     return {N, (uint32_t)nx, (uint32_t)nx * ny};
