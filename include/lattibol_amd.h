@@ -213,6 +213,9 @@ typedef struct {
     int mem_mode;           /* population loads/stores: -1 autotune, 0 plain,
                                1 non-temporal stores, 2 non-temporal loads + stores */
     int layout;             /* -1 autotune, LB_LAYOUT_SOA, LB_LAYOUT_ROWS */
+    int round_mode;         /* FP16 / BF16 stores: 0 round to nearest even, 1 stochastic
+                               rounding (unbiased; counter-based random bits per step,
+                               node and population, reproducible) */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

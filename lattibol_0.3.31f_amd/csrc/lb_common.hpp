@@ -116,6 +116,40 @@ LB_DEV uint16_t f32_to_f16(float f) {
 }
 LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
+// Stochastic rounding of 16-bit stores.  Random bits per (step, node, slot):
+// the SplitMix64 finaliser of step * golden + node, then Wellons' lowbias32
+// of its low word plus slot * 0x9E3779B9 (a counter-based stream: the same
+// bits on any device, launch shape or thread count).  The value rounds up in
+// magnitude with probability (distance to the lower neighbour) / spacing, so
+// the stored value is unbiased and increments below half a spacing survive.
+LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) {
+    uint64_t z = 0x5EEDull + (uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)id;
+    z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
+    uint32_t x = (uint32_t)z + (uint32_t)slot * 0x9E3779B9u;
+    x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
+    return x;
+}
+// BF16: add 16 random bits below the kept ones and truncate.
+LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f32_to_bf16(f);
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+// FP16: the same with 13 dropped bits in the normal range (the truncated
+// value is exact in FP16); below 2^-14 the spacing is 2^-24:
+// floor(|v| 2^24 + u), u in [0, 1) from 23 random bits, exact in double.
+LB_DEV uint16_t f32_to_f16_sr(float f, uint32_t r) {
+    const uint32_t x = __builtin_bit_cast(uint32_t, f);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax >= 0x477ff000u) return f32_to_f16(f);
+    if (ax < 0x38800000u) {
+        const double q = (double)__builtin_bit_cast(float, ax) * 16777216.0 + (double)(r >> 9) * (1.0 / 8388608.0);
+        return (uint16_t)(sign | (uint32_t)q);
+    }
+    const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
+    return (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
+}
+
 // DEV = false: the population itself is stored and the arithmetic follows the
 // reference expressions.  DEV = true: the deviation s_i = f_i - w_i is stored
 // and the whole collision runs on deviations (moments, equilibria, boundary

@@ -26,17 +26,26 @@ namespace {
 // The argument block lives in device memory (uniform pointer -> scalar loads);
 // passed by value it would be copied to per-lane scratch because the device
 // functions index its arrays.
+// dstep: position of this sweep in the launch sequence since the step counter
+// of the stochastic-rounding stream was last advanced (k_bump).
 template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
-__global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa) {
+__global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa, uint32_t dstep) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const StepArgs<S> &a = *pa;
     const int b = tile_of_block(a);
-    const NodePos p = node_of(a, b);
+    NodePos p = node_of(a, b);
+    if (a.round_mode) p.step = ldm(&a.ctr[0]) + dstep;
     if constexpr (SC) {
         load_psi_tile<Q, S>(a, b, (typename S::T *)lds_raw, 1 << a.bx_shift, a.by);
         __syncthreads();
     }
     step_body<Q, S, SC, MACROS, MEM>(a, p, lds_raw);
+}
+
+// Advances the step counter of the stochastic-rounding stream after a launch
+// sequence of n sweeps (one thread).
+__global__ void k_bump(uint32_t *ctr, uint32_t n) {
+    if (threadIdx.x == 0) ctr[0] += n;
 }
 
 template <int Q, class S>
@@ -207,6 +216,7 @@ struct lb_ctx {
     int cur = 0;
     uint8_t *flags = nullptr;
     uint32_t *lmask = nullptr;
+    uint32_t *ctr = nullptr;          // step counter of the stochastic-rounding stream (+1 spare word)
     float *macros = nullptr;
     float *snap = nullptr;
     void *psi = nullptr;
@@ -228,8 +238,8 @@ struct lb_ctx {
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
-        void *ps[] = {buf[0], buf[1], flags, lmask, macros, snap, psi, halo[0], halo[1], partial, dout,
-                      tmp_fields, dargs};
+        void *ps[] = {buf[0], buf[1], flags, lmask, ctr, macros, snap, psi, halo[0], halo[1], partial,
+                      dout, tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -269,6 +279,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     for (int d = 0; d < 3; d++) a.force[d] = c->cfg.force[d];
     a.scG = c->cfg.sc_G; a.scrho0 = c->cfg.sc_rho0; a.scpsi_solid = c->cfg.sc_psi_solid;
     a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he; a.rest_closure = c->cfg.rest_closure;
+    a.round_mode = (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16) ? c->cfg.round_mode : 0;
+    a.ctr = c->ctr;
     a.halo = c->cfg.halo_z && c->Q == 19;
     a.zh_ux = c->cfg.zh_ux_in; a.zh_rho = c->cfg.zh_rho_out;
     a.bx_shift = c->bx_shift; a.by = c->by;
@@ -300,7 +312,7 @@ int upload_args(lb_ctx *c) {
 }
 
 template <int Q, class S>
-int launch_step_t(lb_ctx *c, int src, bool macros) {
+int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     StepArgs<S> a = make_args<S>(c, src);   // host copy: launch geometry only
     const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
@@ -310,24 +322,24 @@ int launch_step_t(lb_ctx *c, int src, bool macros) {
         hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
     }
     if (sc) {
-        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa);
-        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa);
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
+        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
     } else {
-        if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
-        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, false, false, 1>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
-        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, false, false, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
-        else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, false, false, 1>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, false, false, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else hipLaunchKernelGGL((k_step<Q, S, false, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
     }
     return (int)hipGetLastError();
 }
 
-template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros) {
+template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros, uint32_t dstep = 0) {
     switch (c->cfg.storage) {
-    case LB_STORE_F32: return launch_step_t<Q, StF32>(c, src, macros);
-    case LB_STORE_F32S: return launch_step_t<Q, StF32S>(c, src, macros);
-    case LB_STORE_F16: return launch_step_t<Q, StF16>(c, src, macros);
-    case LB_STORE_BF16: return launch_step_t<Q, StBF16>(c, src, macros);
-    default: return launch_step_t<Q, StF64>(c, src, macros);
+    case LB_STORE_F32: return launch_step_t<Q, StF32>(c, src, macros, dstep);
+    case LB_STORE_F32S: return launch_step_t<Q, StF32S>(c, src, macros, dstep);
+    case LB_STORE_F16: return launch_step_t<Q, StF16>(c, src, macros, dstep);
+    case LB_STORE_BF16: return launch_step_t<Q, StBF16>(c, src, macros, dstep);
+    default: return launch_step_t<Q, StF64>(c, src, macros, dstep);
     }
 }
 
@@ -338,8 +350,16 @@ template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros) {
 // the exchange between the two is the identity and alternating the two
 // buffers replaces the copy.  A ring of several slabs moves the planes
 // between sweeps instead (see lb_halo_buffers).
-int launch_step(lb_ctx *c, int src, bool macros) {
-    return c->Q == 9 ? launch_step_q<9>(c, src, macros) : launch_step_q<19>(c, src, macros);
+int launch_step(lb_ctx *c, int src, bool macros, uint32_t dstep = 0) {
+    return c->Q == 9 ? launch_step_q<9>(c, src, macros, dstep) : launch_step_q<19>(c, src, macros, dstep);
+}
+
+// After n sweeps launched with dstep 0..n-1: advance the rounding stream.
+int launch_bump(lb_ctx *c, uint32_t n) {
+    const bool sr = c->cfg.round_mode && (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16);
+    if (!sr || n == 0) return 0;
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, c->stream, c->ctr, n);
+    return (int)hipGetLastError();
 }
 
 template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
@@ -424,7 +444,8 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
             int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (rc) return rc;
             int src = c->cur;
-            for (int s = 0; s < chunk; s++) { launch_step(c, src, false); src = 1 - src; }
+            for (int s = 0; s < chunk; s++) { launch_step(c, src, false, (uint32_t)s); src = 1 - src; }
+            launch_bump(c, (uint32_t)chunk);
             rc = (int)hipStreamEndCapture(c->stream, &g);
             if (rc) return rc;
             rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -439,6 +460,7 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
     }
     for (int s = 0; s < nsteps; s++) {
         int rc = launch_step(c, c->cur, false);
+        if (!rc) rc = launch_bump(c, 1);
         if (rc) return rc;
         c->cur = 1 - c->cur;
         c->steps++;
@@ -511,7 +533,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
-    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS) return LB_ERR_INVALID;
+    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1)
+        return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
     c->cfg = *cfg;
@@ -527,6 +550,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipMalloc(&c->buf[1], pop_bytes);
     if (!rc) rc = (int)hipMalloc((void **)&c->flags, c->N);
     if (!rc) rc = (int)hipMalloc((void **)&c->lmask, (size_t)c->N * 4);
+    if (!rc) rc = (int)hipMalloc((void **)&c->ctr, 2 * sizeof(uint32_t));
+    if (!rc) rc = (int)hipMemsetAsync(c->ctr, 0, 2 * sizeof(uint32_t), c->stream);
     if (!rc) rc = (int)hipMalloc((void **)&c->macros, (size_t)4 * c->N * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&c->snap, (size_t)3 * c->N * sizeof(float));
     if (!rc) rc = (int)hipMalloc((void **)&c->partial, (size_t)kReduceBlocks * 4 * sizeof(double));
@@ -598,6 +623,8 @@ extern "C" int lb_init_equilibrium_device(lb_ctx *c, const float *r, const float
     c->steps = 0;
     c->macros_valid = false;
     c->snap_valid = false;
+    rc = (int)hipMemsetAsync(c->ctr, 0, 2 * sizeof(uint32_t), c->stream);
+    if (rc) return rc;
     if (c->halo[0]) {
         rc = (int)hipMemsetAsync(c->halo[0], 0, (size_t)10 * c->P * c->esz, c->stream);
         if (rc) return rc;
@@ -646,6 +673,7 @@ extern "C" int lb_set_populations(lb_ctx *c, const void *host) {
         if (tmp) (void)hipFree(tmp);
     }
     if (!rc) rc = (int)hipMemcpyAsync(c->buf[1], c->buf[0], bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (!rc) rc = (int)hipMemsetAsync(c->ctr, 0, 2 * sizeof(uint32_t), c->stream);
     c->cur = 0;
     c->steps = 0;
     c->macros_valid = false;
@@ -678,6 +706,7 @@ extern "C" int lb_step(lb_ctx *c, int nsteps) {
     int rc = run_graph_or_eager(c, nsteps - 1);
     if (rc) return rc;
     rc = launch_step(c, c->cur, true);
+    if (!rc) rc = launch_bump(c, 1);
     if (rc) return rc;
     c->cur = 1 - c->cur;
     c->steps++;
@@ -761,10 +790,11 @@ extern "C" int lb_time_step_kernel(lb_ctx *c, int n, float *ms) {
     if (rc) return rc;
     int src = c->cur;
     for (int s = 0; s < n; s++) {
-        if (c->Q == 9) launch_step_q<9>(c, src, false);
-        else launch_step_q<19>(c, src, false);
+        if (c->Q == 9) launch_step_q<9>(c, src, false, (uint32_t)s);
+        else launch_step_q<19>(c, src, false, (uint32_t)s);
         src = 1 - src;
     }
+    launch_bump(c, (uint32_t)n);
     rc = (int)hipStreamEndCapture(c->stream, &g);
     if (rc) return rc;
     rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);

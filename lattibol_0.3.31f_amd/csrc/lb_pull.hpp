@@ -16,6 +16,7 @@
 #pragma once
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "lb_common.hpp"
@@ -48,6 +49,8 @@ template <class S> struct StepArgs {
     float scG, scrho0, scpsi_solid;
     int bb_full, zou_he, halo;
     int rest_closure;             // feq_0 = rho - sum_{i>0} feq_i (exact mass balance)
+    int round_mode;               // 16-bit stores: 0 nearest-even, 1 stochastic
+    uint32_t *ctr;                // step counter of the stochastic-rounding stream
     float zh_ux, zh_rho;
     int bx_shift, by, ntx, nty, nblocks, xcd;
 };
@@ -62,7 +65,7 @@ LB_DEV int xcd_remap(int b, int nb) {
 
 // id: node index x + nx (y + ny z) (flags, macros, psi); e: element offset of
 // the node's population 0 in the population buffers.
-struct NodePos { int x, y, z; uint32_t id, e; bool valid; };
+struct NodePos { int x, y, z; uint32_t id, e; bool valid; uint32_t step; };
 
 // Population layouts.  LB_LAYOUT_SOA: [i][z][y][x], element i*N + id.
 // LB_LAYOUT_ROWS: [z][y][i][x], the Q populations of one x-row stored back to
@@ -103,6 +106,7 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     p.valid = p.x < a.nx && p.y < a.ny;
     p.id = (uint32_t)p.x + (uint32_t)a.nx * ((uint32_t)p.y + (uint32_t)a.ny * (uint32_t)z);
     p.e = (uint32_t)p.x + a.sy * (uint32_t)p.y + a.sz * (uint32_t)z;
+    p.step = 0;
     return p;
 }
 
@@ -214,6 +218,18 @@ template <int Q, class T> LB_DEV void moments(const T *f, T &rho, T &ux, T &uy, 
     else mom19(f, rho, ux, uy, uz);
 }
 
+// Stored value of population slot `slot` of node p: nearest-even, or
+// stochastic rounding for the 16-bit deviation storages.
+template <class S>
+LB_DEV typename S::raw store_val(const StepArgs<S> &a, const NodePos &p, typename S::T v, float w, int slot) {
+    if constexpr (std::is_same<S, StF16>::value) {
+        if (a.round_mode) return f32_to_f16_sr(v, sr_bits(p.step, p.id, slot));
+    } else if constexpr (std::is_same<S, StBF16>::value) {
+        if (a.round_mode) return f32_to_bf16_sr(v, sr_bits(p.step, p.id, slot));
+    }
+    return S::st(v, w);
+}
+
 // REV (full-way bounce-back at a solid node): slot opp(i) <- f_i.  A
 // compile-time flag: a runtime choice between f[i] and f[opp(i)] becomes a
 // dynamically indexed private array, i.e. per-lane scratch memory.
@@ -222,8 +238,9 @@ LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S:
     typedef Vel<Q> L;
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        auto v = S::st(f[i], weight<Q, float>(L::wc(i)));
-        stm<MEM>(&a.B[(size_t)(REV ? L::opp(i) : i) * a.ps + p.e], v);
+        constexpr int slot = REV ? L::opp(i) : i;
+        auto v = store_val<S>(a, p, f[i], weight<Q, float>(L::wc(slot)), slot);
+        stm<MEM>(&a.B[(size_t)slot * a.ps + p.e], v);
     });
     if constexpr (Q == 19) {
         if (a.halo && (p.z == a.nz - 1 || p.z == 0)) {
@@ -235,7 +252,7 @@ LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S:
                 constexpr int cz = L::cz(i);
                 if constexpr (cz != 0) {
                     if ((cz > 0 && p.z == a.nz - 1) || (cz < 0 && p.z == 0)) {
-                        auto v = S::st(f[REV ? L::opp(i) : i], weight<Q, float>(L::wc(i)));
+                        auto v = store_val<S>(a, p, f[REV ? L::opp(i) : i], weight<Q, float>(L::wc(i)), i);
                         stm(&a.send[(size_t)((cz > 0 ? 5 : 0) + zslot(i)) * a.P + xy], v);
                     }
                 }

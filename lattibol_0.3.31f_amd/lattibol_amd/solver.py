@@ -16,6 +16,7 @@ STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3, "f32s": 4}
 RAW_DTYPE = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64, 4: np.float32}
 FACES = ("-x", "+x", "-y", "+y", "-z", "+z")
 LAYOUT = {"auto": -1, "soa": 0, "rows": 1}
+ROUND = {"nearest": 0, "stochastic": 1}
 
 # Lattice weights as float32 (simulation_d2q9.cu:56-58, simulation_d3q19.cu:54-56).
 W9 = np.array([16 / 36] + [4 / 36] * 4 + [1 / 36] * 4, dtype=np.float64)
@@ -25,7 +26,7 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
                 zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
-                rest_closure=None, mem_mode=-1, layout=-1):
+                rest_closure=None, mem_mode=-1, layout=-1, round_mode=0):
     """Plain-dict description of a lattice; the oracle binding reads the same dict.
 
     rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
@@ -55,15 +56,17 @@ def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, 
                 bb_full=int(bool(bb_full)), zou_he=int(bool(zou_he)), zh_ux_in=float(zh_ux_in),
                 zh_rho_out=float(zh_rho_out), halo_z=int(bool(halo_z)), block_x=int(block_x),
                 xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)),
-                mem_mode=int(mem_mode), layout=LAYOUT.get(layout, layout))
+                mem_mode=int(mem_mode), layout=LAYOUT.get(layout, layout),
+                round_mode=ROUND.get(round_mode, round_mode))
 
 
 def to_lbconfig(d):
     c = LbConfig()
+    defaults = dict(block_x=0, xcd_remap=-1, graph_chunk=0, mem_mode=-1, layout=-1, round_mode=0)
     for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid", "bb_full",
               "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure",
-              "mem_mode", "layout"):
-        setattr(c, k, d[k])
+              "mem_mode", "layout", "round_mode"):
+        setattr(c, k, d[k] if k in d else defaults[k])
     for f in range(6):
         c.bc[f] = d["bc"][f]
         c.bc_rho[f] = d["bc_rho"][f]

@@ -166,6 +166,43 @@ uint16_t f2bf(float f) {
 float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
 
 /* ------------------------------------------------------------------------
+ * Stochastic rounding of the 16-bit stores (round_mode 1).  Random bits:
+ * one SplitMix64 finaliser per (step, node), then Wellons' lowbias32 per
+ * population slot -- a counter-based stream, so every run, thread count and
+ * device produces the same bits.  v rounds up (in magnitude) to the next
+ * representable value with probability equal to its distance from the lower
+ * one over the spacing: the expected stored value is v, so increments below
+ * half a spacing are not lost (nearest-even rounding drops them every step).
+ * ---------------------------------------------------------------------- */
+uint32_t sr_bits(uint32_t step, uint32_t id, int slot) {
+    uint64_t z = 0x5EEDull + (uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)id;
+    z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
+    uint32_t x = (uint32_t)z + (uint32_t)slot * 0x9E3779B9u;
+    x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
+    return x;
+}
+/* BF16: add 16 random bits below the kept ones, truncate. */
+uint16_t f2bf_sr(float f, uint32_t r) {
+    uint32_t u = fbits(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f2bf(f);   /* inf / NaN */
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+/* FP16: normal range as for BF16 with 13 dropped bits (the truncated value is
+ * exact in FP16); below 2^-14 the spacing is 2^-24: floor(|v| 2^24 + u),
+ * u in [0, 1) from 23 random bits, exact in double. */
+uint16_t f2h_sr(float f, uint32_t r) {
+    const uint32_t x = fbits(f);
+    const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
+    if (ax >= 0x477ff000u) return f2h(f);                     /* overflow, inf, NaN */
+    if (ax < 0x38800000u) {
+        const double q = (double)bitsf(ax) * 16777216.0 + (double)(r >> 9) * (1.0 / 8388608.0);
+        return (uint16_t)(sign | (uint32_t)q);
+    }
+    const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
+    return (uint16_t)(sign | f2h(bitsf(t)));
+}
+
+/* ------------------------------------------------------------------------
  * Storage policies for the pull specification.  T is the compute type.
  * Shifted storage keeps f_i - w_i (w_i the float lattice weight).
  * ---------------------------------------------------------------------- */
@@ -180,6 +217,18 @@ struct StF16 { typedef uint16_t raw; typedef float T; static const bool DEV = tr
     static T ld(raw v, float) { return h2f(v); } static raw st(T s, float) { return f2h(s); } };
 struct StBF16 { typedef uint16_t raw; typedef float T; static const bool DEV = true;
     static T ld(raw v, float) { return bf2f(v); } static raw st(T s, float) { return f2bf(s); } };
+
+/* Store of one population: nearest-even, or stochastic for 16-bit storage. */
+template <class S> typename S::raw store_pop(typename S::T v, float w, int mode, uint32_t step, uint32_t id, int slot) {
+    (void)mode; (void)step; (void)id; (void)slot;
+    return S::st(v, w);
+}
+template <> uint16_t store_pop<StF16>(float v, float w, int mode, uint32_t step, uint32_t id, int slot) {
+    return mode ? f2h_sr(v, sr_bits(step, id, slot)) : StF16::st(v, w);
+}
+template <> uint16_t store_pop<StBF16>(float v, float w, int mode, uint32_t step, uint32_t id, int slot) {
+    return mode ? f2bf_sr(v, sr_bits(step, id, slot)) : StBF16::st(v, w);
+}
 
 /* Lattice weights in the compute type: float weights for float compute
  * (as the reference), exact double quotients for the FP64 accuracy run. */
@@ -688,7 +737,8 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                         pull_node_pop<Q, S>(cfg, g, A, solid, p, id, dims, f);
                         for (int i = 0; i < Q; i++) {
                             int j = Lat<Q>::opp(i);
-                            rout[(size_t)j * nx + x] = S::st(f[i], (float)wq<Q, float>(j));
+                            rout[(size_t)j * nx + x] = store_pop<S>(f[i], (float)wq<Q, float>(j), cfg->round_mode,
+                                                                    cfg->step, (uint32_t)id, j);
                         }
                         wrote[x] = 1;
                     }
@@ -702,7 +752,9 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     if (S::DEV) feq_dev<Q, T>(f, rho - T(1), ux, uy, uz);
                     else if (Q == 9) feq9<T>(f, rho, ux, uy, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                     else feq19<T>(f, rho, ux, uy, uz, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
-                    for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
+                    for (int i = 0; i < Q; i++)
+                        rout[(size_t)i * nx + x] = store_pop<S>(f[i], (float)wq<Q, float>(i), cfg->round_mode,
+                                                                cfg->step, (uint32_t)id, i);
                     wrote[x] = 1;
                     if (macros) {
                         macros[id] = rho; macros[g.N + id] = ux; macros[2 * g.N + id] = uy; macros[3 * g.N + id] = uz;
@@ -769,7 +821,9 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                     for (int i = 0; i < Q; i++) f[i] = f[i] - ti * (f[i] - eq[i]);
                 } else if (Q == 9) bgk9<T>(f, rho, ex, ey, ti, wq<9, T>(0), wq<9, T>(1), wq<9, T>(5));
                 else bgk19<T>(f, rho, ex, ey, ez, ti, wq<19, T>(0), wq<19, T>(1), wq<19, T>(7));
-                for (int i = 0; i < Q; i++) rout[(size_t)i * nx + x] = S::st(f[i], (float)wq<Q, float>(i));
+                for (int i = 0; i < Q; i++)
+                    rout[(size_t)i * nx + x] = store_pop<S>(f[i], (float)wq<Q, float>(i), cfg->round_mode, cfg->step,
+                                                            (uint32_t)id, i);
                 wrote[x] = 1;
             }
             for (int i = 0; i < Q; i++) {
@@ -874,6 +928,9 @@ void orc_uniform_noise(uint64_t seed, int64_t n, double base, double amp, float 
 uint16_t orc_f32_to_f16(float f) { return f2h(f); }
 float orc_f16_to_f32(uint16_t h) { return h2f(h); }
 uint16_t orc_f32_to_bf16(float f) { return f2bf(f); }
+uint32_t orc_sr_bits(uint32_t step, uint32_t id, int slot) { return sr_bits(step, id, slot); }
+uint16_t orc_f32_to_f16_sr(float f, uint32_t r) { return f2h_sr(f, r); }
+uint16_t orc_f32_to_bf16_sr(float f, uint32_t r) { return f2bf_sr(f, r); }
 float orc_bf16_to_f32(uint16_t h) { return bf2f(h); }
 void orc_f32_to_f16_array(const float *in, uint16_t *out, int64_t n) { for (int64_t k = 0; k < n; k++) out[k] = f2h(in[k]); }
 void orc_f32_to_bf16_array(const float *in, uint16_t *out, int64_t n) { for (int64_t k = 0; k < n; k++) out[k] = f2bf(in[k]); }
@@ -1355,7 +1412,7 @@ void orc_pull_step(const orc_cfg *cfg, const void *A, void *B, const uint8_t *so
     ORC_DISPATCH(cfg, (pull_step_impl<Q, S>(cfg, (const S::raw *)A, (S::raw *)B, solid, (S::T *)macros, nullptr)));
 }
 
-void orc_pull_run(const orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps, void *macros) {
+void orc_pull_run(orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps, void *macros) {
     size_t esz = (cfg->storage == ORC_STORE_F32 || cfg->storage == ORC_STORE_F32S) ? 4 : (cfg->storage == ORC_STORE_F64 ? 8 : 2);
     size_t bytes = esz * (size_t)cfg->lattice * cfg->nx * cfg->ny * cfg->nz;
     void *a = A, *b = B;
@@ -1368,6 +1425,7 @@ void orc_pull_run(const orc_cfg *cfg, void *A, void *B, const uint8_t *solid, in
     for (int s = 0; s < steps; s++) {
         void *mac = s == steps - 1 ? macros : nullptr;
         ORC_DISPATCH(cfg, (pull_step_impl<Q, S>(cfg, (const S::raw *)a, (S::raw *)b, solid, (S::T *)mac, sp)));
+        cfg->step++;
         std::swap(a, b);
     }
     if (a != A) std::memcpy(A, a, bytes);

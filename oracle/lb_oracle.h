@@ -60,6 +60,8 @@ typedef struct {
     int zou_he;             /* 1: reference Zou-He inlet (x=0) / outlet (x=nx-1) */
     float zh_ux_in, zh_rho_out;
     int rest_closure;       /* 1: feq_0 = rho - sum_{i>0} feq_i */
+    int round_mode;         /* 16-bit stores: 0 nearest-even, 1 stochastic (orc_sr_bits) */
+    uint32_t step;          /* steps since initialisation (the stochastic-rounding stream) */
 } orc_cfg;
 
 /* ---- shared numerics -------------------------------------------------- */
@@ -69,6 +71,11 @@ void     orc_uniform_noise(uint64_t seed, int64_t n, double base, double amp, fl
 uint16_t orc_f32_to_f16(float f);
 float    orc_f16_to_f32(uint16_t h);
 uint16_t orc_f32_to_bf16(float f);
+/* Stochastic rounding: 32 random bits of population slot i of node id at
+ * time step `step`, and the rounding of v to FP16 / BF16 with them. */
+uint32_t orc_sr_bits(uint32_t step, uint32_t id, int slot);
+uint16_t orc_f32_to_f16_sr(float f, uint32_t r);
+uint16_t orc_f32_to_bf16_sr(float f, uint32_t r);
 float    orc_bf16_to_f32(uint16_t h);
 void     orc_f32_to_f16_array(const float *in, uint16_t *out, int64_t n);
 void     orc_f32_to_bf16_array(const float *in, uint16_t *out, int64_t n);
@@ -126,7 +133,7 @@ void orc_pull_init_eq(const orc_cfg *cfg, const double *rho, const double *ux,
 void orc_pull_step(const orc_cfg *cfg, const void *A, void *B, const uint8_t *solid,
                    void *macros);
 /* steps time steps; the final state is left in A. */
-void orc_pull_run(const orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps,
+void orc_pull_run(orc_cfg *cfg, void *A, void *B, const uint8_t *solid, int steps,
                   void *macros);
 /* Decode stored populations to float64 (adds w_i back for shifted storage). */
 void orc_pull_decode(const orc_cfg *cfg, const void *A, double *out);

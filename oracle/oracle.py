@@ -29,7 +29,8 @@ class OrcCfg(C.Structure):
                 ("bc_u", (C.c_float * 3) * 6), ("bc_rho", C.c_float * 6), ("force", C.c_float * 3),
                 ("sc_G", C.c_float), ("sc_rho0", C.c_float), ("sc_psi_solid", C.c_float),
                 ("bb_full", C.c_int), ("zou_he", C.c_int), ("zh_ux_in", C.c_float),
-                ("zh_rho_out", C.c_float), ("rest_closure", C.c_int)]
+                ("zh_rho_out", C.c_float), ("rest_closure", C.c_int), ("round_mode", C.c_int),
+                ("step", C.c_uint32)]
 
     @classmethod
     def from_dict(cls, d):
@@ -37,6 +38,8 @@ class OrcCfg(C.Structure):
         for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid",
                   "bb_full", "zou_he", "zh_ux_in", "zh_rho_out", "rest_closure"):
             setattr(c, k, d[k])
+        c.round_mode = int(d.get("round_mode", 0))
+        c.step = 0
         for f in range(6):
             c.bc[f] = d["bc"][f]
             c.bc_rho[f] = d["bc_rho"][f]
@@ -57,6 +60,9 @@ _PROTOS = {
     "orc_f32_to_f16": (C.c_uint16, [f_]),
     "orc_f16_to_f32": (f_, [C.c_uint16]),
     "orc_f32_to_bf16": (C.c_uint16, [f_]),
+    "orc_sr_bits": (C.c_uint32, [C.c_uint32, C.c_uint32, i_]),
+    "orc_f32_to_f16_sr": (C.c_uint16, [f_, C.c_uint32]),
+    "orc_f32_to_bf16_sr": (C.c_uint16, [f_, C.c_uint32]),
     "orc_bf16_to_f32": (f_, [C.c_uint16]),
     "orc_f32_to_f16_array": (None, [vp, vp, C.c_int64]),
     "orc_f32_to_bf16_array": (None, [vp, vp, C.c_int64]),
@@ -128,10 +134,12 @@ class PullOracle:
                 for a in (rho, ux, uy, uz)]
         lib.orc_pull_init_eq(C.byref(self.c), *[P(a) for a in arrs], P(self.A))
         self.B[...] = self.A
+        self.c.step = 0
 
     def set_populations(self, raw):
         self.A[...] = np.asarray(raw).reshape(self.A.shape)
         self.B[...] = self.A
+        self.c.step = 0
 
     def run(self, steps):
         lib.orc_pull_run(C.byref(self.c), P(self.A), P(self.B), P(self.solid), int(steps), P(self.macros_arr))

@@ -142,6 +142,10 @@ FUSED_CASES = {
     "sphere_halo_f32s": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f32s"), 300),
     "spinodal24_f32s": ("spinodal", dict(n=24, storage="f32s"), 300),
     "channel_zouhe_f32s": ("channel", dict(seed=3, index=0, size=(128, 64)), 1000),
+    "cavity48_bf16_sr": ("cavity", dict(n=48, storage="bf16", round_mode=1), 300),
+    "sphere_halo_f16_sr": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f16",
+                                          round_mode=1), 200),
+    "spinodal24_bf16_sr": ("spinodal", dict(n=24, storage="bf16", round_mode=1), 300),
 }
 
 
@@ -174,6 +178,23 @@ def test_fused_solver_matches_oracle(gpu, case, layout):
         np.testing.assert_array_equal(m[k][fluid], mo[k][fluid].astype(np.float32))
     # FP64 totals: GPU tree reduction vs the oracle's long-double sum
     np.testing.assert_allclose(sol.reduce(), orc.reduce(), rtol=1e-10, atol=1e-9)
+    sol.close()
+
+
+def test_stochastic_rounding_stream_continues_across_calls(gpu):
+    """The device step counter of the rounding stream survives graph chunks,
+    eager steps and separate lb_step calls: 100 + 77 steps equal the oracle's
+    177 bit for bit."""
+    from lattibol_amd import scenes
+    cfg, solid, init = scenes.spinodal(n=20, storage="f16", round_mode=1)
+    sol = gpu.Solver(cfg, solid)
+    orc = O.PullOracle(cfg, solid)
+    sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+    orc.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+    sol.step(100)
+    sol.step(77)
+    orc.run(177)
+    np.testing.assert_array_equal(sol.populations(), orc.A)
     sol.close()
 
 

@@ -91,3 +91,60 @@ def test_residual_norm_ratio():
     ref = math.sqrt(math.fsum(((a[:, m].astype(np.float64) - b[:, m]) ** 2).ravel())
                     / math.fsum((a[:, m].astype(np.float64) ** 2).ravel()))
     assert abs(got - ref) <= 1e-12 * ref
+
+
+# ---------------------------------------------------------------------------
+# Stochastic rounding of 16-bit stores (round_mode 1): exact unbiasedness.
+# Over every value of the random bits that the conversion uses, the number of
+# round-ups must equal the dropped fraction exactly (BF16: the low 16 mantissa
+# bits of v; FP16 normal: the low 13; FP16 subnormal: frac(|v| 2^24)), and
+# every result must be one of the two neighbours of v.
+# ---------------------------------------------------------------------------
+def _f32(b):
+    return np.array([b], np.uint32).view(np.float32)[0]
+
+
+def test_bf16_stochastic_rounding_is_exactly_unbiased():
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([rng.standard_normal(12), rng.standard_normal(6) * 1e-20, [1.0, -3.5, 0.0]]).astype(np.float32)
+    r_all = (np.arange(65536, dtype=np.uint64) << np.uint64(16)).astype(np.uint32)
+    for v in vals:
+        bits = int(np.array([v], np.float32).view(np.uint32)[0])
+        lo = bits & 0xFFFF0000
+        out = np.array([O.lib.orc_f32_to_bf16_sr(float(v), int(r)) for r in r_all[::257]], np.uint32)
+        assert set(np.unique(out).tolist()) <= {lo >> 16, (lo >> 16) + 1}
+        # exact count over the full range: round up iff r_hi >= 2^16 - low bits
+        ups = sum(1 for r in range(0, 65536) if ((bits + r) >> 16) > (bits >> 16))
+        assert ups == (bits & 0xFFFF)
+        # and the library agrees with that rule on a sample
+        for r in (0, 1, 0x7FFF, 0x8000, 0xFFFF, (0x10000 - (bits & 0xFFFF)) & 0xFFFF):
+            want = (bits + r) >> 16
+            assert O.lib.orc_f32_to_bf16_sr(float(v), r << 16) == want
+
+
+def test_f16_stochastic_rounding_is_exactly_unbiased():
+    rng = np.random.default_rng(6)
+    # normal FP16 range: the 13 dropped bits
+    for v in np.concatenate([rng.standard_normal(8), rng.uniform(1e-4, 1e-3, 4), [1.0, -0.3]]).astype(np.float32):
+        bits = int(np.array([v], np.float32).view(np.uint32)[0])
+        low = bits & 0x1FFF
+        res = [O.lib.orc_f32_to_f16_sr(float(v), k << 19) for k in range(8192)]
+        lo = O.lib.orc_f32_to_f16(_f32(bits & ~0x1FFF))
+        assert set(res) <= {lo, lo + 1}
+        assert res.count(lo + 1) == low
+    # subnormal FP16 range: |v| 2^24 = n + m / 1024; 1024 evenly spaced draws
+    for n, m in ((0, 1), (3, 512), (700, 1023), (1023, 7)):
+        v = np.float32((n + m / 1024.0) / 16777216.0)
+        res = [O.lib.orc_f32_to_f16_sr(float(v), (j << 13) << 9) for j in range(1024)]
+        assert set(res) <= {n, n + 1}
+        assert res.count(n + 1) == m
+        assert O.lib.orc_f32_to_f16_sr(float(-v), 0) == (0x8000 | n)
+
+
+def test_sr_bits_uniform_and_counter_based():
+    a = np.array([O.lib.orc_sr_bits(s, i, q) for s in range(4) for i in range(1024) for q in range(4)], np.uint64)
+    assert len(np.unique(a)) == a.size                      # no repeats in 16k draws of 32 bits
+    h = np.bincount((a >> np.uint64(24)).astype(np.int64), minlength=256)
+    chi2 = float(((h - a.size / 256) ** 2 / (a.size / 256)).sum())
+    assert chi2 < 340, chi2                                  # 255 dof, p ~ 0.0003 threshold
+    assert O.lib.orc_sr_bits(7, 11, 3) == O.lib.orc_sr_bits(7, 11, 3)

@@ -244,3 +244,28 @@ def test_deviation_form_lowers_fp32_residual_floor():
         floors[st] = np.linalg.norm(u1 - u0) / np.linalg.norm(u1)
     assert floors["f32s"] < 5e-7, floors
     assert floors["f32s"] < 0.5 * floors["f32"], floors
+
+
+def test_stochastic_rounding_stream_continues_across_runs_and_keeps_mass():
+    """round_mode 1: the step counter carries over run() calls (20 + 30 steps
+    equal 50), and FP16 storage with stochastic rounding keeps the spinodal
+    mass to 1e-4 where nearest-even drifts by more than 1e-3."""
+    from lattibol_amd import scenes
+    cfg, solid, init = scenes.spinodal(n=12, storage="f16", round_mode=1)
+    a = O.PullOracle(cfg, solid)
+    b = O.PullOracle(cfg, solid)
+    for o in (a, b):
+        o.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+    a.run(20)
+    a.run(30)
+    b.run(50)
+    np.testing.assert_array_equal(a.A, b.A)
+    drift = {}
+    for mode in (0, 1):
+        cfg, solid, init = scenes.spinodal(n=16, storage="f16", round_mode=mode)
+        o = O.PullOracle(cfg, solid)
+        o.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
+        m0 = o.reduce()[0]
+        o.run(1500)
+        drift[mode] = abs(o.reduce()[0] - m0) / m0
+    assert drift[1] < 1e-4 and drift[0] > 10 * drift[1], drift
