@@ -120,7 +120,8 @@ def bench_cavity(args):
                   roofline={"bound": "hbm", "achieved": round(72 * n / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
                             "unit": "GB/s", "frac": round(72 * n / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                             "traffic": None, "kernel_ms": round(ms, 5)},
-                  mass=float(res[0]))
+                  mass=float(res[0]),
+                  cpu_baseline=None if args.no_cpu_baseline else cpu_baseline(cfg, solid, init, args.cpu_steps))
 
 
 def bench_spinodal(args):
@@ -167,10 +168,24 @@ def bench_spinodal(args):
         out[key]["rho_err_max_vs_fp64"] = float(np.abs(d).max())
         out[key]["phase_density_err_vs_fp64"] = [out[key]["rho_p5"] - out["f64"]["rho_p5"],
                                                  out[key]["rho_p95"] - out["f64"]["rho_p95"]]
+    # one Shan-Chen step = density pass (19 loads, psi store) + sweep (19 loads,
+    # 19 stores, psi tile): 2*19*4 + 19*4 + 2*4 = 236 algorithmic bytes per node
+    n = 192 ** 3
+    ms = out["f32"]["seconds"] / steps * 1e3
+    achieved = 236.0 * n / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        cfg, solid, init = scenes.spinodal(storage="f32")
+        init = dict(init, ux=0.0, uy=0.0, uz=0.0)
+        cpu = cpu_baseline(cfg, solid, init, args.cpu_steps)
     return common("D3Q19 Shan-Chen MLUPS (FP32 storage) + error of FP32/FP16/BF16 vs FP64", out["f32"]["mlups"],
-                  "MLUPS", args, steps, round(out["f32"]["seconds"] / steps * 1e3, 5), "fp32",
+                  "MLUPS", args, steps, round(ms, 5), "fp32",
                   {"workload": "D3Q19 Shan-Chen spinodal decomposition 192^3 periodic, psi=1-exp(-rho), G=-5, "
-                               "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out)
+                               "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out,
+                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                            "kernel": "k_density<19,f32> + k_step<19,f32,SC> per step", "bytes_per_update": 236},
+                  cpu_baseline=cpu)
 
 
 def bench_ensemble(args):
@@ -191,10 +206,32 @@ def bench_ensemble(args):
     wall = time.perf_counter() - t0
     conv = res < 1e-6
     upd = float(sum(int(steps[i]) * specs[i]["nx"] * specs[i]["ny"] for i in range(len(specs))))
+    ms = dt / args.steps * 1e3
+    achieved = 72.0 * n / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        nodes = 0
+        t0 = time.perf_counter()
+        for sp in specs[:16]:
+            orc = O.PullOracle(scenes.channel_config(sp), sp["solid"])
+            orc.init_equilibrium(1.0, 0.0, 0.0)
+            orc.run(200)
+            nodes += sp["nx"] * sp["ny"] * 200
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(nodes / tc / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+               "sample": "the first 16 lattices of the ensemble, 200 steps each, FP32-deviation oracle, %.1f s" % tc}
     return common("D2Q9 batched-ensemble MLUPS", round(mlups, 1), "MLUPS", args, args.steps,
                   round(dt / args.steps * 1e3, 5), "fp32",
                   {"workload": "512 D2Q9 channels, sizes {64x64,128x64,128x128,256x128}, 1-4 circles, "
                                "u_in in [0.02,0.08], tau in [0.52,0.9], one launch per step", "nodes": n},
+                  roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                            "kernel": "k_batch_step (all lattices, one launch per step)", "bytes_per_update": 72},
+                  cpu_baseline=cpu,
                   steady={"converged": int(conv.sum()), "of": len(specs), "wall_s": round(wall, 2),
                           "max_steps": args.max_steady_steps, "updates": upd,
                           "effective_mlups": round(upd / wall / 1e6, 1),
