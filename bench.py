@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--workload", default="sphere",
-                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "unet", "copy"],
+                    choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "unet", "reference", "copy"],
                     help="sphere = D3Q19 channel past a sphere (the headline); the others print their own line")
     ap.add_argument("--train-samples", type=int, default=4096)
     ap.add_argument("--epochs", type=int, default=80)
@@ -320,6 +320,41 @@ def bench_unet(args):
                   mfma_peak_tflops=2500.0)
 
 
+def bench_reference(args):
+    """The reference's launchers (streaming_d3q19, get/copy/apply_border,
+    collision_d3q19, read_value_3d) driven by the reference step loop on the
+    C2 lattice: the drop-in path for callers that keep the reference's call
+    sites.  Reference physics: Zou-He inlet u = 0.05 / outlet rho = 1,
+    full-way bounce-back at the sphere, periodic y and z, tau = 0.56.
+    Traffic of the reference's two passes per update: streaming swaps
+    9 x (2 loads + 2 stores) x 4 B + 1 B mask = 145 B, collision 19 loads +
+    19 stores + 4 macro planes + the plot float4 + 1 B mask = 185 B."""
+    import lattibol_amd as lb
+    from lattibol_amd import scenes
+    out = {}
+    cfg, solid, init = scenes.sphere_channel()
+    nu = (1.0 / cfg["tau_inv"] - 0.5) / 3.0
+    for nd in (1, 2):
+        sim = lb.Simulation(solid, deviceCount=nd, ux=0.05, rho=1.0, viscosity=nu, cursor=(128, 64, 64))
+        sim.runLBM(max(1, args.warmup // 4))
+        sim.valueCur()
+        steps = max(10, args.steps // 5)
+        t0 = time.perf_counter()
+        sim.runLBM(steps)
+        sim.valueCur()                      # the probe read-back synchronises
+        dt = time.perf_counter() - t0
+        n = solid.size
+        out["subdomains_%d" % nd] = {"mlups": round(n * steps / dt / 1e6, 1), "ms_per_step": round(dt / steps * 1e3, 5),
+                                     "achieved_gbs_at_330B": round(330.0 * n * steps / dt / 1e9, 1)}
+        sim.close()
+    v = out["subdomains_1"]
+    return common("D3Q19 MLUPS through the reference launchers", v["mlups"], "MLUPS", args, steps, v["ms_per_step"],
+                  "fp32", {"workload": "D3Q19 256x128x128 sphere channel, reference physics and launch sequence"},
+                  roofline={"bound": "hbm", "achieved": v["achieved_gbs_at_330B"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": round(v["achieved_gbs_at_330B"] / PEAK_HBM_GBS, 4), "traffic": None,
+                            "bytes_per_update": 330}, runs=out)
+
+
 def bench_copy(args):
     """Plain device copy of 512 MiB: HBM bandwidth reference and the
     calibration run for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -346,7 +381,8 @@ def main():
         sys.exit(2)
     if args.workload != "sphere":
         fn = {"cavity": bench_cavity, "spinodal": bench_spinodal, "ensemble": bench_ensemble,
-              "warmstart": bench_warmstart, "unet": bench_unet, "copy": bench_copy}[args.workload]
+              "warmstart": bench_warmstart, "unet": bench_unet, "reference": bench_reference,
+              "copy": bench_copy}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     import numpy as np
