@@ -216,6 +216,8 @@ typedef struct {
     int round_mode;         /* FP16 / BF16 stores: 0 round to nearest even, 1 stochastic
                                rounding (unbiased; counter-based random bits per step,
                                node and population, reproducible) */
+    int node_words;         /* -1 autotune, 0 flag byte (+ link mask for nodes next to
+                               walls / solids), 1 one packed 32-bit word per node */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

@@ -217,6 +217,8 @@ struct lb_ctx {
     uint8_t *flags = nullptr;
     uint32_t *lmask = nullptr;
     uint32_t *ctr = nullptr;          // step counter of the stochastic-rounding stream (+1 spare word)
+    uint32_t *nodew = nullptr;        // packed node words (flags << 24 | link mask)
+    int packed = 0;                   // kernels read nodew instead of flags + lmask
     float *macros = nullptr;
     float *snap = nullptr;
     void *psi = nullptr;
@@ -238,8 +240,8 @@ struct lb_ctx {
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
-        void *ps[] = {buf[0], buf[1], flags, lmask, ctr, macros, snap, psi, halo[0], halo[1], partial,
-                      dout, tmp_fields, dargs};
+        void *ps[] = {buf[0], buf[1], flags, lmask, ctr, nodew, macros, snap, psi, halo[0], halo[1],
+                      partial, dout, tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -259,6 +261,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     a.B = (typename S::raw *)c->buf[1 - src];
     a.flags = c->flags;
     a.lmask = c->lmask;
+    a.nodew = c->nodew;
+    a.packed = c->packed;
     a.macros = c->macros;
     a.psi = (typename S::T *)c->psi;
     a.send = (typename S::raw *)c->halo[src];
@@ -428,8 +432,11 @@ int upload_flags(lb_ctx *c) {
     std::vector<uint8_t> fl;
     std::vector<uint32_t> lm;
     build_flags(c->cfg, c->solid, fl, lm);
+    std::vector<uint32_t> nw(c->N);
+    for (uint32_t k = 0; k < c->N; k++) nw[k] = ((uint32_t)fl[k] << 24) | lm[k];
     int rc = (int)hipMemcpyAsync(c->flags, fl.data(), c->N, hipMemcpyHostToDevice, c->stream);
     if (!rc) rc = (int)hipMemcpyAsync(c->lmask, lm.data(), (size_t)c->N * 4, hipMemcpyHostToDevice, c->stream);
+    if (!rc) rc = (int)hipMemcpyAsync(c->nodew, nw.data(), (size_t)c->N * 4, hipMemcpyHostToDevice, c->stream);
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     return rc;
 }
@@ -473,7 +480,7 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
 // lattice and keeps the fastest.  Runs at creation, before any populations
 // are set, so switching the layout moves no data.
 int autotune(lb_ctx *c) {
-    struct Cand { int bx, by, xcd, mem, layout; };
+    struct Cand { int bx, by, xcd, mem, layout, packed; };
     std::vector<Cand> cands;
     const int bxs[4] = {32, 64, 128, 256};
     for (int bx : bxs) {
@@ -485,12 +492,16 @@ int autotune(lb_ctx *c) {
                 if (c->cfg.mem_mode >= 0 && mem != c->cfg.mem_mode) continue;
                 for (int lay = 0; lay < 2; lay++) {
                     if (c->cfg.layout >= 0 && lay != c->cfg.layout) continue;
-                    cands.push_back({bx, kThreads / bx, xcd, mem, lay});
+                    for (int pk = 0; pk < 2; pk++) {
+                        // packed node words only pay on small, latency-bound lattices
+                        if (c->cfg.node_words >= 0 ? pk != c->cfg.node_words : (pk && c->N > (1u << 22))) continue;
+                        cands.push_back({bx, kThreads / bx, xcd, mem, lay, pk});
+                    }
                 }
             }
         }
     }
-    if (cands.empty()) cands.push_back({64, 4, 0, 0, 0});
+    if (cands.empty()) cands.push_back({64, 4, 0, 0, 0, 0});
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
     float best = 1e30f;
@@ -498,6 +509,7 @@ int autotune(lb_ctx *c) {
     const int reps = 10;
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
+        c->packed = k.packed;
         if (int rc = upload_args(c)) return rc;
         launch_step(c, 0, false);   // warm
         (void)hipEventRecord(e0, c->stream);
@@ -512,6 +524,7 @@ int autotune(lb_ctx *c) {
     (void)hipEventDestroy(e1);
     c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd; c->mem = bestc.mem;
     c->layout = bestc.layout;
+    c->packed = bestc.packed;
     int rc = (int)hipGetLastError();
     if (!rc) rc = upload_args(c);
     return rc;
@@ -533,7 +546,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
-    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1)
+    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1 ||
+        cfg->node_words > 1)
         return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
@@ -550,6 +564,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipMalloc(&c->buf[1], pop_bytes);
     if (!rc) rc = (int)hipMalloc((void **)&c->flags, c->N);
     if (!rc) rc = (int)hipMalloc((void **)&c->lmask, (size_t)c->N * 4);
+    if (!rc) rc = (int)hipMalloc((void **)&c->nodew, (size_t)c->N * 4);
     if (!rc) rc = (int)hipMalloc((void **)&c->ctr, 2 * sizeof(uint32_t));
     if (!rc) rc = (int)hipMemsetAsync(c->ctr, 0, 2 * sizeof(uint32_t), c->stream);
     if (!rc) rc = (int)hipMalloc((void **)&c->macros, (size_t)4 * c->N * sizeof(float));
@@ -571,12 +586,13 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc && c->psi) rc = (int)hipMemsetAsync(c->psi, 0, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4), c->stream);
     if (!rc) rc = upload_flags(c);
     if (!rc) {
-        if (cfg->block_x > 0 && cfg->xcd_remap >= 0 && cfg->mem_mode >= 0 && cfg->layout >= 0) {
+        if (cfg->block_x > 0 && cfg->xcd_remap >= 0 && cfg->mem_mode >= 0 && cfg->layout >= 0 && cfg->node_words >= 0) {
             c->bx_shift = ilog2(cfg->block_x);
             c->by = std::max(1, kThreads / cfg->block_x);
             c->xcd = cfg->xcd_remap;
             c->mem = cfg->mem_mode;
             c->layout = cfg->layout;
+            c->packed = cfg->node_words;
             rc = upload_args(c);
         } else {
             rc = autotune(c);
@@ -589,6 +605,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.xcd_remap = c->xcd;
     c->cfg.mem_mode = c->mem;
     c->cfg.layout = c->layout;
+    c->cfg.node_words = c->packed;
     c->cfg.graph_chunk = c->graph_steps;
     *out = c;
     return 0;

@@ -33,6 +33,8 @@ template <class S> struct StepArgs {
     raw *B;
     const uint8_t *flags;
     const uint32_t *lmask;  // link masks of F_SLOW nodes (see pull_node)
+    const uint32_t *nodew;  // packed node words: flags << 24 | link mask, every node
+    int packed;             // 1: read nodew (one load per node), 0: flags (+ lmask)
     float *macros;          // 4*N: rho, ux, uy, uz
     T *psi;                 // Shan-Chen pseudo-potential
     raw *send;              // [send_lo (5 planes, -z pops of z=0) | send_hi (+z pops of z=nz-1)]
@@ -156,8 +158,23 @@ template <int Q, class S> LB_DEV int wall_face(const StepArgs<S> &a, const NodeP
 // to walls or solids (F_SLOW) carry a link mask: bit i set = link i bounces,
 // bit 31 = one of those links bounces off a moving wall.  No per-link
 // branches on loaded data, so the loads of a node are all in flight at once.
+// Flags and link mask of a node: one packed word, or the flag byte followed by
+// the mask of F_SLOW nodes (a second, dependent load for those).  The packed
+// form costs 3 more bytes per node and saves a round trip to memory: the
+// autotuner picks it for small, latency-bound lattices.
+template <class S> LB_DEV void node_word(const StepArgs<S> &a, const NodePos &p, uint8_t &fl, uint32_t &bm) {
+    if (a.packed) {
+        const uint32_t w = ldm(&a.nodew[p.id]);
+        fl = (uint8_t)((w >> 24) & 0x7fu);
+        bm = w & 0x80ffffffu;
+    } else {
+        fl = ldm(&a.flags[p.id]);
+        bm = (fl & F_SLOW) ? ldm(&a.lmask[p.id]) : 0u;
+    }
+}
+
 template <int Q, class S, int MEM = 0>
-LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typename S::T *f) {
+LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint32_t bm, typename S::T *f) {
     typedef typename S::T T;
     typedef typename S::raw R;
     typedef Vel<Q> L;
@@ -165,7 +182,6 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint8_t fl, typena
     const int loy = p.y == 0 ? a.corr_lo[1] : 0, hiy = p.y == a.ny - 1 ? a.corr_hi[1] : 0;
     int loz = 0, hiz = 0;
     if (Q == 19) { loz = p.z == 0 ? a.corr_lo[2] : 0; hiz = p.z == a.nz - 1 ? a.corr_hi[2] : 0; }
-    const uint32_t bm = (fl & F_SLOW) ? ldm(&a.lmask[p.id]) : 0u;
     // z-slab halo: links crossing z = -1 / z = nz read the received planes
     // (a tile lies in one z-plane, so these tests are uniform per workgroup)
     const bool glo = Q == 19 && a.halo && p.z == 0;
@@ -396,11 +412,13 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
     typedef Vel<Q> L;
     const int bx = 1 << a.bx_shift;
     if (!p.valid) return;
-    const uint8_t fl = ldm(&a.flags[p.id]);
+    uint8_t fl;
+    uint32_t bm;
+    node_word(a, p, fl, bm);
     T f[Q];
     if (fl & F_SOLID) {
         if (a.bb_full) {
-            pull_node<Q, S, MEM>(a, p, fl, f);
+            pull_node<Q, S, MEM>(a, p, bm, f);
             store_node<Q, S, MEM, true>(a, p, f);
         }
         if (MACROS) {
@@ -417,7 +435,7 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
         if constexpr (S::DEV) feq_dev<Q, T>(f, rho - T(1), ux, uy, uz);
         else feq<Q, T>(f, rho, ux, uy, uz);
     } else {
-        pull_node<Q, S, MEM>(a, p, fl, f);
+        pull_node<Q, S, MEM>(a, p, bm, f);
         apply_zou_he<Q, S>(a, p, f);
         T drho = T(0);
         if constexpr (S::DEV) {
@@ -464,14 +482,16 @@ template <int Q, class S>
 LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
     typedef typename S::T T;
     if (!p.valid) return;
-    const uint8_t fl = ldm(&a.flags[p.id]);
+    uint8_t fl;
+    uint32_t bm;
+    node_word(a, p, fl, bm);
     if (fl & F_SOLID) { stm(&a.psi[p.id], (T)a.scpsi_solid); return; }
     T rho;
     if (fl & F_INLET) {
         rho = (T)a.bcrho[fl >> 4];
     } else {
         T f[Q];
-        pull_node<Q, S>(a, p, fl, f);
+        pull_node<Q, S>(a, p, bm, f);
         apply_zou_he<Q, S>(a, p, f);
         T ux, uy, uz;
         if constexpr (S::DEV) {

@@ -161,16 +161,19 @@ def _scene(kind, kw):
             "channel": _channel_scene}[kind](**kw)
 
 
+@pytest.mark.parametrize("words", [0, 1])
 @pytest.mark.parametrize("layout", ["soa", "rows"])
 @pytest.mark.parametrize("case", sorted(FUSED_CASES))
-def test_fused_solver_matches_oracle(gpu, case, layout):
-    """Both device population layouts; populations come back in [i][z][y][x]."""
+def test_fused_solver_matches_oracle(gpu, case, layout, words):
+    """Both device population layouts and both node-flag forms (byte + link
+    mask, packed word); populations come back in [i][z][y][x]."""
     kind, kw, steps = FUSED_CASES[case]
     cfg, solid, init = _scene(kind, kw)
     cfg["layout"] = {"soa": 0, "rows": 1}[layout]
+    cfg["node_words"] = words
     print("case", case, layout, cfg["nx"], cfg["ny"], cfg["nz"], "storage", cfg["storage"], flush=True)
     sol, orc = _run_pair(gpu, cfg, solid, init, steps)
-    assert sol.launch_config["layout"] == cfg["layout"]
+    assert sol.launch_config["layout"] == cfg["layout"] and sol.launch_config["node_words"] == words
     fluid = solid.reshape(orc.shape) == 0
     _assert_pop_parity(sol.populations(), orc.A, cfg["storage"], fluid)
     m, mo = sol.macros(), orc.macros()
