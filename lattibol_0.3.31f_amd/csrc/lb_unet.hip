@@ -53,6 +53,10 @@ __device__ __forceinline__ bf16x8 load8(const float *p) {
 }
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0}); }
 
+// BF16 bits of an activation (FP32 activations round to nearest even).
+__device__ __forceinline__ uint16_t bf16_bits(bf16 v) { return __builtin_bit_cast(uint16_t, v); }
+__device__ __forceinline__ uint16_t bf16_bits(float v) { return lb::f32_to_bf16(v); }
+
 template <class T> __device__ __forceinline__ float to_f(T v);
 template <> __device__ __forceinline__ float to_f<float>(float v) { return v; }
 template <> __device__ __forceinline__ float to_f<bf16>(bf16 v) { return from_bf16(v); }
@@ -132,21 +136,25 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const float *__restrict__
     const int pend = min(npix, pbeg + chunk / 4);
     f32x16 acc;
     for (int r = 0; r < 16; r++) acc[r] = 0.f;
+    // unrolled so the loads of several steps are in flight before their MFMAs
+#pragma unroll 8
     for (int p0 = pbeg; p0 < pend; p0 += 16) {
         u16x8 av, bv;
-        for (int j = 0; j < 8; j++) {
-            const int p = p0 + kh + j;
-            float g = 0.f, x = 0.f;
+        // pixel coordinates of this lane's first element, then stepped along w
+        int p = p0 + kh;
+        int n = p / HW, h = (p - n * HW) / Wd, w = p - n * HW - h * Wd;
+        for (int j = 0; j < 8; j++, p++) {
+            float g = 0.f;
+            uint16_t xb = 0;
             if (p < pend) {
                 g = dout[(size_t)p * Co + co0 + row];
-                if (kvalid) {
-                    const int n = p / HW, rem = p - n * HW;
-                    const int hh = rem / Wd + dy, ww = rem - (rem / Wd) * Wd + dx;
-                    if (hh >= 0 && hh < H && ww >= 0 && ww < Wd) x = to_f<TIn>(in[((size_t)(n * H + hh) * Wd + ww) * Ci + ci]);
-                }
+                const int hh = h + dy, ww = w + dx;
+                if (kvalid && hh >= 0 && hh < H && ww >= 0 && ww < Wd)
+                    xb = bf16_bits(in[((size_t)(n * H + hh) * Wd + ww) * Ci + ci]);
             }
             av[j] = lb::f32_to_bf16(g);
-            bv[j] = lb::f32_to_bf16(x);
+            bv[j] = xb;
+            if (++w == Wd) { w = 0; if (++h == H) { h = 0; n++; } }
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc, 0, 0, 0);
     }
