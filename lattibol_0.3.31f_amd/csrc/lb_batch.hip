@@ -288,14 +288,19 @@ int upload_batch_args(lb_batch *b) {
 // (at creation, before any populations are set: nothing to relayout).
 int autotune_batch(lb_batch *b) {
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
+    if (hipEventCreate(&e0)) return LB_ERR_INVALID;
+    if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
     float best = 1e30f;
     int best_layout = 0, best_mem = 0;
     for (int lay = 0; lay < 2; lay++)
         for (int mem = 0; mem < 3; mem++) {
             b->layout = lay;
             b->mem = mem;
-            if (int rc = upload_batch_args(b)) return rc;
+            if (int rc = upload_batch_args(b)) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                return rc;
+            }
             launch_batch_step(b, 0, false);
             (void)hipEventRecord(e0, b->stream);
             for (int r = 0; r < 6; r++) launch_batch_step(b, r & 1, false);

@@ -503,14 +503,19 @@ int autotune(lb_ctx *c) {
     }
     if (cands.empty()) cands.push_back({64, 4, 0, 0, 0, 0});
     hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) || hipEventCreate(&e1)) return LB_ERR_INVALID;
+    if (hipEventCreate(&e0)) return LB_ERR_INVALID;
+    if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
     float best = 1e30f;
     Cand bestc = cands[0];
     const int reps = 10;
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
         c->packed = k.packed;
-        if (int rc = upload_args(c)) return rc;
+        if (int rc = upload_args(c)) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
         launch_step(c, 0, false);   // warm
         (void)hipEventRecord(e0, c->stream);
         for (int r = 0; r < reps; r++) launch_step(c, r & 1, false);
