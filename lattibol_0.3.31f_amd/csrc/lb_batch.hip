@@ -77,6 +77,7 @@ StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, co
     a.bx_shift = 6; a.by = kBY;
     a.ntx = L.ntx; a.nty = L.nty;
     a.nblocks = L.ntx * L.nty;
+    fast_path_args<9>(a, 1 << a.bx_shift, a.by);
     return a;
 }
 

@@ -292,6 +292,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     a.nty = (c->cfg.ny + c->by - 1) / c->by;
     a.nblocks = a.ntx * a.nty * c->cfg.nz;
     a.xcd = c->xcd;
+    if (c->Q == 9) fast_path_args<9>(a, 1 << c->bx_shift, c->by);
+    else fast_path_args<19>(a, 1 << c->bx_shift, c->by);
     return a;
 }
 

@@ -47,6 +47,15 @@ template <class S> struct StepArgs {
     float bcu[6][3];
     float bcrho[6];
     int corr_lo[3], corr_hi[3];   // fast-path index corrections at faces
+    // Buffer-descriptor accesses of the 16-bit storages (pull_node,
+    // store_node): usable when every offset fits 32 bits.  Byte offset of
+    // population slot i from the tile's load descriptor A + (e0 - offmax -
+    // bias) * esz: lso[i] = (i*ps - off_i + offmax) * esz
+    int fast;
+    int bias;                     // -(corr_hi[0] + corr_hi[1]) >= 0
+    int offmax;                   // largest link offset off_i = cx + cy*sy + cz*sz
+    uint32_t lso[19];
+    uint32_t sso[19];             // stores: i*ps * esz from the descriptor base B + e0 * esz
     float force[3];
     float scG, scrho0, scpsi_solid;
     int bb_full, zou_he, halo;
@@ -66,8 +75,9 @@ LB_DEV int xcd_remap(int b, int nb) {
 }
 
 // id: node index x + nx (y + ny z) (flags, macros, psi); e: element offset of
-// the node's population 0 in the population buffers.
-struct NodePos { int x, y, z; uint32_t id, e; bool valid; uint32_t step; };
+// the node's population 0 in the population buffers; e0: that of the tile's
+// first node (the same for the whole workgroup).
+struct NodePos { int x, y, z; uint32_t id, e, e0; bool valid; uint32_t step; };
 
 // Population layouts.  LB_LAYOUT_SOA: [i][z][y][x], element i*N + id.
 // LB_LAYOUT_ROWS: [z][y][i][x], the Q populations of one x-row stored back to
@@ -107,7 +117,8 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     p.z = z;
     p.valid = p.x < a.nx && p.y < a.ny;
     p.id = (uint32_t)p.x + (uint32_t)a.nx * ((uint32_t)p.y + (uint32_t)a.ny * (uint32_t)z);
-    p.e = (uint32_t)p.x + a.sy * (uint32_t)p.y + a.sz * (uint32_t)z;
+    p.e0 = ((uint32_t)tx << a.bx_shift) + a.sy * (uint32_t)(ty * a.by) + a.sz * (uint32_t)z;
+    p.e = p.e0 + (uint32_t)lx + a.sy * (uint32_t)ly;
     p.step = 0;
     return p;
 }
@@ -130,6 +141,35 @@ template <int MEM = 0, class R> LB_DEV void stm(R *p, R v) {
     if constexpr (MEM >= 1) __builtin_nontemporal_store(v, g);
     else *g = v;
 }
+// Raw buffer accesses: a descriptor built once per tile (base, size in
+// bytes; 0x00020000 = 32-bit data format, the raw-buffer value for gfx9
+// parts), a scalar byte offset per population slot and a per-lane byte
+// offset, so a link costs no address arithmetic beyond its face correction,
+// and an access past the descriptor's size reads zero (a store is dropped)
+// instead of faulting.
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+LB_DEV rsrc_t make_rsrc(const void *base, uint64_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(uint32_t)(bytes < 0xffffffffull ? bytes : 0xffffffffull),
+                                             0x00020000);
+}
+// aux 2 = non-temporal
+template <int AUX, class R> LB_DEV R buf_ld(rsrc_t r, uint32_t vo, uint32_t so) {
+    static_assert(sizeof(R) == 2, "descriptor path: 16-bit storages");
+    return __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, AUX));
+}
+template <int AUX, class R> LB_DEV void buf_st(R v, rsrc_t r, uint32_t vo, uint32_t so) {
+    static_assert(sizeof(R) == 2, "descriptor path: 16-bit storages");
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, vo, so, AUX);
+}
+
+// A wave-uniform base plus a 32-bit per-lane byte offset: the compiler issues
+// the access with the base in scalar registers and the offset as its only
+// per-lane operand, so the address costs no per-lane 64-bit arithmetic.
+template <class R> LB_DEV const R *at_bytes(const R *base, uint32_t boff) {
+    return (const R *)((const char *)base + boff);
+}
+template <class R> LB_DEV R *at_bytes(R *base, uint32_t boff) { return (R *)((char *)base + boff); }
 
 // Source coordinate of a link along one axis: periodic wrap, clamp at
 // outflow / equilibrium-inlet faces; false where the link leaves through a
@@ -186,6 +226,33 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint32_t bm, typen
     // (a tile lies in one z-plane, so these tests are uniform per workgroup)
     const bool glo = Q == 19 && a.halo && p.z == 0;
     const bool ghi = Q == 19 && a.halo && p.z == a.nz - 1;
+    // Fast path of the 16-bit storages: no lane of the wave bounces a link,
+    // and the tile's links do not cross a halo plane or wrap in z (all
+    // uniform per wave).  Every link is then a neighbour load through the
+    // tile's descriptor: scalar offset lso[i], per-lane offset (e - e0 + bias)
+    // plus the lane's x / y face correction (one add for links with a
+    // correction term, none otherwise).  These sweeps are bound by
+    // instruction issue, and the per-link address arithmetic below costs
+    // them about 10 %.  The 4- and 8-byte sweeps are bound by HBM and run
+    // 3-5 % faster with the per-link form (measured), so they keep it.
+    if constexpr (sizeof(R) == 2) {
+        const bool zedge = Q == 19 && ((p.z == 0 && (a.halo || a.corr_lo[2])) ||
+                                       (p.z == a.nz - 1 && (a.halo || a.corr_hi[2])));
+        if (a.fast && !zedge && __builtin_amdgcn_ballot_w64(bm != 0u) == 0) {
+            constexpr int ES = (int)sizeof(R);
+            const int64_t b0 = (int64_t)p.e0 - a.offmax - a.bias;   // elements; may be negative
+            const rsrc_t rs = make_rsrc((const char *)a.A + b0 * ES, ((int64_t)Q * a.N - b0) * ES);
+            const uint32_t vb = (p.e - p.e0 + (uint32_t)a.bias) * (uint32_t)ES;
+            const int loxb = lox * ES, hixb = hix * ES, loyb = loy * ES, hiyb = hiy * ES;
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                constexpr int cx = L::cx(i), cy = L::cy(i);
+                const int cb = (cx > 0 ? loxb : (cx < 0 ? hixb : 0)) + (cy > 0 ? loyb : (cy < 0 ? hiyb : 0));
+                f[i] = S::ld(buf_ld<MEM >= 2 ? 2 : 0, R>(rs, vb + (uint32_t)cb, a.lso[i]), weight<Q, float>(L::wc(i)));
+            });
+            return;
+        }
+    }
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
@@ -210,8 +277,14 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint32_t bm, typen
                          (cz > 0 ? loz : (cz < 0 ? hiz : 0));
         const int off = cx + cy * (int)a.sy + cz * (int)a.sz;
         const uint32_t src = (uint32_t)((int)p.e - off + corr);
-        const R *ptr = bounce ? &a.A[(size_t)j * a.ps + p.e] : &a.A[(size_t)i * a.ps + src];
-        f[i] = S::ld(ldm<MEM>(ptr), weight<Q, float>(L::wc(i)));
+        if constexpr (sizeof(R) == 2) {
+            // element indices stay below 2^31 (checked at creation)
+            const uint32_t idx = bounce ? (uint32_t)j * a.ps + p.e : (uint32_t)i * a.ps + src;
+            f[i] = S::ld(ldm<MEM>(a.A + idx), weight<Q, float>(L::wc(i)));
+        } else {
+            const R *ptr = bounce ? &a.A[(size_t)j * a.ps + p.e] : &a.A[(size_t)i * a.ps + src];
+            f[i] = S::ld(ldm<MEM>(ptr), weight<Q, float>(L::wc(i)));
+        }
     });
     if (bm >> 31) {
         // bounced links that left through a moving wall gain 6 w_i (c_i . u_w)
@@ -236,28 +309,65 @@ template <int Q, class T> LB_DEV void moments(const T *f, T &rho, T &ux, T &uy, 
 
 // Stored value of population slot `slot` of node p: nearest-even, or
 // stochastic rounding for the 16-bit deviation storages.
+template <class S> constexpr bool kRoundable = std::is_same<S, StF16>::value || std::is_same<S, StBF16>::value;
+
+template <class S, bool SR>
+LB_DEV typename S::raw store_val_m(const NodePos &p, typename S::T v, float w, int slot) {
+    if constexpr (SR && std::is_same<S, StF16>::value) return f32_to_f16_sr(v, sr_bits(p.step, p.id, slot));
+    else if constexpr (SR && std::is_same<S, StBF16>::value) return f32_to_bf16_sr(v, sr_bits(p.step, p.id, slot));
+    else return S::st(v, w);
+}
 template <class S>
 LB_DEV typename S::raw store_val(const StepArgs<S> &a, const NodePos &p, typename S::T v, float w, int slot) {
-    if constexpr (std::is_same<S, StF16>::value) {
-        if (a.round_mode) return f32_to_f16_sr(v, sr_bits(p.step, p.id, slot));
-    } else if constexpr (std::is_same<S, StBF16>::value) {
-        if (a.round_mode) return f32_to_bf16_sr(v, sr_bits(p.step, p.id, slot));
+    if constexpr (kRoundable<S>) {
+        if (a.round_mode) return store_val_m<S, true>(p, v, w, slot);
     }
-    return S::st(v, w);
+    return store_val_m<S, false>(p, v, w, slot);
 }
 
 // REV (full-way bounce-back at a solid node): slot opp(i) <- f_i.  A
 // compile-time flag: a runtime choice between f[i] and f[opp(i)] becomes a
 // dynamically indexed private array, i.e. per-lane scratch memory.
+// Stores address slot s of the node as the uniform base B + s*ps + e0 plus
+// the lane's byte offset (e - e0)*sizeof, the same for every slot.
 template <int Q, class S, int MEM = 0, bool REV = false>
 LB_DEV void store_node(const StepArgs<S> &a, const NodePos &p, const typename S::T *f) {
     typedef Vel<Q> L;
-    static_for<0, Q>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        constexpr int slot = REV ? L::opp(i) : i;
-        auto v = store_val<S>(a, p, f[i], weight<Q, float>(L::wc(slot)), slot);
-        stm<MEM>(&a.B[(size_t)slot * a.ps + p.e], v);
-    });
+    typedef typename S::raw R;
+    if constexpr (sizeof(R) == 2) {
+        // 16-bit storages: the tile's store descriptor (B + e0*2), scalar
+        // offset sso[slot] and the per-lane byte offset (e - e0)*2, the same
+        // for every slot (or, past 32-bit offsets, a scalar pointer per slot
+        // plus that offset).  Uniform branches around all Q stores keep them
+        // in one basic block.
+        const uint32_t vs = (p.e - p.e0) * (uint32_t)sizeof(R);
+        auto put = [&](auto sr, auto buf) {
+            constexpr bool BUF = decltype(buf)::value;
+            rsrc_t ws;
+            if constexpr (BUF) ws = make_rsrc(a.B + p.e0, ((uint64_t)Q * a.N - p.e0) * sizeof(R));
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                constexpr int slot = REV ? L::opp(i) : i;
+                auto v = store_val_m<S, decltype(sr)::value>(p, f[i], weight<Q, float>(L::wc(slot)), slot);
+                if constexpr (BUF) buf_st<MEM >= 1 ? 2 : 0>(v, ws, vs, a.sso[slot]);
+                else stm<MEM>(at_bytes(a.B + ((size_t)slot * a.ps + p.e0), vs), v);
+            });
+        };
+        if (a.fast) {
+            if (a.round_mode) put(std::true_type{}, std::true_type{});
+            else put(std::false_type{}, std::true_type{});
+        } else {
+            if (a.round_mode) put(std::true_type{}, std::false_type{});
+            else put(std::false_type{}, std::false_type{});
+        }
+    } else {
+        static_for<0, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr int slot = REV ? L::opp(i) : i;
+            auto v = store_val<S>(a, p, f[i], weight<Q, float>(L::wc(slot)), slot);
+            stm<MEM>(&a.B[(size_t)slot * a.ps + p.e], v);
+        });
+    }
     if constexpr (Q == 19) {
         if (a.halo && (p.z == a.nz - 1 || p.z == 0)) {
             // pack (get_border): the +z populations of the top plane and the -z
@@ -571,6 +681,34 @@ inline void build_flags(const lb_config &g, const std::vector<uint8_t> &solid, s
 // Fast-path index corrections across each face (element units of the
 // population layout): periodic wrap or clamp; 0 where the face needs the
 // per-link path.
+// Scalar offsets and usability of the descriptor path for Q populations and
+// tiles of bx x by nodes: scalar + per-lane byte offset must stay below 2^32
+// for every load, lso + (e - e0 + bias + correction) * esz, and store,
+// sso + (e - e0) * esz.
+template <int Q, class S> void fast_path_args(StepArgs<S> &a, int bx, int by) {
+    typedef Vel<Q> L;
+    const uint64_t es = sizeof(typename S::raw);
+    a.bias = -(a.corr_hi[0] + a.corr_hi[1]);
+    int64_t off[19], offmax = 0;
+    for (int i = 0; i < Q; i++) {
+        off[i] = (int64_t)L::cx(i) + (int64_t)L::cy(i) * a.sy + (int64_t)L::cz(i) * a.sz;
+        offmax = std::max(offmax, off[i]);
+    }
+    a.offmax = (int)offmax;
+    uint64_t lmax = 0, smax = 0;
+    for (int i = 0; i < 19; i++) {
+        const uint64_t l = i < Q ? ((uint64_t)i * a.ps + (uint64_t)(offmax - off[i])) * es : 0;
+        const uint64_t s = i < Q ? (uint64_t)i * a.ps * es : 0;
+        a.lso[i] = (uint32_t)l;
+        a.sso[i] = (uint32_t)s;
+        lmax = std::max(lmax, l);
+        smax = std::max(smax, s);
+    }
+    const uint64_t lane = (uint64_t)bx + (uint64_t)by * a.sy;   // > e - e0
+    const uint64_t lv = (lane + (uint64_t)(a.corr_lo[0] + a.corr_lo[1]) + (uint64_t)a.bias) * es;
+    a.fast = lmax + lv < (1ull << 32) && smax + lane * es < (1ull << 32) && offmax < (1ll << 31);
+}
+
 inline void face_corrections(const lb_config &g, const PopLayout &L, int lo[3], int hi[3]) {
     const int n[3] = {g.nx, g.ny, g.nz};
     const int stride[3] = {1, (int)L.sy, (int)L.sz};
