@@ -382,51 +382,45 @@ template <class T> LB_DEV void zouhe19_out(T *f, float rho_out) {
 
 // ---------------------------------------------------------------------------
 // Deviation form (s_i = f_i - w_i).  With drho = rho - 1:
-//   drho = sum s_i,  rho u = sum s_i c_i  (the w_i c_i cancel pairwise),
-//   seq_i = w_i (drho + rho g_i),  g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2,
-//   s_i <- s_i - (1/tau) (s_i - seq_i),
-// i.e. the same BGK as the reference written for deviations.  The summation
-// orders mirror the population form.
+//   drho = sum s_i,  u = (sum s_i c_i) * (1 / rho)  (the w_i c_i cancel pairwise),
+//   seq_i = w_i fma(rho, g_i, drho),  g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2,
+//   s_i <- fma(1/tau, seq_i - s_i, s_i),
+// i.e. the BGK of the reference written for deviations, in an operation
+// sequence chosen for instruction count: one reciprocal for the three
+// velocity components, and per opposite pair (c, -c) one a = 4.5 (c.u)^2 -
+// 1.5 u^2 (a fused multiply-add) shared by g = a + 3 c.u and g = a - 3 c.u.
+// Explicit fmas are exact-rounded on both the GPU and the host, so the
+// oracle evaluates the same sequence bit for bit.
 // ---------------------------------------------------------------------------
+LB_DEV float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+LB_DEV double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class T> LB_DEV void geq_pair(T cu, T r15, T &gp, T &gm) {
+    const T t = T(3) * cu;
+    const T a = fma_t(T(4.5) * cu, cu, -r15);
+    gp = a + t;
+    gm = a - t;
+}
 template <class T> LB_DEV void geq9(T *g, T ux, T uy) {
-    T uxs = ux * ux, uys = uy * uy;
-    T r15 = T(1.5) * (uxs + uys);
-    T s5 = +ux + uy, s6 = -ux + uy;
-    const T three = T(3), c45 = T(4.5);
+    const T r15 = T(1.5) * fma_t(ux, ux, uy * uy);
     g[0] = -r15;
-    g[1] = three * ux + c45 * uxs - r15;
-    g[2] = three * uy + c45 * uys - r15;
-    g[3] = -three * ux + c45 * uxs - r15;
-    g[4] = -three * uy + c45 * uys - r15;
-    g[5] = three * s5 + c45 * s5 * s5 - r15;
-    g[6] = three * s6 + c45 * s6 * s6 - r15;
-    g[7] = -three * s5 + c45 * s5 * s5 - r15;
-    g[8] = -three * s6 + c45 * s6 * s6 - r15;
+    geq_pair<T>(ux, r15, g[1], g[3]);
+    geq_pair<T>(uy, r15, g[2], g[4]);
+    geq_pair<T>(ux + uy, r15, g[5], g[7]);
+    geq_pair<T>(uy - ux, r15, g[6], g[8]);
 }
 template <class T> LB_DEV void geq19(T *g, T ux, T uy, T uz) {
-    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
-    T r15 = T(1.5) * (uxsq + uysq + uzsq);
-    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
-    const T three = T(3), c45 = T(4.5);
+    const T r15 = T(1.5) * fma_t(ux, ux, fma_t(uy, uy, uz * uz));
     g[0] = -r15;
-    g[1] = three * ux + c45 * uxsq - r15;
-    g[2] = -three * ux + c45 * uxsq - r15;
-    g[3] = three * uy + c45 * uysq - r15;
-    g[4] = -three * uy + c45 * uysq - r15;
-    g[5] = three * uz + c45 * uzsq - r15;
-    g[6] = -three * uz + c45 * uzsq - r15;
-    g[7] = three * s7 + c45 * s7 * s7 - r15;
-    g[8] = three * s8 + c45 * s8 * s8 - r15;
-    g[9] = three * s9 + c45 * s9 * s9 - r15;
-    g[10] = three * s10 + c45 * s10 * s10 - r15;
-    g[11] = -three * s8 + c45 * s8 * s8 - r15;
-    g[12] = -three * s7 + c45 * s7 * s7 - r15;
-    g[13] = -three * s10 + c45 * s10 * s10 - r15;
-    g[14] = -three * s9 + c45 * s9 * s9 - r15;
-    g[15] = three * s15 + c45 * s15 * s15 - r15;
-    g[16] = three * s16 + c45 * s16 * s16 - r15;
-    g[17] = -three * s16 + c45 * s16 * s16 - r15;
-    g[18] = -three * s15 + c45 * s15 * s15 - r15;
+    geq_pair<T>(ux, r15, g[1], g[2]);
+    geq_pair<T>(uy, r15, g[3], g[4]);
+    geq_pair<T>(uz, r15, g[5], g[6]);
+    geq_pair<T>(ux + uy, r15, g[7], g[12]);
+    geq_pair<T>(ux - uy, r15, g[8], g[11]);
+    geq_pair<T>(ux + uz, r15, g[9], g[14]);
+    geq_pair<T>(ux - uz, r15, g[10], g[13]);
+    geq_pair<T>(uy + uz, r15, g[15], g[18]);
+    geq_pair<T>(uy - uz, r15, g[16], g[17]);
 }
 
 // Zou-He in deviation form (the known-population weights sum to one, the

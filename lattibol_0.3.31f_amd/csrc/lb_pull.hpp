@@ -414,20 +414,20 @@ LB_DEV void collide(T *f, T rho, T ux, T uy, T uz, T ti, bool rest_closure) {
 
 // Deviation form: moments, equilibrium deviations and relaxation.
 template <int Q, class T> LB_DEV void moments_dev(const T *s, T &drho, T &ux, T &uy, T &uz) {
-    T rho;
+    T inv;
     if constexpr (Q == 9) {
         drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8];
-        rho = T(1) + drho;
-        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) / rho;
-        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) / rho;
+        inv = T(1) / (T(1) + drho);
+        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) * inv;
+        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) * inv;
         uz = T(0);
     } else {
         drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8] + s[9] + s[10] + s[11] +
                s[12] + s[13] + s[14] + s[15] + s[16] + s[17] + s[18];
-        rho = T(1) + drho;
-        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) / rho;
-        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) / rho;
-        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) / rho;
+        inv = T(1) / (T(1) + drho);
+        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) * inv;
+        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) * inv;
+        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) * inv;
     }
 }
 
@@ -438,7 +438,7 @@ template <int Q, class T> LB_DEV void feq_dev(T *seq, T drho, T ux, T uy, T uz) 
     const T rho = T(1) + drho;
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        seq[i] = weight<Q, T>(Vel<Q>::wc(i)) * (drho + rho * g[i]);
+        seq[i] = weight<Q, T>(Vel<Q>::wc(i)) * fma_t(rho, g[i], drho);
     });
 }
 
@@ -453,7 +453,7 @@ LB_DEV void collide_dev(T *s, T drho, T ux, T uy, T uz, T ti, bool rest_closure)
     }
     static_for<0, Q>([&](auto I) {
         constexpr int i = decltype(I)::value;
-        s[i] = s[i] - ti * (s[i] - eq[i]);
+        s[i] = fma_t(ti, eq[i] - s[i], s[i]);
     });
 }
 

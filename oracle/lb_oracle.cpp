@@ -414,47 +414,39 @@ inline void zouhe19_out(T *f, float rho_out) {
 }
 
 /* ---- deviation form: s_i = f_i - w_i, drho = rho - 1 ---------------------
- * drho = sum s_i, rho u = sum s_i c_i, seq_i = w_i (drho + rho g_i) with
- * g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2, s_i <- s_i - (1/tau)(s_i - seq_i).
+ * drho = sum s_i, u = (sum s_i c_i) * (1 / rho), seq_i = w_i fma(rho, g_i,
+ * drho) with g_i = 3 c.u + 4.5 (c.u)^2 - 1.5 u^2 evaluated per opposite pair
+ * (a = fma(4.5 c.u, c.u, -1.5 u^2), g = a +- 3 c.u), s_i <- fma(1/tau,
+ * seq_i - s_i, s_i): the kernels' operation sequence (lb_common.hpp,
+ * lb_pull.hpp), with std::fma exact-rounded like the GPU's fma.
  * Zou-He closures follow because the known weights sum to one and paired
  * weights cancel; the D3Q19 z correction uses f17. */
+template <class T> void geq_pair(T cu, T r15, T &gp, T &gm) {
+    const T t = T(3) * cu;
+    const T a = std::fma(T(4.5) * cu, cu, -r15);
+    gp = a + t;
+    gm = a - t;
+}
 template <class T> void geq9(T *g, T ux, T uy) {
-    T uxs = ux * ux, uys = uy * uy;
-    T r15 = T(1.5) * (uxs + uys);
-    T s5 = +ux + uy, s6 = -ux + uy;
+    const T r15 = T(1.5) * std::fma(ux, ux, uy * uy);
     g[0] = -r15;
-    g[1] = T(3) * ux + T(4.5) * uxs - r15;
-    g[2] = T(3) * uy + T(4.5) * uys - r15;
-    g[3] = -T(3) * ux + T(4.5) * uxs - r15;
-    g[4] = -T(3) * uy + T(4.5) * uys - r15;
-    g[5] = T(3) * s5 + T(4.5) * s5 * s5 - r15;
-    g[6] = T(3) * s6 + T(4.5) * s6 * s6 - r15;
-    g[7] = -T(3) * s5 + T(4.5) * s5 * s5 - r15;
-    g[8] = -T(3) * s6 + T(4.5) * s6 * s6 - r15;
+    geq_pair<T>(ux, r15, g[1], g[3]);
+    geq_pair<T>(uy, r15, g[2], g[4]);
+    geq_pair<T>(ux + uy, r15, g[5], g[7]);
+    geq_pair<T>(uy - ux, r15, g[6], g[8]);
 }
 template <class T> void geq19(T *g, T ux, T uy, T uz) {
-    T uxsq = ux * ux, uysq = uy * uy, uzsq = uz * uz;
-    T r15 = T(1.5) * (uxsq + uysq + uzsq);
-    T s7 = +ux + uy, s8 = +ux - uy, s9 = +ux + uz, s10 = +ux - uz, s15 = +uy + uz, s16 = +uy - uz;
+    const T r15 = T(1.5) * std::fma(ux, ux, std::fma(uy, uy, uz * uz));
     g[0] = -r15;
-    g[1] = T(3) * ux + T(4.5) * uxsq - r15;
-    g[2] = -T(3) * ux + T(4.5) * uxsq - r15;
-    g[3] = T(3) * uy + T(4.5) * uysq - r15;
-    g[4] = -T(3) * uy + T(4.5) * uysq - r15;
-    g[5] = T(3) * uz + T(4.5) * uzsq - r15;
-    g[6] = -T(3) * uz + T(4.5) * uzsq - r15;
-    g[7] = T(3) * s7 + T(4.5) * s7 * s7 - r15;
-    g[8] = T(3) * s8 + T(4.5) * s8 * s8 - r15;
-    g[9] = T(3) * s9 + T(4.5) * s9 * s9 - r15;
-    g[10] = T(3) * s10 + T(4.5) * s10 * s10 - r15;
-    g[11] = -T(3) * s8 + T(4.5) * s8 * s8 - r15;
-    g[12] = -T(3) * s7 + T(4.5) * s7 * s7 - r15;
-    g[13] = -T(3) * s10 + T(4.5) * s10 * s10 - r15;
-    g[14] = -T(3) * s9 + T(4.5) * s9 * s9 - r15;
-    g[15] = T(3) * s15 + T(4.5) * s15 * s15 - r15;
-    g[16] = T(3) * s16 + T(4.5) * s16 * s16 - r15;
-    g[17] = -T(3) * s16 + T(4.5) * s16 * s16 - r15;
-    g[18] = -T(3) * s15 + T(4.5) * s15 * s15 - r15;
+    geq_pair<T>(ux, r15, g[1], g[2]);
+    geq_pair<T>(uy, r15, g[3], g[4]);
+    geq_pair<T>(uz, r15, g[5], g[6]);
+    geq_pair<T>(ux + uy, r15, g[7], g[12]);
+    geq_pair<T>(ux - uy, r15, g[8], g[11]);
+    geq_pair<T>(ux + uz, r15, g[9], g[14]);
+    geq_pair<T>(ux - uz, r15, g[10], g[13]);
+    geq_pair<T>(uy + uz, r15, g[15], g[18]);
+    geq_pair<T>(uy - uz, r15, g[16], g[17]);
 }
 template <class T> void zouhe9_in_dev(T *s, T u) {
     T S = (s[0] + s[2] + s[4]) + T(2) * (s[3] + s[7] + s[6]);
@@ -593,17 +585,17 @@ template <int Q, class T>
 inline void moments_dev(const T *s, T &drho, T &ux, T &uy, T &uz) {
     if (Q == 9) {
         drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8];
-        T rho = T(1) + drho;
-        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) / rho;
-        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) / rho;
+        T inv = T(1) / (T(1) + drho);
+        ux = (s[1] - s[3] + s[5] - s[6] - s[7] + s[8]) * inv;
+        uy = (s[2] - s[4] + s[5] + s[6] - s[7] - s[8]) * inv;
         uz = T(0);
     } else {
         drho = s[0] + s[1] + s[2] + s[3] + s[4] + s[5] + s[6] + s[7] + s[8] + s[9] + s[10] + s[11] + s[12] +
                s[13] + s[14] + s[15] + s[16] + s[17] + s[18];
-        T rho = T(1) + drho;
-        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) / rho;
-        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) / rho;
-        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) / rho;
+        T inv = T(1) / (T(1) + drho);
+        ux = (s[1] + s[7] + s[8] + s[9] + s[10] - s[2] - s[11] - s[12] - s[13] - s[14]) * inv;
+        uy = (s[3] + s[7] + s[11] + s[15] + s[16] - s[4] - s[8] - s[12] - s[17] - s[18]) * inv;
+        uz = (s[5] + s[9] + s[13] + s[15] + s[17] - s[6] - s[10] - s[14] - s[16] - s[18]) * inv;
     }
 }
 template <int Q, class T>
@@ -612,7 +604,7 @@ inline void feq_dev(T *seq, T drho, T ux, T uy, T uz) {
     if (Q == 9) geq9<T>(g, ux, uy);
     else geq19<T>(g, ux, uy, uz);
     T rho = T(1) + drho;
-    for (int i = 0; i < Q; i++) seq[i] = wq<Q, T>(i) * (drho + rho * g[i]);
+    for (int i = 0; i < Q; i++) seq[i] = wq<Q, T>(i) * std::fma(rho, g[i], drho);
 }
 template <int Q, class S>
 inline void zou_he_any(const orc_cfg *cfg, int x, int nx, typename S::T *f) {
@@ -808,7 +800,7 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                         for (int i = 2; i < Q; i++) sum = sum + eq[i];
                         eq[0] = drho - sum;
                     }
-                    for (int i = 0; i < Q; i++) f[i] = f[i] - ti * (f[i] - eq[i]);
+                    for (int i = 0; i < Q; i++) f[i] = std::fma(ti, eq[i] - f[i], f[i]);
                 } else if (cfg->rest_closure) {
                     /* equilibria first (same roundings as the BGK expressions),
                        rest population closes the mass balance */
