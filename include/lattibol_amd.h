@@ -346,6 +346,15 @@ int lb_unet_to_fields(lb_unet *u, const float *d_y, int B, const float *d_uin, c
  * W is BF16 [Co][9][Ci]; Ci % 16 == 0, Co % 32 == 0. */
 int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B, int H,
                     int W, int Ci, int Co, lb_stream_t stream);
+/* Weight gradient of one 3x3 convolution (pad 1): d_dw[Co][9][Ci] (FP32) =
+ * sum over pixels of d_dout (FP32 NHWC, rounded to BF16 as an MFMA operand)
+ * times the shifted input d_in (BF16 NHWC).  method 0: per-element gathers;
+ * 1: channel-major transposed operands, 16-byte loads (W % 16 == 0, else
+ * LB_ERR_UNSUPPORTED).  Both sum in the same order (identical results).
+ * Allocates its scratch and synchronises: a test and diagnostic entry point;
+ * lb_unet_backward uses the same kernels on the handle's buffers. */
+int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,
+                          int Ci, int Co, lb_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* 5. Input decks and VTK output (host only)                                */

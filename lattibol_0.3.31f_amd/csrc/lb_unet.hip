@@ -72,12 +72,15 @@ template <> __device__ __forceinline__ bf16 from_f<bf16>(float v) { return to_bf
 // .. +7; B column j = lane % 32 holds the same k; accumulator r of lane l is
 // element (8 (r/4) + 4 (l/32) + r%4, l % 32).
 // ---------------------------------------------------------------------------
-template <class TIn, class TOut, int NB, bool BIAS_RELU>
+// KS = 4: the 4 wavefronts share one 32-pixel tile and each sums a quarter
+// of K (partial tiles added in LDS): 4x the workgroups for the small pixel
+// counts of the deep levels.
+template <class TIn, class TOut, int NB, bool BIAS_RELU, int KS = 1>
 __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, const bf16 *__restrict__ W,
                                                  const float *__restrict__ bias, TOut *__restrict__ out,
                                                  int npix, int H, int Wd, int Ci, int Co) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int m0 = (blockIdx.x * 4 + wave) * 32;
+    const int m0 = KS == 1 ? (blockIdx.x * 4 + wave) * 32 : blockIdx.x * 32;
     const int n0 = blockIdx.y * 32 * NB;
     if (m0 >= npix) return;
     const int row = lane & 31, kh = (lane >> 5) * 8;
@@ -91,7 +94,13 @@ __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, con
     for (int b = 0; b < NB; b++)
         for (int r = 0; r < 16; r++) acc[b][r] = 0.f;
     const int K = 9 * Ci;
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    int kbeg = 0, kend = K;
+    if (KS > 1) {
+        const int steps = K / 16;
+        kbeg = (wave * steps / KS) * 16;
+        kend = ((wave + 1) * steps / KS) * 16;
+    }
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
         const int tap = k0 / Ci, ci = k0 - tap * Ci + kh;
         const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
         bf16x8 a = zero8();
@@ -102,15 +111,32 @@ __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, con
             acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw, acc[b], 0, 0, 0);
         }
     }
-    for (int b = 0; b < NB; b++) {
-        const int co = n0 + 32 * b + row;
-        const float bv = BIAS_RELU ? bias[co] : 0.f;
-        for (int r = 0; r < 16; r++) {
-            const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const int mm = m0 + i;
+    if constexpr (KS == 1) {
+        for (int b = 0; b < NB; b++) {
+            const int co = n0 + 32 * b + row;
+            const float bv = BIAS_RELU ? bias[co] : 0.f;
+            for (int r = 0; r < 16; r++) {
+                const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int mm = m0 + i;
+                if (mm >= npix) continue;
+                float v = acc[b][r];
+                if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
+                out[(size_t)mm * Co + co] = from_f<TOut>(v);
+            }
+        }
+    } else {
+        __shared__ float sh[KS][NB * 16][64];
+        for (int b = 0; b < NB; b++)
+            for (int r = 0; r < 16; r++) sh[wave][b * 16 + r][lane] = acc[b][r];
+        __syncthreads();
+        for (int e = threadIdx.x; e < NB * 16 * 64; e += 256) {
+            const int l = e & 63, br = e >> 6, b = br >> 4, r = br & 15;
+            float v = 0.f;
+            for (int w = 0; w < KS; w++) v += sh[w][br][l];   // fixed order
+            const int i = 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+            const int mm = m0 + i, co = n0 + 32 * b + (l & 31);
             if (mm >= npix) continue;
-            float v = acc[b][r];
-            if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
+            if (BIAS_RELU) { v += bias[co]; v = v > 0.f ? v : 0.f; }
             out[(size_t)mm * Co + co] = from_f<TOut>(v);
         }
     }
@@ -171,6 +197,97 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const float *__restrict__
     }
 }
 
+// NHWC -> channel-major [C][npix] BF16 (64-pixel x 64-channel tiles through
+// LDS: coalesced reads along channels, coalesced writes along pixels).
+template <class T>
+__global__ __launch_bounds__(256) void k_to_cmajor(const T *__restrict__ src, bf16 *__restrict__ dst, int npix, int C) {
+    __shared__ uint16_t tile[64][65];
+    const int m0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int t = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+    for (int r = r0; r < 64; r += 4) {
+        const int m = m0 + r, c = c0 + t;
+        tile[t][r] = (m < npix && c < C) ? bf16_bits(src[(size_t)m * C + c]) : (uint16_t)0;
+    }
+    __syncthreads();
+    for (int r = r0; r < 64; r += 4) {
+        const int c = c0 + r, m = m0 + t;
+        if (c < C && m < npix) dst[(size_t)c * npix + m] = __builtin_bit_cast(bf16, tile[r][t]);
+    }
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u32x4 ld8w(const bf16 *p) { return *(const u32x4 *)p; }
+
+// Weight gradient from channel-major BF16 operands (dout_t [Co][npix],
+// in_t [Ci][npix]): a lane's 8 consecutive pixels of one channel are one
+// 16-byte load (A: output channel co, B: input channel ci of tap (dy, dx),
+// pixels shifted by dy rows and dx columns).  A 16-pixel step stays in one
+// image row (W % 16 == 0), so the row shift dy only masks whole groups and
+// the column shift dx = +-1 is a one-element funnel shift across two aligned
+// loads, with the zero padding at the row ends.  Same blocks, pixel split
+// and partial layout as k_conv3x3_wgrad.
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad_cm(const bf16 *__restrict__ dout_t, const bf16 *__restrict__ in_t,
+                                                          float *__restrict__ partial, int npix, int H, int Wd, int Ci,
+                                                          int Co, int chunk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int co0 = blockIdx.x * 32, kt0 = blockIdx.y * 32;
+    const int K = 9 * Ci;
+    const int row = lane & 31, kh = (lane >> 5) * 8;
+    const int k = kt0 + row;
+    const bool kvalid = k < K;
+    const int tap = kvalid ? k / Ci : 4, ci = kvalid ? k - (k / Ci) * Ci : 0;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int HW = H * Wd;
+    const int pbeg = blockIdx.z * chunk + wave * (chunk / 4);
+    const int pend = min(npix, pbeg + chunk / 4);
+    const bf16 *arow = dout_t + (size_t)(co0 + row) * npix;
+    const bf16 *brow = in_t + (size_t)ci * npix;
+    f32x16 acc;
+    for (int r = 0; r < 16; r++) acc[r] = 0.f;
+    int p = pbeg + kh;
+    int n = p / HW, h = (p - n * HW) / Wd, w = p - n * HW - h * Wd;
+#pragma unroll 4
+    for (int p0 = pbeg; p0 < pend; p0 += 16) {
+        const u32x4 av = ld8w(arow + p0 + kh);
+        u32x4 bv = {0u, 0u, 0u, 0u};
+        const int hh = h + dy;
+        if (kvalid && hh >= 0 && hh < H) {
+            const bf16 *q = brow + (size_t)(n * H + hh) * Wd + w;
+            const u32x4 cur = ld8w(q);
+            if (dx == 0) {
+                bv = cur;
+            } else if (dx < 0) {   // element j <- x[w + j - 1]
+                const unsigned prev = w > 0 ? ld8w(q - 8)[3] : 0u;
+                bv[0] = __builtin_amdgcn_alignbit(cur[0], prev, 16);
+                bv[1] = __builtin_amdgcn_alignbit(cur[1], cur[0], 16);
+                bv[2] = __builtin_amdgcn_alignbit(cur[2], cur[1], 16);
+                bv[3] = __builtin_amdgcn_alignbit(cur[3], cur[2], 16);
+            } else {               // element j <- x[w + j + 1]
+                const unsigned next = w + 8 < Wd ? ld8w(q + 8)[0] : 0u;
+                bv[0] = __builtin_amdgcn_alignbit(cur[1], cur[0], 16);
+                bv[1] = __builtin_amdgcn_alignbit(cur[2], cur[1], 16);
+                bv[2] = __builtin_amdgcn_alignbit(cur[3], cur[2], 16);
+                bv[3] = __builtin_amdgcn_alignbit(next, cur[3], 16);
+            }
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc,
+                                                      0, 0, 0);
+        w += 16;
+        if (w >= Wd) { w -= Wd; if (++h == H) { h = 0; n++; } }
+    }
+    __shared__ float sh[4][32][33];
+    for (int r = 0; r < 16; r++) {
+        const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        sh[wave][i][row] = acc[r];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+        const int i = e >> 5, j = e & 31;
+        const float s = sh[0][i][j] + sh[1][i][j] + sh[2][i][j] + sh[3][i][j];
+        if (kt0 + j < K) partial[((size_t)blockIdx.z * Co + co0 + i) * K + kt0 + j] = s;
+    }
+}
+
 // Sum of partials over chunks (fixed order) -> gradient.
 __global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *out) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,6 +295,17 @@ __global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *
     float s = 0.f;
     for (int c = 0; c < nchunk; c++) s += partial[(size_t)c * n + e];
     out[e] = s;
+}
+
+// First pass of a two-level chunk sum: group g (blockIdx.y) adds chunks
+// [g S, min((g+1) S, nchunk)) in order into out[g][e].
+__global__ void k_sum_chunk_groups(const float *partial, int nchunk, int S, size_t n, float *out) {
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int c0 = blockIdx.y * S, c1 = min(nchunk, c0 + S);
+    float s = 0.f;
+    for (int c = c0; c < c1; c++) s += partial[(size_t)c * n + e];
+    out[(size_t)blockIdx.y * n + e] = s;
 }
 
 // Bias-gradient partials: db[co] = sum_m dout[m, co], block (32-channel
@@ -431,8 +559,9 @@ struct lb_unet {
     bf16 *xin = nullptr;
     float *y = nullptr, *dy = nullptr;
     float *gA = nullptr, *gB = nullptr, *gC = nullptr;   // gradient scratch (largest tensor)
+    bf16 *tA = nullptr, *tB = nullptr;                   // channel-major copies for the weight gradient
     std::vector<float *> gskip;    // gradient of each encoder output (skip)
-    float *partial = nullptr;
+    float *partial = nullptr, *partial2 = nullptr;   // chunk partials, group sums
     size_t partial_n = 0;
     double *lpart = nullptr, *lsum = nullptr;
     std::vector<void *> allocs;
@@ -458,28 +587,76 @@ struct Level { int H, W, C; };
 // Conv index layout: enc l (2 convs), bottom (2), dec l (2), l descending.
 inline int enc_conv(int l) { return 2 * l; }
 
+// Chunk count of the groups of the two-level sum over nchunk chunks.
+inline int chunk_groups(int nchunk) {
+    int g = 1;
+    while (g * g < nchunk) g++;
+    return g;
+}
+constexpr int kBiasChunk = 256;
+
+// Fixed-order sum of nchunk partial arrays of n floats: one pass for a few
+// chunks, else about sqrt(nchunk) groups in parallel, then the groups (the
+// long serial loops of a single pass leave most of the GPU idle when n is
+// small).
+void sum_chunks(lb_unet *u, const float *partial, int nchunk, size_t n, float *out) {
+    if (nchunk <= 8 || !u->partial2) {
+        hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, partial, nchunk, n, out);
+        return;
+    }
+    const int G = chunk_groups(nchunk), S = (nchunk + G - 1) / G;
+    hipLaunchKernelGGL(k_sum_chunk_groups, dim3(nb256(n), G), dim3(256), 0, u->stream, partial, nchunk, S, n, u->partial2);
+    hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, u->partial2, G, n, out);
+}
+
 template <class TIn, class TOut, bool BR>
 void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
                  int Ci, int Co) {
     dim3 blk(256);
-    if (Co % 64 == 0) {
-        dim3 grd((npix + 127) / 128, Co / 64);
-        hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, npix, H, Wd, Ci, Co);
+    const int nb = Co % 64 == 0 ? 2 : 1;
+    // fewer than ~4 workgroups per CU with one 32-pixel tile per wavefront:
+    // split K over the wavefronts instead
+    const bool ksplit = (size_t)((npix + 127) / 128) * (Co / (32 * nb)) < 1024 && (9 * Ci) / 16 >= 8;
+    if (nb == 2) {
+        if (ksplit)
+            hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 2, BR, 4>), dim3((npix + 31) / 32, Co / 64), blk, 0, u->stream, in, W,
+                               bias, out, npix, H, Wd, Ci, Co);
+        else
+            hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 2, BR>), dim3((npix + 127) / 128, Co / 64), blk, 0, u->stream, in, W,
+                               bias, out, npix, H, Wd, Ci, Co);
     } else {
-        dim3 grd((npix + 127) / 128, Co / 32);
-        hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, npix, H, Wd, Ci, Co);
+        if (ksplit)
+            hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 1, BR, 4>), dim3((npix + 31) / 32, Co / 32), blk, 0, u->stream, in, W,
+                               bias, out, npix, H, Wd, Ci, Co);
+        else
+            hipLaunchKernelGGL((k_conv3x3<TIn, TOut, 1, BR>), dim3((npix + 127) / 128, Co / 32), blk, 0, u->stream, in, W,
+                               bias, out, npix, H, Wd, Ci, Co);
     }
 }
 
+// Weight gradient dW[co][tap][ci] from dout (FP32 NHWC) and the conv input
+// (BF16 NHWC).  With W % 16 == 0 and channel-major scratch (tA >= npix Co,
+// tB >= npix Ci elements) both operands are transposed first and the
+// vector-load kernel runs; otherwise the gathering kernel.
 template <class TIn>
-void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co) {
+void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
+                  bf16 *tA, bf16 *tB) {
     const int K = 9 * Ci;
     const int chunk = 1024;
     const int nchunk = (npix + chunk - 1) / chunk;
     dim3 grd(Co / 32, (K + 31) / 32, nchunk);
-    hipLaunchKernelGGL((k_conv3x3_wgrad<TIn>), grd, dim3(256), 0, u->stream, dout, in, u->partial, npix, H, Wd, Ci, Co, chunk);
-    size_t n = (size_t)Co * K;
-    hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, u->partial, nchunk, n, dW);
+    if (tA && tB && Wd % 16 == 0) {
+        hipLaunchKernelGGL(k_to_cmajor<float>, dim3((npix + 63) / 64, (Co + 63) / 64), dim3(256), 0, u->stream, dout, tA,
+                           npix, Co);
+        hipLaunchKernelGGL(k_to_cmajor<TIn>, dim3((npix + 63) / 64, (Ci + 63) / 64), dim3(256), 0, u->stream, in, tB,
+                           npix, Ci);
+        hipLaunchKernelGGL(k_conv3x3_wgrad_cm, grd, dim3(256), 0, u->stream, tA, tB, u->partial, npix, H, Wd, Ci, Co,
+                           chunk);
+    } else {
+        hipLaunchKernelGGL((k_conv3x3_wgrad<TIn>), grd, dim3(256), 0, u->stream, dout, in, u->partial, npix, H, Wd, Ci, Co,
+                           chunk);
+    }
+    sum_chunks(u, u->partial, nchunk, (size_t)Co * K, dW);
 }
 
 std::vector<Level> levels(const lb_unet *u) {
@@ -577,10 +754,12 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         big = std::max(big, (size_t)B * L[l].H * L[l].W * std::max(cmax, (size_t)u->cin_pad));
     }
     u->gA = u->alloc<float>(big); u->gB = u->alloc<float>(big); u->gC = u->alloc<float>(big);
+    u->tA = u->alloc<bf16>(big); u->tB = u->alloc<bf16>(big);
     u->xin = u->alloc<bf16>((size_t)B * L[0].H * L[0].W * u->cin_pad);
     u->y = u->alloc<float>((size_t)B * L[0].H * L[0].W * desc->out_ch);
     u->dy = u->alloc<float>((size_t)B * L[0].H * L[0].W * desc->out_ch);
     // weight-gradient partials: max over convs of nchunk * Co * K
+    size_t p2 = 0;
     for (size_t ci = 0; ci < u->convs.size(); ci++) {
         int l;
         if (ci < (size_t)2 * (Lv - 1)) l = (int)ci / 2;
@@ -588,13 +767,23 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         else l = Lv - 2 - (int)(ci - 2 * Lv) / 2;
         size_t npix = (size_t)B * L[l].H * L[l].W;
         size_t nchunk = (npix + 1023) / 1024;
-        u->partial_n = std::max(u->partial_n, nchunk * u->convs[ci].co * 9 * u->convs[ci].ci);
+        const size_t nw = (size_t)u->convs[ci].co * 9 * u->convs[ci].ci;
+        u->partial_n = std::max(u->partial_n, nchunk * nw);
+        // group sums of the two-level reductions (weights, bias)
+        p2 = std::max(p2, (size_t)chunk_groups((int)nchunk) * nw);
+        p2 = std::max(p2, (size_t)chunk_groups((int)((npix + kBiasChunk - 1) / kBiasChunk)) * u->convs[ci].co);
+    }
+    {
+        const size_t npix0 = (size_t)B * L[0].H * L[0].W;
+        p2 = std::max(p2, (size_t)chunk_groups((int)((npix0 + 4095) / 4096)) * desc->out_ch * (L[0].C + 1));
     }
     u->partial = u->alloc<float>(u->partial_n);
+    u->partial2 = u->alloc<float>(p2);
     u->lpart = u->alloc<double>(nb256((size_t)B * L[0].H * L[0].W * desc->out_ch));
     u->lsum = u->alloc<double>(1);
-    bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->gA && u->gB && u->gC && u->xin && u->y &&
-              u->dy && u->partial && u->lpart && u->lsum;
+    bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->gA && u->gB && u->gC && u->tA && u->tB &&
+              u->xin && u->y &&
+              u->dy && u->partial && u->partial2 && u->lpart && u->lsum;
     for (auto p : u->act) ok = ok && p;
     for (auto p : u->pool) ok = ok && p;
     for (auto p : u->cat) ok = ok && p;
@@ -726,8 +915,7 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const int chunk = 4096, nchunk = (npix0 + chunk - 1) / chunk, np = O * (C0 + 1);
         hipLaunchKernelGGL(k_head_bwd_w, dim3(np, nchunk), dim3(256), 0, u->stream, u->dy, u->act[last], npix0, C0, O, chunk,
                            u->partial);
-        hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial, nchunk, (size_t)np,
-                           u->partial + (size_t)nchunk * np);
+        sum_chunks(u, u->partial, nchunk, (size_t)np, u->partial + (size_t)nchunk * np);
         hipLaunchKernelGGL(k_head_scatter, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial + (size_t)nchunk * np, C0, O,
                            u->g + u->head_woff, u->g + u->head_boff);
     }
@@ -743,13 +931,12 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         hipLaunchKernelGGL(k_relu_bwd, dim3(nb256(n)), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, n);
         {
             // the weight-gradient partial buffer is free until wgrad_launch below
-            const int chunk = 1024, nchunk = (npix + chunk - 1) / chunk;
+            const int chunk = kBiasChunk, nchunk = (npix + chunk - 1) / chunk;
             hipLaunchKernelGGL(k_bias_grad, dim3((c.co + 31) / 32, nchunk), dim3(256), 0, u->stream, gpre, npix, c.co, chunk,
                                u->partial);
-            hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(c.co)), dim3(256), 0, u->stream, u->partial, nchunk, (size_t)c.co,
-                               u->g + c.boff);
+            sum_chunks(u, u->partial, nchunk, (size_t)c.co, u->g + c.boff);
         }
-        wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co);
+        wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB);
         if (need_din)
             conv_launch<float, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
         std::swap(gcur, gtmp);
@@ -827,6 +1014,32 @@ extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, cons
     else conv_launch<float, float, false>(&tmp, (const float *)d_in, (const bf16 *)d_w, nullptr, (float *)d_out, npix, H, W, Ci, Co);
     tmp.stream = nullptr;
     return (int)hipGetLastError();
+}
+
+// Weight gradient of one 3x3 convolution (test and diagnostic entry point;
+// allocates its scratch).  method 0: gathering kernel, 1: channel-major
+// operands (needs W % 16 == 0).
+extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,
+                                     int Ci, int Co, lb_stream_t stream) {
+    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || method < 0 || method > 1) return LB_ERR_INVALID;
+    if (method == 1 && W % 16) return LB_ERR_UNSUPPORTED;
+    const int npix = B * H * W;
+    const size_t nchunk = ((size_t)npix + 1023) / 1024;
+    lb_unet tmp;
+    tmp.stream = (hipStream_t)stream;
+    tmp.partial = tmp.alloc<float>(nchunk * Co * 9 * Ci);
+    tmp.partial2 = tmp.alloc<float>((size_t)chunk_groups((int)nchunk) * Co * 9 * Ci);
+    bf16 *tA = method ? tmp.alloc<bf16>((size_t)npix * Co) : nullptr;
+    bf16 *tB = method ? tmp.alloc<bf16>((size_t)npix * Ci) : nullptr;
+    int rc = 0;
+    if (!tmp.partial || !tmp.partial2 || (method && (!tA || !tB))) rc = LB_ERR_INVALID;
+    if (!rc) {
+        wgrad_launch<bf16>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB);
+        rc = (int)hipGetLastError();
+    }
+    if (!rc) rc = (int)hipStreamSynchronize(tmp.stream);
+    tmp.stream = nullptr;
+    return rc;
 }
 
 // Predicted fields for the batched solver's warm start (see k_unet_fields).

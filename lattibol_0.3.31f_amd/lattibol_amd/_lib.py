@@ -181,6 +181,7 @@ _PROTOS = {
     "lb_unet_sync": (_i, [_vp]),
     "lb_unet_to_fields": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "lb_conv3x3_bf16": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "lb_conv3x3_wgrad_bf16": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_deck_read_params": (_i, [C.c_char_p, C.POINTER(LbDeckParams)]),
     "lb_deck_read_bmp": (_i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "lb_deck_voxelize_stl": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
@@ -206,6 +207,10 @@ def bind_optional(name, res, args):
     fn.argtypes = args
     _PROTOS[name] = (res, args)
     return fn
+
+
+# error codes (include/lattibol_amd.h)
+LB_OK, LB_ERR_INVALID, LB_ERR_SHAPE, LB_ERR_UNSUPPORTED, LB_ERR_NO_DEVICE, LB_ERR_NOT_READY = 0, -1, -2, -3, -4, -5
 
 
 def check(rc, what=""):

@@ -111,6 +111,43 @@ def test_gpu_conv3x3_matches_reference(gpu, ci, co):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("ci,co,H,W", [(16, 32, 16, 16), (32, 64, 8, 32), (96, 32, 16, 16), (384, 128, 16, 16),
+                                       (64, 32, 8, 8)])
+def test_gpu_conv3x3_wgrad_matches_reference(gpu, ci, co, H, W):
+    from lattibol_amd import _lib
+    rng = np.random.default_rng(ci * 7 + co + H)
+    B = 2
+    x = _bf16_round(rng.standard_normal((B, H, W, ci)))
+    g = rng.standard_normal((B, H, W, co)).astype(np.float32)
+    to_u16 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).view(torch.int16).numpy()
+    xd = _lib.DeviceArray.from_numpy(to_u16(x))
+    gd = _lib.DeviceArray.from_numpy(g)
+    out = {}
+    for method in (0, 1):
+        dwd = _lib.DeviceArray((co, 9, ci), np.float32)
+        rc = _lib.lib.lb_conv3x3_wgrad_bf16(method, gd.ptr, xd.ptr, dwd.ptr, B, H, W, ci, co, None)
+        if method == 1 and W % 16:
+            assert rc == _lib.LB_ERR_UNSUPPORTED
+            continue
+        _lib.check(rc, "wgrad")
+        out[method] = dwd.numpy()
+    # float64 reference on the BF16-rounded operands: dW[co, tap, ci] =
+    # sum_{n,h,w} g[n,h,w,co] x[n, h+dy, w+dx, ci] (zero padding)
+    gb = _bf16_round(g).astype(np.float64)
+    xp = np.zeros((B, H + 2, W + 2, ci))
+    xp[:, 1:-1, 1:-1] = x
+    ref = np.zeros((co, 9, ci))
+    for tap in range(9):
+        ky, kx = divmod(tap, 3)
+        ref[:, tap] = np.einsum("nhwo,nhwi->oi", gb, xp[:, ky:ky + H, kx:kx + W])
+    scale = np.abs(ref).max()
+    for method, dw in out.items():
+        np.testing.assert_allclose(dw, ref, rtol=1e-4, atol=1e-5 * scale, err_msg="method %d" % method)
+    if 1 in out:
+        assert np.array_equal(out[0], out[1])   # same operands, same summation order
+
+
+@pytest.mark.gpu
 def test_gpu_unet_forward_backward_match_oracle(gpu):
     from lattibol_amd import _lib
     from lattibol_amd.unet import UNet
