@@ -100,16 +100,32 @@ __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, con
         kbeg = (wave * steps / KS) * 16;
         kend = ((wave + 1) * steps / KS) * 16;
     }
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-        const int tap = k0 / Ci, ci = k0 - tap * Ci + kh;
-        const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
-        bf16x8 a = zero8();
-        if (mvalid && hh >= 0 && hh < H && ww >= 0 && ww < Wd)
-            a = load8(in + ((size_t)(nimg * H + hh) * Wd + ww) * Ci + ci);
-        for (int b = 0; b < NB; b++) {
-            const bf16x8 bw = load8(W + (size_t)(n0 + 32 * b + row) * K + k0 + kh);
-            acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bw, acc[b], 0, 0, 0);
+    // Batches of U K-steps: all operand loads of a batch are issued before
+    // its MFMAs, so a wavefront waits for memory once per batch, not once per
+    // step.  (tap, channel group) advance incrementally (Ci % 16 == 0).
+    constexpr int U = 4;
+    int tap = kbeg / Ci, cc = kbeg - tap * Ci;
+    const bf16 *wrow[NB];
+    for (int b = 0; b < NB; b++) wrow[b] = W + (size_t)(n0 + 32 * b + row) * K + kh;
+    for (int k0 = kbeg; k0 < kend; k0 += 16 * U) {
+        bf16x8 a[U], bw[U][NB];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + 16 * u;
+            a[u] = zero8();
+            for (int b = 0; b < NB; b++) bw[u][b] = zero8();
+            if (k < kend) {
+                const int hh = h + tap / 3 - 1, ww = w + tap % 3 - 1;
+                if (mvalid && hh >= 0 && hh < H && ww >= 0 && ww < Wd)
+                    a[u] = load8(in + ((size_t)(nimg * H + hh) * Wd + ww) * Ci + cc + kh);
+                for (int b = 0; b < NB; b++) bw[u][b] = load8(wrow[b] + k);
+                cc += 16;
+                if (cc == Ci) { cc = 0; tap++; }
+            }
         }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            for (int b = 0; b < NB; b++) acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], bw[u][b], acc[b], 0, 0, 0);
     }
     if constexpr (KS == 1) {
         for (int b = 0; b < NB; b++) {
@@ -246,34 +262,47 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_cm(const bf16 *__restrict
     for (int r = 0; r < 16; r++) acc[r] = 0.f;
     int p = pbeg + kh;
     int n = p / HW, h = (p - n * HW) / Wd, w = p - n * HW - h * Wd;
-#pragma unroll 4
-    for (int p0 = pbeg; p0 < pend; p0 += 16) {
-        const u32x4 av = ld8w(arow + p0 + kh);
-        u32x4 bv = {0u, 0u, 0u, 0u};
-        const int hh = h + dy;
-        if (kvalid && hh >= 0 && hh < H) {
-            const bf16 *q = brow + (size_t)(n * H + hh) * Wd + w;
-            const u32x4 cur = ld8w(q);
-            if (dx == 0) {
-                bv = cur;
-            } else if (dx < 0) {   // element j <- x[w + j - 1]
-                const unsigned prev = w > 0 ? ld8w(q - 8)[3] : 0u;
-                bv[0] = __builtin_amdgcn_alignbit(cur[0], prev, 16);
-                bv[1] = __builtin_amdgcn_alignbit(cur[1], cur[0], 16);
-                bv[2] = __builtin_amdgcn_alignbit(cur[2], cur[1], 16);
-                bv[3] = __builtin_amdgcn_alignbit(cur[3], cur[2], 16);
-            } else {               // element j <- x[w + j + 1]
-                const unsigned next = w + 8 < Wd ? ld8w(q + 8)[0] : 0u;
-                bv[0] = __builtin_amdgcn_alignbit(cur[1], cur[0], 16);
-                bv[1] = __builtin_amdgcn_alignbit(cur[2], cur[1], 16);
-                bv[2] = __builtin_amdgcn_alignbit(cur[3], cur[2], 16);
-                bv[3] = __builtin_amdgcn_alignbit(next, cur[3], 16);
+    // batches of U pixel steps: loads first, then the MFMAs (one memory wait
+    // per batch); steps past pend carry zero operands
+    constexpr int U = 4;
+    for (int p0 = pbeg; p0 < pend; p0 += 16 * U) {
+        u32x4 av[U], cur[U];
+        unsigned edge[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            av[u] = u32x4{0u, 0u, 0u, 0u};
+            cur[u] = u32x4{0u, 0u, 0u, 0u};
+            edge[u] = 0u;
+            if (p0 + 16 * u < pend) {
+                av[u] = ld8w(arow + p0 + 16 * u + kh);
+                const int hh = h + dy;
+                if (kvalid && hh >= 0 && hh < H) {
+                    const bf16 *q = brow + (size_t)(n * H + hh) * Wd + w;
+                    cur[u] = ld8w(q);
+                    if (dx < 0 && w > 0) edge[u] = ld8w(q - 8)[3];
+                    if (dx > 0 && w + 8 < Wd) edge[u] = ld8w(q + 8)[0];
+                }
+                w += 16;
+                if (w >= Wd) { w -= Wd; if (++h == H) { h = 0; n++; } }
             }
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av), __builtin_bit_cast(bf16x8, bv), acc,
-                                                      0, 0, 0);
-        w += 16;
-        if (w >= Wd) { w -= Wd; if (++h == H) { h = 0; n++; } }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            u32x4 bv = cur[u];
+            if (dx < 0) {          // element j <- x[w + j - 1]
+                bv[0] = __builtin_amdgcn_alignbit(cur[u][0], edge[u], 16);
+                bv[1] = __builtin_amdgcn_alignbit(cur[u][1], cur[u][0], 16);
+                bv[2] = __builtin_amdgcn_alignbit(cur[u][2], cur[u][1], 16);
+                bv[3] = __builtin_amdgcn_alignbit(cur[u][3], cur[u][2], 16);
+            } else if (dx > 0) {   // element j <- x[w + j + 1]
+                bv[0] = __builtin_amdgcn_alignbit(cur[u][1], cur[u][0], 16);
+                bv[1] = __builtin_amdgcn_alignbit(cur[u][2], cur[u][1], 16);
+                bv[2] = __builtin_amdgcn_alignbit(cur[u][3], cur[u][2], 16);
+                bv[3] = __builtin_amdgcn_alignbit(edge[u], cur[u][3], 16);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, av[u]), __builtin_bit_cast(bf16x8, bv),
+                                                          acc, 0, 0, 0);
+        }
     }
     __shared__ float sh[4][32][33];
     for (int r = 0; r < 16; r++) {
