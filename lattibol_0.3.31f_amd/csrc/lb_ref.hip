@@ -24,6 +24,11 @@ constexpr int kBlock = 256;
 
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+// Population traffic of the sweeps is streamed once per kernel: non-temporal
+// loads and stores keep it from evicting the flags, borders and plot data.
+__device__ __forceinline__ float ldn(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void stn(float *p, float v) { __builtin_nontemporal_store(v, p); }
+
 inline bool shape_ok(lb_dim3 g, lb_dim3 b, int w, int h, int d) {
     return (int64_t)g.x * b.x == w && (int64_t)g.y * b.y == h && (int64_t)g.z * b.z == d && w > 0 &&
            h > 0 && d > 0;
@@ -45,10 +50,10 @@ __global__ __launch_bounds__(kBlock) void k_ref_stream9(DataD2Q9 d) {
     int64_t o1 = xp + (int64_t)y * W, o2 = x + (int64_t)yp * W;
     int64_t o3 = xp + (int64_t)yp * W, o4 = xn + (int64_t)yp * W;
     float t;
-    t = d.f1[o0]; d.f1[o0] = d.f3[o1]; d.f3[o1] = t;
-    t = d.f2[o0]; d.f2[o0] = d.f4[o2]; d.f4[o2] = t;
-    t = d.f5[o0]; d.f5[o0] = d.f7[o3]; d.f7[o3] = t;
-    t = d.f6[o0]; d.f6[o0] = d.f8[o4]; d.f8[o4] = t;
+    t = ldn(&d.f1[o0]); stn(&d.f1[o0], ldn(&d.f3[o1])); stn(&d.f3[o1], t);
+    t = ldn(&d.f2[o0]); stn(&d.f2[o0], ldn(&d.f4[o2])); stn(&d.f4[o2], t);
+    t = ldn(&d.f5[o0]); stn(&d.f5[o0], ldn(&d.f7[o3])); stn(&d.f7[o3], t);
+    t = ldn(&d.f6[o0]); stn(&d.f6[o0], ldn(&d.f8[o4])); stn(&d.f8[o4], t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_ref_stream19(DataD3Q19 d) {
@@ -64,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_stream19(DataD3Q19 d) {
     int zn = z > 0 ? z - 1 : D - 1, zp = z < D - 1 ? z + 1 : 0;
     auto id = [&](int a, int b, int c) { return a + (int64_t)W * (b + (int64_t)H * c); };
     float t;
-#define LB_SWAP(A, B, OFF) { int64_t o = OFF; t = d.A[o0]; d.A[o0] = d.B[o]; d.B[o] = t; }
+#define LB_SWAP(A, B, OFF) { int64_t o = OFF; t = ldn(&d.A[o0]); stn(&d.A[o0], ldn(&d.B[o])); stn(&d.B[o], t); }
     LB_SWAP(f1, f2, id(xp, y, z));
     LB_SWAP(f3, f4, id(x, yp, z));
     LB_SWAP(f5, f6, id(x, y, zp));
@@ -91,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll9(float *plot, DataD2Q9 d, f
     if (o >= N) return;
     int x = (int)(o % W);
     if (!d.solid[o]) {
-        float v[9] = {d.f0[o], d.f3[o], d.f4[o], d.f1[o], d.f2[o], d.f7[o], d.f8[o], d.f5[o], d.f6[o]};
+        float v[9] = {ldn(&d.f0[o]), ldn(&d.f3[o]), ldn(&d.f4[o]), ldn(&d.f1[o]), ldn(&d.f2[o]),
+                      ldn(&d.f7[o]), ldn(&d.f8[o]), ldn(&d.f5[o]), ldn(&d.f6[o])};
         if (x == 0 && d.dev == 0) zouhe9_in(v, ux_in);
         else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe9_out(v, rho_out);
         float rho, ux, uy;
@@ -108,8 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll9(float *plot, DataD2Q9 d, f
             *(float4 *)&plot[o * 4] = make_float4(ux, uy, 0.f, rho);
             bgk9<float>(v, rho, ex, ey, ti, w0, w1, w5);
         }
-        d.f0[o] = v[0]; d.f1[o] = v[1]; d.f2[o] = v[2]; d.f3[o] = v[3]; d.f4[o] = v[4];
-        d.f5[o] = v[5]; d.f6[o] = v[6]; d.f7[o] = v[7]; d.f8[o] = v[8];
+        stn(&d.f0[o], v[0]); stn(&d.f1[o], v[1]); stn(&d.f2[o], v[2]); stn(&d.f3[o], v[3]); stn(&d.f4[o], v[4]);
+        stn(&d.f5[o], v[5]); stn(&d.f6[o], v[6]); stn(&d.f7[o], v[7]); stn(&d.f8[o], v[8]);
     } else {
         *(float4 *)&plot[o * 4] = make_float4(0.f, 0.f, 0.f, 0.f);
         plot[(o + N) * 4] = 0.f;
@@ -167,10 +173,11 @@ __global__ __launch_bounds__(kBlock) void k_ref_sc9(float *plot, DataD2Q9 d, flo
         ey = ey + (Fsy / (ti * rho));
     }
     *(float4 *)&plot[o * 4] = make_float4(d.ux[o], d.uy[o], 0.f, rho);
-    float v[9] = {d.f0[o], d.f1[o], d.f2[o], d.f3[o], d.f4[o], d.f5[o], d.f6[o], d.f7[o], d.f8[o]};
+    float v[9] = {ldn(&d.f0[o]), ldn(&d.f1[o]), ldn(&d.f2[o]), ldn(&d.f3[o]), ldn(&d.f4[o]),
+                  ldn(&d.f5[o]), ldn(&d.f6[o]), ldn(&d.f7[o]), ldn(&d.f8[o])};
     bgk9<float>(v, rho, ex, ey, ti, w0, w1, w5);
-    d.f0[o] = v[0]; d.f1[o] = v[1]; d.f2[o] = v[2]; d.f3[o] = v[3]; d.f4[o] = v[4];
-    d.f5[o] = v[5]; d.f6[o] = v[6]; d.f7[o] = v[7]; d.f8[o] = v[8];
+    stn(&d.f0[o], v[0]); stn(&d.f1[o], v[1]); stn(&d.f2[o], v[2]); stn(&d.f3[o], v[3]); stn(&d.f4[o], v[4]);
+    stn(&d.f5[o], v[5]); stn(&d.f6[o], v[6]); stn(&d.f7[o], v[7]); stn(&d.f8[o], v[8]);
 }
 
 // Collision D3Q19 (collision.cu:357-552).
@@ -184,9 +191,10 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll19(float *plot, DataD3Q19 d,
     int x = (int)(o % W);
     if (!d.solid[o]) {
         // natural slots after the local swap (collision.cu:375-410)
-        float v[19] = {d.f0[o],  d.f2[o],  d.f1[o],  d.f4[o],  d.f3[o],  d.f6[o],  d.f5[o],
-                       d.f12[o], d.f11[o], d.f14[o], d.f13[o], d.f8[o],  d.f7[o],  d.f10[o],
-                       d.f9[o],  d.f18[o], d.f17[o], d.f16[o], d.f15[o]};
+        float v[19] = {ldn(&d.f0[o]),  ldn(&d.f2[o]),  ldn(&d.f1[o]),  ldn(&d.f4[o]),  ldn(&d.f3[o]),
+                       ldn(&d.f6[o]),  ldn(&d.f5[o]),  ldn(&d.f12[o]), ldn(&d.f11[o]), ldn(&d.f14[o]),
+                       ldn(&d.f13[o]), ldn(&d.f8[o]),  ldn(&d.f7[o]),  ldn(&d.f10[o]), ldn(&d.f9[o]),
+                       ldn(&d.f18[o]), ldn(&d.f17[o]), ldn(&d.f16[o]), ldn(&d.f15[o])};
         if (x == 0 && d.dev == 0) zouhe19_in(v, ux_in);
         else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe19_out(v, rho_out);
         float rho, ux, uy, uz;
@@ -194,10 +202,10 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll19(float *plot, DataD3Q19 d,
         d.ux[o] = ux; d.uy[o] = uy; d.uz[o] = uz; d.rho[o] = rho;
         *(float4 *)&plot[o * 4] = make_float4(ux, uy, uz, rho);
         bgk19<float>(v, rho, ux, uy, uz, ti, w0, w1, w7);
-        d.f0[o] = v[0]; d.f1[o] = v[1]; d.f2[o] = v[2]; d.f3[o] = v[3]; d.f4[o] = v[4];
-        d.f5[o] = v[5]; d.f6[o] = v[6]; d.f7[o] = v[7]; d.f8[o] = v[8]; d.f9[o] = v[9];
-        d.f10[o] = v[10]; d.f11[o] = v[11]; d.f12[o] = v[12]; d.f13[o] = v[13];
-        d.f14[o] = v[14]; d.f15[o] = v[15]; d.f16[o] = v[16]; d.f17[o] = v[17]; d.f18[o] = v[18];
+        float *dst[19] = {d.f0,  d.f1,  d.f2,  d.f3,  d.f4,  d.f5,  d.f6,  d.f7,  d.f8, d.f9,
+                          d.f10, d.f11, d.f12, d.f13, d.f14, d.f15, d.f16, d.f17, d.f18};
+#pragma unroll
+        for (int i = 0; i < 19; i++) stn(&dst[i][o], v[i]);
     } else {
         *(float4 *)&plot[o * 4] = make_float4(0.f, 0.f, 0.f, 0.f);
         d.ux[o] = 0.f; d.uy[o] = 0.f; d.uz[o] = 0.f; d.rho[o] = 0.f;
@@ -336,18 +344,49 @@ extern "C" void apply_border_d3q19(DataD3Q19 *data, lb_dim3 grid, lb_dim3 block)
 }
 
 // copy_border.cu:26-81: nb_border_* <- border_* with stream-ordered copies.
+// Border planes of copy_border_devices_*: one launch for all of them when
+// every pointer is device memory of the current device (the subdomains of
+// one GPU), else one hipMemcpyAsync per plane (peer or host memory).
+namespace {
+struct PlaneCopy { const float *src[10]; float *dst[10]; };
+__global__ __launch_bounds__(kBlock) void k_copy_planes(PlaneCopy c, int64_t n) {
+    const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (e < n) c.dst[blockIdx.y][e] = c.src[blockIdx.y][e];
+}
+
+bool on_current_device(const void *p) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+    return a.type == hipMemoryTypeDevice && a.device == dev;
+}
+
+void copy_planes(float *const *dst, float *const *src, int k, int64_t n, hipStream_t st) {
+    bool local = true;
+    for (int i = 0; i < k && local; i++) local = on_current_device(dst[i]) && on_current_device(src[i]);
+    if (local) {
+        PlaneCopy c{};
+        for (int i = 0; i < k; i++) { c.src[i] = src[i]; c.dst[i] = dst[i]; }
+        hipLaunchKernelGGL(k_copy_planes, dim3(nblocks(n), k), dim3(kBlock), 0, st, c, n);
+        check_launch();
+        return;
+    }
+    for (int i = 0; i < k; i++)
+        if (lb_hip_err(hipMemcpyAsync(dst[i], src[i], sizeof(float) * (size_t)n, hipMemcpyDefault, st))) return;
+}
+}  // namespace
+
 extern "C" void copy_border_devices_d2q9(float *nb1, float *nb5, float *nb8, float *nb3, float *nb6,
                                          float *nb7, float *b1, float *b5, float *b8, float *b3,
                                          float *b6, float *b7, int height, lb_stream_t *s) {
     if (height <= 0) { lb_set_error(LB_ERR_INVALID); return; }
     hipStream_t st = s ? (hipStream_t)*s : nullptr;
-    size_t bytes = sizeof(float) * (size_t)height;
     float *dst[6] = {nb1, nb5, nb8, nb3, nb6, nb7};
     float *src[6] = {b1, b5, b8, b3, b6, b7};
-    for (int k = 0; k < 6; k++) {
+    for (int k = 0; k < 6; k++)
         if (!dst[k] || !src[k]) { lb_set_error(LB_ERR_INVALID); return; }
-        if (lb_hip_err(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDefault, st))) return;
-    }
+    copy_planes(dst, src, 6, height, st);
 }
 
 extern "C" void copy_border_devices_d3q19(float *nb1, float *nb7, float *nb8, float *nb9, float *nb10,
@@ -357,13 +396,11 @@ extern "C" void copy_border_devices_d3q19(float *nb1, float *nb7, float *nb8, fl
                                           float *b14, int height, int depth, lb_stream_t *s) {
     if (height <= 0 || depth <= 0) { lb_set_error(LB_ERR_INVALID); return; }
     hipStream_t st = s ? (hipStream_t)*s : nullptr;
-    size_t bytes = sizeof(float) * (size_t)height * depth;
     float *dst[10] = {nb1, nb7, nb8, nb9, nb10, nb2, nb11, nb12, nb13, nb14};
     float *src[10] = {b1, b7, b8, b9, b10, b2, b11, b12, b13, b14};
-    for (int k = 0; k < 10; k++) {
+    for (int k = 0; k < 10; k++)
         if (!dst[k] || !src[k]) { lb_set_error(LB_ERR_INVALID); return; }
-        if (lb_hip_err(hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDefault, st))) return;
-    }
+    copy_planes(dst, src, 10, (int64_t)height * depth, st);
 }
 
 extern "C" void read_value_2d(float *plot_data, DataD2Q9 *data, int x, int y, int z) {
