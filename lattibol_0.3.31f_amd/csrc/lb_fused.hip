@@ -48,10 +48,10 @@ __global__ void k_bump(uint32_t *ctr, uint32_t n) {
     if (threadIdx.x == 0) ctr[0] += n;
 }
 
-template <int Q, class S>
+template <int Q, class S, int MEM = 0>
 __global__ __launch_bounds__(kThreads) void k_density(const StepArgs<S> *__restrict__ pa) {
     const StepArgs<S> &a = *pa;
-    density_body<Q, S>(a, node_of(a, tile_of_block(a)));
+    density_body<Q, S, MEM>(a, node_of(a, tile_of_block(a)));
 }
 
 // Equilibrium initialisation from device fields.
@@ -325,10 +325,13 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     size_t lds = 0;
     if (sc) {
         lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
-        hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        if (c->mem == 2) hipLaunchKernelGGL((k_density<Q, S, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        else hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
     }
     if (sc) {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
+        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, true, false, 1>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
+        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, true, false, 2>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
         else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
     } else {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);

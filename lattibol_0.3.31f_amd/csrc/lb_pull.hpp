@@ -588,7 +588,7 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
 
 // Shan-Chen density pre-pass: psi(rho) of the post-streaming density, with
 // the same pull / boundary handling (and summation order) as k_step.
-template <int Q, class S>
+template <int Q, class S, int MEM = 0>
 LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
     typedef typename S::T T;
     if (!p.valid) return;
@@ -601,7 +601,7 @@ LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
         rho = (T)a.bcrho[fl >> 4];
     } else {
         T f[Q];
-        pull_node<Q, S>(a, p, bm, f);
+        pull_node<Q, S, MEM>(a, p, bm, f);
         apply_zou_he<Q, S>(a, p, f);
         T ux, uy, uz;
         if constexpr (S::DEV) {
