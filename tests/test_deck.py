@@ -187,6 +187,37 @@ def test_stl_sphere(tmp_path):
     assert v[inside].all() and not v[outside].any()
 
 
+def test_reference_sphere_deck_voxelises_to_its_sphere():
+    """The reference's 3D sample deck (samples/esfera.dat + esfera.stl): a
+    sphere inside the 600 x 300 x 300 extent set by four corner triangles,
+    voxelised to maxDim = 256.  The sphere's centre and radius come from the
+    mesh's own vertices; the solid voxels must have the sphere's volume and
+    centre in voxel units and lie inside / outside it away from the surface."""
+    path = os.path.join(DECKS, "esfera.stl")
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[80:84], "<u4")[0])
+    rec = np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")])
+    verts = np.frombuffer(raw[84:84 + 50 * n], rec)["v"].reshape(-1, 3).astype(np.float64)
+    lo, hi = verts.min(0), verts.max(0)
+    on_box = np.any(np.isclose(verts, lo) | np.isclose(verts, hi), axis=1)
+    sph = verts[~on_box]
+    centre = (sph.min(0) + sph.max(0)) / 2
+    radius = np.linalg.norm(sph - centre, axis=1).mean()
+    p = deck.read_params(os.path.join(DECKS, "esfera.dat"))
+    assert p["max_dim"] == 256
+    v = deck.voxelize_stl(path, p["max_dim"])
+    scale = (hi - lo).max() / 256                     # mesh units per voxel
+    assert v.shape == tuple(int(round(e / scale)) for e in (hi - lo)[::-1])
+    r_vox, c_vox = radius / scale, (centre - lo) / scale
+    count = int(v.sum())
+    assert abs(count / (4 / 3 * np.pi * r_vox ** 3) - 1) < 0.01, (count, r_vox)
+    cz, cy, cx = np.argwhere(v).mean(0) + 0.5
+    np.testing.assert_allclose([cx, cy, cz], c_vox, atol=0.05)
+    z, y, x = np.meshgrid(*(np.arange(k) + 0.5 for k in v.shape), indexing="ij")
+    d = np.sqrt((x - c_vox[0]) ** 2 + (y - c_vox[1]) ** 2 + (z - c_vox[2]) ** 2)
+    assert v[d < r_vox - 0.5].all() and not v[d > r_vox + 0.5].any()
+
+
 # ---------------------------------------------------------------------------
 # VTK
 # ---------------------------------------------------------------------------

@@ -1,7 +1,9 @@
 """The reference's sample decks as regression scenes (tests/golden/decks):
-loaded with the deck readers, run by the reference driver on the GPU, checked
-bit for bit against the oracle over a short run and, for the body-force
-channel, against the physics of a plane Poiseuille channel at steady state."""
+loaded with the deck readers (BMP decks in 2D; esfera: the STL sphere in a
+256 x 128 x 128 channel, D3Q19), run by the reference driver on the GPU,
+checked bit for bit against the oracle over a short run and, for the
+body-force channel, against the physics of a plane Poiseuille channel at
+steady state."""
 import os
 
 import numpy as np
@@ -19,7 +21,7 @@ def ref_oracle(d):
                        rho=p["rho"], viscosity=p["viscosity"], Fx=p["fx"], Fy=p["fy"], G=p["G"], Gads=p["Gads"])
 
 
-@pytest.mark.parametrize("name,steps", [("fig31_force", 40), ("fig52_angle90", 60)])
+@pytest.mark.parametrize("name,steps", [("fig31_force", 40), ("fig52_angle90", 60), ("esfera", 6)])
 def test_sample_deck_matches_oracle(gpu, name, steps):
     from lattibol_amd import deck
     d = deck.load(os.path.join(DECKS, name + ".dat"))
