@@ -239,6 +239,52 @@ def test_halo_planes_equal_periodic_z(gpu, layout):
     np.testing.assert_array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_two_slab_ring_equals_one_lattice(gpu, layout):
+    """The z-slab ring protocol on one GPU: two solvers own z = 0..11 and
+    12..23 of a periodic-z channel; before every single step the planes each
+    one packed (left in its recv regions, see lb_halo_buffers) are swapped
+    between them (recv_hi <-> recv_hi, recv_lo <-> recv_lo), which hands each
+    slab its neighbour's outgoing planes.  The two slabs must reproduce the
+    24-plane lattice bit for bit.  The obstacle stays clear of the slab
+    boundaries: each slab's link masks see only its own solids."""
+    import ctypes as C
+    from lattibol_amd import scenes
+    from lattibol_amd._lib import check, lib
+    nx, ny, nz, half, steps = 64, 32, 24, 12, 60
+    kw = dict(radius=4, center=(20, 16, 6), halo_z=True, layout=layout)
+    cfg, solid, init = scenes.sphere_channel(nx, ny, nz, **kw)
+    assert not solid[[0, half - 1, half, nz - 1]].any()
+    full = gpu.Solver(cfg, solid)
+    full.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
+    slabs = []
+    for k in range(2):
+        cfg_k, _, _ = scenes.sphere_channel(nx, ny, half, **kw)
+        s_k = gpu.Solver(cfg_k, np.ascontiguousarray(solid[k * half:(k + 1) * half]))
+        s_k.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
+        slabs.append(s_k)
+    nbytes = 5 * nx * ny * 4
+    host = [np.empty(nbytes, np.uint8) for _ in range(2)]
+    hp = [h.ctypes.data_as(C.c_void_p) for h in host]
+    for t in range(steps):
+        for s_k in slabs:
+            s_k.sync()
+        hb = [s_k.halo_buffers() for s_k in slabs]
+        for key in ("recv_hi", "recv_lo"):
+            for k in range(2):
+                check(lib.lb_memcpy_d2h(hp[k], C.c_void_p(hb[k][key]), nbytes), "d2h")
+            for k in range(2):
+                check(lib.lb_memcpy_h2d(C.c_void_p(hb[k][key]), hp[1 - k], nbytes), "h2d")
+        for s_k in slabs:
+            s_k.step(1)
+    full.step(steps)
+    ref = full.populations()
+    for k, s_k in enumerate(slabs):
+        np.testing.assert_array_equal(s_k.populations(), ref[:, k * half:(k + 1) * half], err_msg="slab %d" % k)
+        s_k.close()
+    full.close()
+
+
 def test_residual_read_value_and_steps(gpu):
     from lattibol_amd import scenes
     cfg, solid, init = scenes.cavity(n=64)
