@@ -355,6 +355,11 @@ int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_
  * lb_unet_backward uses the same kernels on the handle's buffers. */
 int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,
                           int Ci, int Co, lb_stream_t stream);
+/* Diagnostic: rounds every 32-bit pattern to BF16 on the device with the
+ * conversion instruction the BF16 stores use and with the bit-pattern
+ * round-to-nearest-even of the oracle; *mismatches = non-NaN inputs where
+ * the two differ (0 expected).  Synchronises; allocates 8 bytes. */
+int lb_check_bf16_rounding(uint64_t *mismatches);
 
 /* ---------------------------------------------------------------------- */
 /* 5. Input decks and VTK output (host only)                                */

@@ -181,7 +181,36 @@ __global__ __launch_bounds__(256) void k_copy16(const uint4 *src, uint4 *dst, si
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256) dst[e] = src[e];
 }
 
+// Every 32-bit pattern through both BF16 roundings; counts the non-NaN
+// inputs where they differ (one atomic per wavefront).
+__global__ __launch_bounds__(256) void k_check_bf16(unsigned long long *bad) {
+    unsigned long long n = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += (uint64_t)gridDim.x * 256) {
+        const float f = __builtin_bit_cast(float, (uint32_t)i);
+        if (f == f && f32_to_bf16(f) != f32_to_bf16_bits(f)) n++;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(bad, n);
+}
+
 }  // namespace
+
+extern "C" int lb_check_bf16_rounding(uint64_t *mismatches) {
+    if (!mismatches) return LB_ERR_INVALID;
+    unsigned long long *d = nullptr;
+    int rc = (int)hipMalloc((void **)&d, sizeof(*d));
+    if (rc) return rc;
+    rc = (int)hipMemset(d, 0, sizeof(*d));
+    if (!rc) {
+        hipLaunchKernelGGL(k_check_bf16, dim3(16384), dim3(256), 0, 0, d);
+        rc = (int)hipGetLastError();
+    }
+    unsigned long long h = 0;
+    if (!rc) rc = (int)hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    *mismatches = (uint64_t)h;
+    return rc;
+}
 
 extern "C" int lb_copy_device(void *dst, const void *src, size_t bytes, int repeats, float *ms_per_copy) {
     if (!dst || !src || bytes % 16 || repeats < 1) return LB_ERR_INVALID;

@@ -182,6 +182,7 @@ _PROTOS = {
     "lb_unet_to_fields": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "lb_conv3x3_bf16": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_conv3x3_wgrad_bf16": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "lb_check_bf16_rounding": (_i, [C.POINTER(C.c_uint64)]),
     "lb_deck_read_params": (_i, [C.c_char_p, C.POINTER(LbDeckParams)]),
     "lb_deck_read_bmp": (_i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "lb_deck_voxelize_stl": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),

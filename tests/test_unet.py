@@ -111,6 +111,17 @@ def test_gpu_conv3x3_matches_reference(gpu, ci, co):
 
 
 @pytest.mark.gpu
+def test_gpu_bf16_conversion_instruction_matches_bit_rounding(gpu):
+    # every non-NaN float: v_cvt_pk_bf16_f32 == round-to-nearest-even on the
+    # bit pattern (the oracle's formula), so BF16 stores stay bit-exact
+    import ctypes as C
+    from lattibol_amd import _lib
+    bad = C.c_uint64(123)
+    _lib.check(_lib.lib.lb_check_bf16_rounding(C.byref(bad)), "lb_check_bf16_rounding")
+    assert bad.value == 0
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("ci,co,H,W", [(16, 32, 16, 16), (32, 64, 8, 32), (96, 32, 16, 16), (384, 128, 16, 16),
                                        (64, 32, 8, 8)])
 def test_gpu_conv3x3_wgrad_matches_reference(gpu, ci, co, H, W):
