@@ -75,7 +75,7 @@ def cpu_baseline(cfg, solid, init, steps_hint):
         t0 = time.perf_counter()
         orc.run(2)
         per = (time.perf_counter() - t0) / 2
-        steps = max(2, min(200, int(15.0 / max(per, 1e-6))))
+        steps = max(2, min(2000, int(15.0 / max(per, 1e-6))))   # about 15 s of CPU work
     t0 = time.perf_counter()
     orc.run(steps)
     dt = time.perf_counter() - t0
