@@ -218,6 +218,10 @@ typedef struct {
                                node and population, reproducible) */
     int node_words;         /* -1 autotune, 0 flag byte (+ link mask for nodes next to
                                walls / solids), 1 one packed 32-bit word per node */
+    int time_block;         /* steps per launch through an LDS tile (temporal blocking,
+                               for small launch-latency-bound lattices): -1 autotune,
+                               0 off, 4 on.  D2Q9 FP32 only, faces periodic / wall /
+                               moving wall, no force, Shan-Chen or Zou-He; else 0 */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

@@ -48,6 +48,96 @@ __global__ void k_bump(uint32_t *ctr, uint32_t n) {
     if (threadIdx.x == 0) ctr[0] += n;
 }
 
+// Temporal blocking for small D2Q9 FP32 lattices, whose steps are bound by
+// launch latency rather than HBM: a workgroup loads a 16 x 16 tile plus a
+// TB-node halo into LDS, advances it TB steps there (the valid region shrinks
+// by one node per step) and stores the tile once.  Each node runs the same
+// pull / moving-wall / BGK arithmetic as k_step, so the result is identical
+// to TB single steps.  Faces are periodic (the halo wraps) or walls (links
+// leaving the lattice bounce, as their link masks say); the host admits no
+// other boundary (lb_create: time_block).
+constexpr int kTB = 4, kTBI = 16, kTBE = kTBI + 2 * kTB;   // steps, tile, tile + halo
+constexpr uint32_t kTBSkip = 0x7fffffffu;                    // cell not updated (solid / outside)
+
+// One thread per tile cell (576 = 9 wavefronts): each of the TB steps is one
+// pass of independent LDS loads, BGK and LDS stores between two barriers.
+constexpr int kTBThreads = kTBE * kTBE;
+__global__ __launch_bounds__(kTBThreads) void k_step_tb9(const StepArgs<StF32> *__restrict__ pa) {
+    typedef Vel<9> L;
+    const StepArgs<StF32> &a = *pa;
+    __shared__ float fs[2][9][kTBE * kTBE];
+    __shared__ uint32_t bms[kTBE * kTBE];
+    const int ntx = (a.nx + kTBI - 1) / kTBI;
+    const int x0 = (int)(blockIdx.x % ntx) * kTBI - kTB, y0 = (int)(blockIdx.x / ntx) * kTBI - kTB;
+    const int c = threadIdx.x;
+    const int lx = c % kTBE, ly = c / kTBE;
+    {
+        int gx = x0 + lx, gy = y0 + ly;
+        const bool in = axis_src(gx, a.nx, a.bc[0], a.bc[1]) && axis_src(gy, a.ny, a.bc[2], a.bc[3]);
+        uint32_t code = kTBSkip;
+        if (in) {
+            NodePos p{};
+            p.x = gx; p.y = gy; p.z = 0;
+            p.id = (uint32_t)gx + (uint32_t)a.nx * (uint32_t)gy;
+            p.e = (uint32_t)gx + a.sy * (uint32_t)gy;
+            uint8_t fl;
+            uint32_t bm;
+            node_word(a, p, fl, bm);
+            if (!(fl & F_SOLID)) code = bm;
+            static_for<0, 9>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                fs[0][i][c] = ldm(&a.A[(size_t)i * a.ps + p.e]);
+            });
+        }
+        bms[c] = code;
+    }
+    __syncthreads();
+    const uint32_t bm = bms[c];
+    for (int s = 1; s <= kTB; s++) {
+        const int sb = (s - 1) & 1, db = s & 1;
+        if (bm != kTBSkip && lx >= s && lx < kTBE - s && ly >= s && ly < kTBE - s) {
+            float f[9];
+            static_for<0, 9>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                constexpr int j = L::opp(i);
+                f[i] = ((bm >> i) & 1u) ? fs[sb][j][c] : fs[sb][i][c - L::cx(i) - L::cy(i) * kTBE];
+            });
+            if (bm >> 31) {
+                NodePos p{};
+                p.x = x0 + lx; p.y = y0 + ly; p.z = 0;   // inside the lattice: the cell is not skipped
+                static_for<1, 9>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    if ((bm >> i) & 1u) {
+                        const int wf = wall_face<9>(a, p, L::cx(i), L::cy(i), L::cz(i));
+                        if (wf >= 0 && a.bc[wf] == LB_BC_MOVING_WALL) {
+                            float cu = (float)L::cx(i) * a.bcu[wf][0] + (float)L::pcy(i) * a.bcu[wf][1] +
+                                       (float)L::cz(i) * a.bcu[wf][2];
+                            f[i] = f[i] + (6.0f * weight<9, float>(L::wc(i))) * cu;
+                        }
+                    }
+                });
+            }
+            float rho, ux, uy, uz;
+            moments<9, float>(f, rho, ux, uy, uz);
+            collide<9, float>(f, rho, ux, uy, uz, a.ti, a.rest_closure != 0);
+            static_for<0, 9>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                fs[db][i][c] = f[i];
+            });
+        }
+        __syncthreads();
+    }
+    constexpr int fb = kTB & 1;
+    const int gx = x0 + lx, gy = y0 + ly;
+    if (lx >= kTB && lx < kTB + kTBI && ly >= kTB && ly < kTB + kTBI && gx < a.nx && gy < a.ny && bm != kTBSkip) {
+        const uint32_t e = (uint32_t)gx + a.sy * (uint32_t)gy;
+        static_for<0, 9>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            stm(&a.B[(size_t)i * a.ps + e], fs[fb][i][c]);
+        });
+    }
+}
+
 template <int Q, class S, int MEM = 0>
 __global__ __launch_bounds__(kThreads) void k_density(const StepArgs<S> *__restrict__ pa) {
     const StepArgs<S> &a = *pa;
@@ -259,11 +349,13 @@ struct lb_ctx {
     long long steps = 0;
     std::vector<uint8_t> solid;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_end[2] = {0, 1};        // buffer holding the result of graph[s]
     int graph_steps = 0;
     int bx_shift = 6, by = 4;
     int xcd = 0;
     int mem = 0;                      // population load/store hints (see ldm/stm)
     int layout = LB_LAYOUT_SOA;       // population layout (LB_LAYOUT_*)
+    int tb = 0;                       // steps per temporally blocked launch (0: off)
     void *dargs = nullptr;            // two argument blocks (source buffer 0 / 1)
     static constexpr size_t kArgBytes = 512;
 
@@ -400,6 +492,40 @@ int launch_bump(lb_ctx *c, uint32_t n) {
     return (int)hipGetLastError();
 }
 
+// Lattices the temporally blocked kernel reproduces exactly (see k_step_tb9).
+bool tb_eligible(const lb_ctx *c) {
+    const lb_config &g = c->cfg;
+    if (c->Q != 9 || g.storage != LB_STORE_F32 || g.sc_G != 0.0f || g.zou_he || g.bb_full) return false;
+    if (g.force[0] != 0.0f || g.force[1] != 0.0f || g.force[2] != 0.0f) return false;
+    // the halo of the last tile row / column reaches 18 nodes past the edge:
+    // one periodic wrap must bring every halo node back into the lattice
+    if (g.nx < kTBI + kTB || g.ny < kTBI + kTB) return false;
+    for (int k = 0; k < 4; k++)
+        if (g.bc[k] != LB_BC_PERIODIC && g.bc[k] != LB_BC_WALL && g.bc[k] != LB_BC_MOVING_WALL) return false;
+    return true;
+}
+
+// One launch advancing c->tb steps from buffer src into buffer 1 - src.
+int launch_tb(lb_ctx *c, int src) {
+    const StepArgs<StF32> *pa = (const StepArgs<StF32> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
+    const unsigned nb = (unsigned)(((c->cfg.nx + kTBI - 1) / kTBI) * ((c->cfg.ny + kTBI - 1) / kTBI));
+    hipLaunchKernelGGL(k_step_tb9, dim3(nb), dim3(kTBThreads), 0, c->stream, pa);
+    return (int)hipGetLastError();
+}
+
+// n plain steps from buffer src (temporally blocked launches where enabled);
+// returns the buffer holding the result.
+int launch_plain_steps(lb_ctx *c, int src, int n, int *rc) {
+    int s = 0;
+    *rc = 0;
+    while (s < n && !*rc) {
+        if (c->tb && n - s >= c->tb) { *rc = launch_tb(c, src); s += c->tb; }
+        else { *rc = launch_step(c, src, false, (uint32_t)s); s++; }
+        src = 1 - src;
+    }
+    return src;
+}
+
 template <int Q, class S> int launch_reduce_t(lb_ctx *c) {
     hipLaunchKernelGGL((k_reduce<Q, S>), dim3(kReduceBlocks), dim3(kThreads), 0, c->stream,
                        (const typename S::raw *)c->buf[c->cur], c->flags, c->N, (uint32_t)c->cfg.nx,
@@ -484,27 +610,28 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
             hipGraph_t g = nullptr;
             int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
             if (rc) return rc;
-            int src = c->cur;
-            for (int s = 0; s < chunk; s++) { launch_step(c, src, false, (uint32_t)s); src = 1 - src; }
+            int lrc = 0;
+            c->graph_end[c->cur] = launch_plain_steps(c, c->cur, chunk, &lrc);
             launch_bump(c, (uint32_t)chunk);
             rc = (int)hipStreamEndCapture(c->stream, &g);
-            if (rc) return rc;
+            if (!rc) rc = lrc;
+            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
             rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             if (rc) { ge = nullptr; return rc; }
         }
         int rc = (int)hipGraphLaunch(ge, c->stream);
         if (rc) return rc;
-        if (chunk & 1) c->cur = 1 - c->cur;
+        c->cur = c->graph_end[c->cur];
         nsteps -= chunk;
         c->steps += chunk;
     }
-    for (int s = 0; s < nsteps; s++) {
-        int rc = launch_step(c, c->cur, false);
-        if (!rc) rc = launch_bump(c, 1);
+    if (nsteps > 0) {
+        int rc = 0;
+        c->cur = launch_plain_steps(c, c->cur, nsteps, &rc);
+        if (!rc) rc = launch_bump(c, (uint32_t)nsteps);
         if (rc) return rc;
-        c->cur = 1 - c->cur;
-        c->steps++;
+        c->steps += nsteps;
     }
     return 0;
 }
@@ -569,6 +696,45 @@ int autotune(lb_ctx *c) {
     return rc;
 }
 
+// Temporal blocking on or off: the faster of 64 steps through a graph each
+// way (launch-latency-bound lattices gain, large ones lose to the halo
+// recomputation).
+int tune_tb(lb_ctx *c) {
+    float best = 1e30f;
+    int best_tb = 0;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0)) return LB_ERR_INVALID;
+    if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
+    int rc = 0;
+    for (int tb : {0, kTB}) {
+        c->tb = tb;
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (rc) break;
+        int lrc = 0;
+        (void)launch_plain_steps(c, 0, 64, &lrc);
+        rc = (int)hipStreamEndCapture(c->stream, &g);
+        if (!rc) rc = lrc;
+        if (!rc) rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        float ms = 1e30f;
+        if (!rc) rc = (int)hipGraphLaunch(ge, c->stream);   // warm
+        if (!rc) rc = (int)hipEventRecord(e0, c->stream);
+        for (int r = 0; r < 3 && !rc; r++) rc = (int)hipGraphLaunch(ge, c->stream);
+        if (!rc) rc = (int)hipEventRecord(e1, c->stream);
+        if (!rc) rc = (int)hipEventSynchronize(e1);
+        if (!rc) rc = (int)hipEventElapsedTime(&ms, e0, e1);
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (rc) break;
+        if (ms < best) { best = ms; best_tb = tb; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->tb = best_tb;
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
@@ -586,7 +752,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
     if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1 ||
-        cfg->node_words > 1)
+        cfg->node_words > 1 || (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != kTB))
         return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
@@ -637,8 +803,14 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             rc = autotune(c);
         }
     }
+    if (!rc && cfg->time_block == kTB && !tb_eligible(c)) rc = LB_ERR_UNSUPPORTED;
+    if (!rc) {
+        if (cfg->time_block == kTB) c->tb = kTB;
+        else if (cfg->time_block < 0 && tb_eligible(c)) rc = tune_tb(c);
+    }
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     if (rc) { delete c; return rc; }
+    c->cfg.time_block = c->tb;
     c->cfg.block_x = 1 << c->bx_shift;
     c->cfg.block_y = c->by;
     c->cfg.xcd_remap = c->xcd;

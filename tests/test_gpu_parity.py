@@ -184,6 +184,55 @@ def test_fused_solver_matches_oracle(gpu, case, layout, words):
     sol.close()
 
 
+def _tb_scene(kind):
+    from lattibol_amd import scenes
+    from lattibol_amd.solver import make_config
+    if kind == "cavity64":
+        return scenes.cavity(n=64, time_block=4)
+    if kind == "ragged50x37":
+        cfg = make_config(9, (50, 37), tau=0.7, bc={"-x": "wall", "+x": "wall", "-y": "moving_wall", "+y": "wall"},
+                          bc_u={"-y": (0.08, 0.0, 0.0)}, time_block=4)
+        y, x = np.mgrid[0:37, 0:50]
+        solid = (((x - 30) ** 2 + (y - 18) ** 2) <= 25).astype(np.uint8)[None]
+        return cfg, solid, dict(rho=1.0, ux=0.0, uy=0.0, uz=0.0)
+    # periodic box with an obstacle and a uniform initial flow
+    cfg = make_config(9, (40, 24), tau=0.6, bc={f: "periodic" for f in ("-x", "+x", "-y", "+y")}, time_block=4)
+    y, x = np.mgrid[0:24, 0:40]
+    solid = (((x - 12) ** 2 + (y - 11) ** 2) <= 16).astype(np.uint8)[None]
+    return cfg, solid, dict(rho=1.0, ux=0.05, uy=0.01, uz=0.0)
+
+
+@pytest.mark.parametrize("words", [0, 1])
+@pytest.mark.parametrize("layout", [0, 1])
+@pytest.mark.parametrize("kind,steps", [("cavity64", 1003), ("ragged50x37", 501), ("periodic40x24", 402)])
+def test_time_blocked_steps_match_oracle(gpu, kind, steps, layout, words):
+    """Four steps per launch through the LDS tile: the same populations as
+    the oracle's single steps, bit for bit (walls, moving lid, obstacles,
+    periodic faces, ragged tiles, step counts not divisible by 4)."""
+    cfg, solid, init = _tb_scene(kind)
+    cfg["layout"], cfg["node_words"] = layout, words
+    print("case", kind, layout, words, flush=True)
+    sol, orc = _run_pair(gpu, cfg, solid, init, steps)
+    assert sol.launch_config["time_block"] == 4
+    fluid = solid.reshape(orc.shape) == 0
+    _assert_pop_parity(sol.populations(), orc.A, cfg["storage"], fluid)
+    m, mo = sol.macros(), orc.macros()
+    for k in ("rho", "ux", "uy"):
+        np.testing.assert_array_equal(m[k][fluid], mo[k][fluid].astype(np.float32))
+    sol.close()
+
+
+def test_time_block_rejected_where_it_cannot_apply(gpu):
+    from lattibol_amd import scenes
+    from lattibol_amd._lib import LatticeError
+    cfg, solid, init = scenes.sphere_channel(32, 16, 8, radius=2, center=(8, 8, 4), time_block=4)
+    with pytest.raises(LatticeError):
+        gpu.Solver(cfg, solid)
+    cfg, solid, init = scenes.cavity(n=32, storage="f16", time_block=4)
+    with pytest.raises(LatticeError):
+        gpu.Solver(cfg, solid)
+
+
 def test_stochastic_rounding_stream_continues_across_calls(gpu):
     """The device step counter of the rounding stream survives graph chunks,
     eager steps and separate lb_step calls: 100 + 77 steps equal the oracle's
