@@ -220,8 +220,10 @@ typedef struct {
                                walls / solids), 1 one packed 32-bit word per node */
     int time_block;         /* steps per launch through an LDS tile (temporal blocking,
                                for small launch-latency-bound lattices): -1 autotune,
-                               0 off, 4 on.  D2Q9 FP32 only, faces periodic / wall /
-                               moving wall, no force, Shan-Chen or Zou-He; else 0 */
+                               0 off, 4 or 8.  D2Q9 FP32 only, faces periodic / wall /
+                               moving wall, no force, Shan-Chen or Zou-He, nx and ny
+                               >= 16 + time_block; else 0 (a fixed value that cannot
+                               apply is LB_ERR_UNSUPPORTED) */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

@@ -55,22 +55,24 @@ __global__ void k_bump(uint32_t *ctr, uint32_t n) {
 // pull / moving-wall / BGK arithmetic as k_step, so the result is identical
 // to TB single steps.  Faces are periodic (the halo wraps) or walls (links
 // leaving the lattice bounce, as their link masks say); the host admits no
-// other boundary (lb_create: time_block).
-constexpr int kTB = 4, kTBI = 16, kTBE = kTBI + 2 * kTB;   // steps, tile, tile + halo
-constexpr uint32_t kTBSkip = 0x7fffffffu;                    // cell not updated (solid / outside)
+// other boundary (lb_create: time_block).  One thread per tile cell; each
+// step reads the neighbours into registers, waits, then overwrites the
+// cell in place (one LDS copy of the tile).
+constexpr int kTBI = 16;                        // interior tile edge
+constexpr uint32_t kTBSkip = 0x7fffffffu;       // cell not updated (solid / outside)
+template <int TB> constexpr int tb_threads() { return (kTBI + 2 * TB) * (kTBI + 2 * TB); }
 
-// One thread per tile cell (576 = 9 wavefronts): each of the TB steps is one
-// pass of independent LDS loads, BGK and LDS stores between two barriers.
-constexpr int kTBThreads = kTBE * kTBE;
-__global__ __launch_bounds__(kTBThreads) void k_step_tb9(const StepArgs<StF32> *__restrict__ pa) {
+template <int TB>
+__global__ __launch_bounds__(tb_threads<TB>()) void k_step_tb9(const StepArgs<StF32> *__restrict__ pa) {
     typedef Vel<9> L;
+    constexpr int E = kTBI + 2 * TB;
     const StepArgs<StF32> &a = *pa;
-    __shared__ float fs[2][9][kTBE * kTBE];
-    __shared__ uint32_t bms[kTBE * kTBE];
+    __shared__ float fs[9][E * E];
+    __shared__ uint32_t bms[E * E];
     const int ntx = (a.nx + kTBI - 1) / kTBI;
-    const int x0 = (int)(blockIdx.x % ntx) * kTBI - kTB, y0 = (int)(blockIdx.x / ntx) * kTBI - kTB;
+    const int x0 = (int)(blockIdx.x % ntx) * kTBI - TB, y0 = (int)(blockIdx.x / ntx) * kTBI - TB;
     const int c = threadIdx.x;
-    const int lx = c % kTBE, ly = c / kTBE;
+    const int lx = c % E, ly = c / E;
     {
         int gx = x0 + lx, gy = y0 + ly;
         const bool in = axis_src(gx, a.nx, a.bc[0], a.bc[1]) && axis_src(gy, a.ny, a.bc[2], a.bc[3]);
@@ -86,22 +88,25 @@ __global__ __launch_bounds__(kTBThreads) void k_step_tb9(const StepArgs<StF32> *
             if (!(fl & F_SOLID)) code = bm;
             static_for<0, 9>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                fs[0][i][c] = ldm(&a.A[(size_t)i * a.ps + p.e]);
+                fs[i][c] = ldm(&a.A[(size_t)i * a.ps + p.e]);
             });
         }
         bms[c] = code;
     }
     __syncthreads();
     const uint32_t bm = bms[c];
-    for (int s = 1; s <= kTB; s++) {
-        const int sb = (s - 1) & 1, db = s & 1;
-        if (bm != kTBSkip && lx >= s && lx < kTBE - s && ly >= s && ly < kTBE - s) {
-            float f[9];
+    for (int s = 1; s <= TB; s++) {
+        const bool act = bm != kTBSkip && lx >= s && lx < E - s && ly >= s && ly < E - s;
+        float f[9];
+        if (act) {
             static_for<0, 9>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 constexpr int j = L::opp(i);
-                f[i] = ((bm >> i) & 1u) ? fs[sb][j][c] : fs[sb][i][c - L::cx(i) - L::cy(i) * kTBE];
+                f[i] = ((bm >> i) & 1u) ? fs[j][c] : fs[i][c - L::cx(i) - L::cy(i) * E];
             });
+        }
+        __syncthreads();
+        if (act) {
             if (bm >> 31) {
                 NodePos p{};
                 p.x = x0 + lx; p.y = y0 + ly; p.z = 0;   // inside the lattice: the cell is not skipped
@@ -122,18 +127,17 @@ __global__ __launch_bounds__(kTBThreads) void k_step_tb9(const StepArgs<StF32> *
             collide<9, float>(f, rho, ux, uy, uz, a.ti, a.rest_closure != 0);
             static_for<0, 9>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                fs[db][i][c] = f[i];
+                fs[i][c] = f[i];
             });
         }
         __syncthreads();
     }
-    constexpr int fb = kTB & 1;
     const int gx = x0 + lx, gy = y0 + ly;
-    if (lx >= kTB && lx < kTB + kTBI && ly >= kTB && ly < kTB + kTBI && gx < a.nx && gy < a.ny && bm != kTBSkip) {
+    if (lx >= TB && lx < TB + kTBI && ly >= TB && ly < TB + kTBI && gx < a.nx && gy < a.ny && bm != kTBSkip) {
         const uint32_t e = (uint32_t)gx + a.sy * (uint32_t)gy;
         static_for<0, 9>([&](auto I) {
             constexpr int i = decltype(I)::value;
-            stm(&a.B[(size_t)i * a.ps + e], fs[fb][i][c]);
+            stm(&a.B[(size_t)i * a.ps + e], fs[i][c]);
         });
     }
 }
@@ -493,13 +497,13 @@ int launch_bump(lb_ctx *c, uint32_t n) {
 }
 
 // Lattices the temporally blocked kernel reproduces exactly (see k_step_tb9).
-bool tb_eligible(const lb_ctx *c) {
+bool tb_eligible(const lb_ctx *c, int tb) {
     const lb_config &g = c->cfg;
     if (c->Q != 9 || g.storage != LB_STORE_F32 || g.sc_G != 0.0f || g.zou_he || g.bb_full) return false;
     if (g.force[0] != 0.0f || g.force[1] != 0.0f || g.force[2] != 0.0f) return false;
-    // the halo of the last tile row / column reaches 18 nodes past the edge:
-    // one periodic wrap must bring every halo node back into the lattice
-    if (g.nx < kTBI + kTB || g.ny < kTBI + kTB) return false;
+    // the halo of the last tile row / column reaches tb + 14 nodes past the
+    // edge: one periodic wrap must bring every halo node back into the lattice
+    if (g.nx < kTBI + tb || g.ny < kTBI + tb) return false;
     for (int k = 0; k < 4; k++)
         if (g.bc[k] != LB_BC_PERIODIC && g.bc[k] != LB_BC_WALL && g.bc[k] != LB_BC_MOVING_WALL) return false;
     return true;
@@ -509,7 +513,8 @@ bool tb_eligible(const lb_ctx *c) {
 int launch_tb(lb_ctx *c, int src) {
     const StepArgs<StF32> *pa = (const StepArgs<StF32> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const unsigned nb = (unsigned)(((c->cfg.nx + kTBI - 1) / kTBI) * ((c->cfg.ny + kTBI - 1) / kTBI));
-    hipLaunchKernelGGL(k_step_tb9, dim3(nb), dim3(kTBThreads), 0, c->stream, pa);
+    if (c->tb == 8) hipLaunchKernelGGL(k_step_tb9<8>, dim3(nb), dim3(tb_threads<8>()), 0, c->stream, pa);
+    else hipLaunchKernelGGL(k_step_tb9<4>, dim3(nb), dim3(tb_threads<4>()), 0, c->stream, pa);
     return (int)hipGetLastError();
 }
 
@@ -706,7 +711,8 @@ int tune_tb(lb_ctx *c) {
     if (hipEventCreate(&e0)) return LB_ERR_INVALID;
     if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
     int rc = 0;
-    for (int tb : {0, kTB}) {
+    for (int tb : {0, 4, 8}) {
+        if (tb && !tb_eligible(c, tb)) continue;
         c->tb = tb;
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
@@ -752,7 +758,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
     if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1 ||
-        cfg->node_words > 1 || (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != kTB))
+        cfg->node_words > 1 ||
+        (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != 4 && cfg->time_block != 8))
         return LB_ERR_INVALID;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
@@ -803,10 +810,10 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             rc = autotune(c);
         }
     }
-    if (!rc && cfg->time_block == kTB && !tb_eligible(c)) rc = LB_ERR_UNSUPPORTED;
+    if (!rc && cfg->time_block > 0 && !tb_eligible(c, cfg->time_block)) rc = LB_ERR_UNSUPPORTED;
     if (!rc) {
-        if (cfg->time_block == kTB) c->tb = kTB;
-        else if (cfg->time_block < 0 && tb_eligible(c)) rc = tune_tb(c);
+        if (cfg->time_block > 0) c->tb = cfg->time_block;
+        else if (cfg->time_block < 0 && tb_eligible(c, 4)) rc = tune_tb(c);
     }
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     if (rc) { delete c; return rc; }

@@ -202,18 +202,20 @@ def _tb_scene(kind):
     return cfg, solid, dict(rho=1.0, ux=0.05, uy=0.01, uz=0.0)
 
 
+@pytest.mark.parametrize("tb", [4, 8])
 @pytest.mark.parametrize("words", [0, 1])
 @pytest.mark.parametrize("layout", [0, 1])
 @pytest.mark.parametrize("kind,steps", [("cavity64", 1003), ("ragged50x37", 501), ("periodic40x24", 402)])
-def test_time_blocked_steps_match_oracle(gpu, kind, steps, layout, words):
-    """Four steps per launch through the LDS tile: the same populations as
-    the oracle's single steps, bit for bit (walls, moving lid, obstacles,
-    periodic faces, ragged tiles, step counts not divisible by 4)."""
+def test_time_blocked_steps_match_oracle(gpu, kind, steps, layout, words, tb):
+    """Four or eight steps per launch through the LDS tile: the same
+    populations as the oracle's single steps, bit for bit (walls, moving lid,
+    obstacles, periodic faces, ragged tiles, step counts not divisible by the
+    block)."""
     cfg, solid, init = _tb_scene(kind)
-    cfg["layout"], cfg["node_words"] = layout, words
-    print("case", kind, layout, words, flush=True)
+    cfg["layout"], cfg["node_words"], cfg["time_block"] = layout, words, tb
+    print("case", kind, layout, words, tb, flush=True)
     sol, orc = _run_pair(gpu, cfg, solid, init, steps)
-    assert sol.launch_config["time_block"] == 4
+    assert sol.launch_config["time_block"] == tb
     fluid = solid.reshape(orc.shape) == 0
     _assert_pop_parity(sol.populations(), orc.A, cfg["storage"], fluid)
     m, mo = sol.macros(), orc.macros()
@@ -229,6 +231,9 @@ def test_time_block_rejected_where_it_cannot_apply(gpu):
     with pytest.raises(LatticeError):
         gpu.Solver(cfg, solid)
     cfg, solid, init = scenes.cavity(n=32, storage="f16", time_block=4)
+    with pytest.raises(LatticeError):
+        gpu.Solver(cfg, solid)
+    cfg, solid, init = scenes.cavity(n=20, time_block=8)     # smaller than 16 + 8
     with pytest.raises(LatticeError):
         gpu.Solver(cfg, solid)
 
