@@ -120,7 +120,7 @@ def bench_cavity(args):
                   round(ms, 5), "fp32",
                   {"workload": "D2Q9 BGK lid-driven cavity 256x256, tau=0.6, lid u=0.1, half-way bounce-back",
                    "launch": sol.launch_config},
-                  roofline={"bound": "latency (launch + one HBM round trip per launch)",
+                  roofline={"bound": "hbm", "note": "latency-bound: launch + one HBM round trip per launch",
                             "achieved": round(72 * n / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS,
                             "unit": "GB/s", "frac": round(72 * n / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
                             "traffic": None,
