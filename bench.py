@@ -69,12 +69,12 @@ def cpu_baseline(cfg, solid, init, steps_hint):
     O.set_threads(threads)
     orc = O.PullOracle(cfg, solid)
     orc.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
-    orc.run(1)   # warm
+    orc.run(3)   # warm (threads, pages)
     steps = steps_hint
     if steps <= 0:
         t0 = time.perf_counter()
-        orc.run(2)
-        per = (time.perf_counter() - t0) / 2
+        orc.run(8)
+        per = (time.perf_counter() - t0) / 8
         steps = max(2, min(2000, int(15.0 / max(per, 1e-6))))   # about 15 s of CPU work
     t0 = time.perf_counter()
     orc.run(steps)
