@@ -660,7 +660,8 @@ int autotune(lb_ctx *c) {
                     if (c->cfg.layout >= 0 && lay != c->cfg.layout) continue;
                     for (int pk = 0; pk < 2; pk++) {
                         // packed node words only pay on small, latency-bound lattices
-                        if (c->cfg.node_words >= 0 ? pk != c->cfg.node_words : (pk && c->N > (1u << 22))) continue;
+                        // (on HBM-bound ones their 3 extra bytes per node cost ~2 %)
+                        if (c->cfg.node_words >= 0 ? pk != c->cfg.node_words : (pk && c->N > (1u << 20))) continue;
                         cands.push_back({bx, kThreads / bx, xcd, mem, lay, pk});
                     }
                 }
@@ -673,7 +674,7 @@ int autotune(lb_ctx *c) {
     if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
     float best = 1e30f;
     Cand bestc = cands[0];
-    const int reps = 10;
+    const int reps = 20;
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
         c->packed = k.packed;
