@@ -158,6 +158,164 @@ __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// 3x3 convolution through LDS tiles (W % 32 == 0, Ci % 32 == 0,
+// Co % (32 NB) == 0, H % (4 PM) == 0).  A workgroup computes 4 PM image rows
+// x 32 pixels x 32 NB output channels: wavefront w owns rows w PM .. w PM +
+// PM - 1 of the tile, a 32-pixel row segment per MFMA tile.  Input channels
+// go in stages of 32: the stage's input window (4 PM + 2 rows x 34 pixels,
+// zero outside the image) and weight block (32 NB channels x 9 taps x 32)
+// are written to LDS once, converted to BF16 on the way (FP32 input of the
+// input-gradient form), and every tap reads its A operand from the shifted
+// window, so an input element leaves HBM/L2 about (4 PM + 2) 34 / (4 PM 32)
+// times instead of 9 times.  The next stage's global loads are issued into
+// registers before the current stage's MFMAs.  K order: stage, tap, 16-channel
+// half (FP32 accumulation).
+// ---------------------------------------------------------------------------
+constexpr int kTileCK = 32;                  // input channels per stage
+constexpr int kTilePS = kTileCK + 8;         // LDS stride of one window pixel (bf16)
+constexpr int kTileWS = 9 * kTileCK + 8;     // LDS stride of one output channel's weights (bf16)
+
+// Eight consecutive channels held in registers between their global load and
+// their LDS store: BF16 as loaded, FP32 converted (nearest even) only at the
+// store, so the loads of the next stage stay in flight during the MFMAs.
+template <class T> struct Piece8;
+template <> struct Piece8<bf16> {
+    u16x8 v;
+    __device__ __forceinline__ void zero() { v = u16x8{0, 0, 0, 0, 0, 0, 0, 0}; }
+    __device__ __forceinline__ void load(const bf16 *p) { v = *(const u16x8 *)p; }
+    __device__ __forceinline__ u16x8 bits() const { return v; }
+};
+template <> struct Piece8<float> {
+    float4 a, b;
+    __device__ __forceinline__ void zero() { a = b = float4{0.f, 0.f, 0.f, 0.f}; }
+    __device__ __forceinline__ void load(const float *p) { a = *(const float4 *)p; b = *(const float4 *)(p + 4); }
+    __device__ __forceinline__ u16x8 bits() const {
+        u16x8 r;
+        r[0] = lb::f32_to_bf16(a.x); r[1] = lb::f32_to_bf16(a.y); r[2] = lb::f32_to_bf16(a.z); r[3] = lb::f32_to_bf16(a.w);
+        r[4] = lb::f32_to_bf16(b.x); r[5] = lb::f32_to_bf16(b.y); r[6] = lb::f32_to_bf16(b.z); r[7] = lb::f32_to_bf16(b.w);
+        return r;
+    }
+};
+
+template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU>
+__global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in, const bf16 *__restrict__ W,
+                                                      const float *__restrict__ bias, TOut *__restrict__ out, int H,
+                                                      int Wd, int Ci, int Co) {
+    constexpr int ROWS = 4 * PM, IR = ROWS + 2, IC = 34;
+    constexpr int NCO = 32 * NB;
+    constexpr int NIN = IR * IC * 4;             // 16-byte pieces of the input window
+    constexpr int NW = NCO * 9 * 4;              // 16-byte pieces of the weight block
+    constexpr int RIN = (NIN + 255) / 256, RW = (NW + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[IR * IC * kTilePS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[NCO * kTileWS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, kh = (lane >> 5) * 8;
+    const int segs = Wd >> 5, rblocks = H / ROWS;
+    int t = blockIdx.x;
+    const int seg = t % segs;
+    t /= segs;
+    const int rb = t % rblocks, img = t / rblocks;
+    const int x0 = seg * 32, y0 = rb * ROWS;
+    const int n0 = blockIdx.y * NCO;
+    const int nstage = Ci / kTileCK;
+
+    // register prefetch of one stage
+    Piece8<TIn> pin[RIN];
+    Piece8<bf16> pw[RW];
+    auto fetch = [&](int c0) {
+#pragma unroll
+        for (int j = 0; j < RIN; j++) {
+            const int e = threadIdx.x + 256 * j;
+            pin[j].zero();
+            if (e < NIN) {
+                const int q = e & 3, p = e >> 2;
+                const int iy = p / IC, ix = p - iy * IC;
+                const int yy = y0 + iy - 1, xx = x0 + ix - 1;
+                if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
+                    pin[j].load(in + (((size_t)img * H + yy) * Wd + xx) * Ci + c0 + q * 8);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = threadIdx.x + 256 * j;
+            pw[j].zero();
+            if (e < NW) {
+                const int q = e & 3, r = e >> 2;
+                const int co = r / 9, tap = r - co * 9;
+                pw[j].load(W + ((size_t)(n0 + co) * 9 + tap) * Ci + c0 + q * 8);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < RIN; j++) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < NIN) *(u16x8 *)&s_in[(e >> 2) * kTilePS + (e & 3) * 8] = pin[j].bits();
+        }
+#pragma unroll
+        for (int j = 0; j < RW; j++) {
+            const int e = threadIdx.x + 256 * j;
+            if (e < NW) {
+                const int q = e & 3, r = e >> 2;
+                const int co = r / 9, tap = r - co * 9;
+                *(u16x8 *)&s_w[co * kTileWS + tap * kTileCK + q * 8] = pw[j].bits();
+            }
+        }
+    };
+
+    f32x16 acc[PM][NB];
+#pragma unroll
+    for (int j = 0; j < PM; j++)
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+            for (int r = 0; r < 16; r++) acc[j][b][r] = 0.f;
+
+    fetch(0);
+    for (int s = 0; s < nstage; s++) {
+        __syncthreads();                 // every wavefront is done with the previous stage
+        store();
+        __syncthreads();
+        if (s + 1 < nstage) fetch((s + 1) * kTileCK);
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                bf16x8 a[PM], bw[NB];
+#pragma unroll
+                for (int j = 0; j < PM; j++)
+                    a[j] = __builtin_bit_cast(
+                        bf16x8, *(const u16x8 *)&s_in[((wave * PM + j + dy) * IC + row + dx) * kTilePS + kk * 16 + kh]);
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+                    bw[b] = __builtin_bit_cast(
+                        bf16x8, *(const u16x8 *)&s_w[(b * 32 + row) * kTileWS + tap * kTileCK + kk * 16 + kh]);
+#pragma unroll
+                for (int j = 0; j < PM; j++)
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+                        acc[j][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[j], bw[b], acc[j][b], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PM; j++) {
+        const size_t pix0 = ((size_t)img * H + y0 + wave * PM + j) * Wd + x0;
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            const int co = n0 + 32 * b + row;
+            const float bv = BIAS_RELU ? bias[co] : 0.f;
+            for (int r = 0; r < 16; r++) {
+                const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                float v = acc[j][b][r];
+                if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
+                out[(pix0 + i) * Co + co] = from_f<TOut>(v);
+            }
+        }
+    }
+}
+
 // Weight gradient partials: block (co tile, k tile, pixel chunk); each of the
 // 4 wavefronts reduces a quarter of the chunk, the block sums them in LDS.
 template <class TIn>
@@ -638,9 +796,33 @@ void sum_chunks(lb_unet *u, const float *partial, int nchunk, size_t n, float *o
     hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, u->partial2, G, n, out);
 }
 
+// LDS-tile launch for eligible shapes; returns false (nothing launched) when
+// the shape does not fit the tile kernel.  Tile choice: the largest tile
+// (4 PM rows x 32 pixels x 32 NB channels) that still gives at least two
+// workgroups per CU, else the smallest.
+template <class TIn, class TOut, bool BR>
+bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
+                      int Ci, int Co) {
+    if (Wd % 32 || Ci % kTileCK || Co % 32 || H % 4) return false;
+    const int B = npix / (H * Wd);
+    const int NB = Co % 64 == 0 ? 2 : 1;
+    const int PM = (H % 8 == 0 && (size_t)B * (H / 8) * (Wd / 32) * (Co / (32 * NB)) >= 512) ? 2 : 1;
+    dim3 grd((unsigned)(B * (H / (4 * PM)) * (Wd / 32)), (unsigned)(Co / (32 * NB))), blk(256);
+    if (NB == 2 && PM == 2)
+        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
+    else if (NB == 2)
+        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
+    else if (PM == 2)
+        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
+    else
+        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
+    return true;
+}
+
 template <class TIn, class TOut, bool BR>
 void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
-                 int Ci, int Co) {
+                 int Ci, int Co, bool gather_only = false) {
+    if (!gather_only && conv_tile_launch<TIn, TOut, BR>(u, in, W, bias, out, npix, H, Wd, Ci, Co)) return;
     dim3 blk(256);
     const int nb = Co % 64 == 0 ? 2 : 1;
     // fewer than ~4 workgroups per CU with one 32-pixel tile per wavefront:
@@ -1032,15 +1214,22 @@ extern "C" int lb_unet_sync(lb_unet *u) {
 
 // Building block exposed for testing and reuse: one 3x3 convolution on
 // device tensors (NHWC).  mode 0: bf16 in -> bf16 out with bias + ReLU;
-// mode 1: fp32 in -> fp32 out, no bias / activation.
+// mode 1: fp32 in -> fp32 out, no bias / activation; modes 2 / 3: the same
+// forms on the gathering kernel only (the LDS-tile kernel is otherwise taken
+// for every shape it supports).
 extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B,
                                int H, int W, int Ci, int Co, lb_stream_t stream) {
-    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1) return LB_ERR_INVALID;
+    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || mode < 0 || mode > 3) return LB_ERR_INVALID;
     lb_unet tmp;
     tmp.stream = (hipStream_t)stream;
     const int npix = B * H * W;
-    if (mode == 0) conv_launch<bf16, bf16, true>(&tmp, (const bf16 *)d_in, (const bf16 *)d_w, d_bias, (bf16 *)d_out, npix, H, W, Ci, Co);
-    else conv_launch<float, float, false>(&tmp, (const float *)d_in, (const bf16 *)d_w, nullptr, (float *)d_out, npix, H, W, Ci, Co);
+    const bool gather = mode >= 2;
+    if (mode % 2 == 0)
+        conv_launch<bf16, bf16, true>(&tmp, (const bf16 *)d_in, (const bf16 *)d_w, d_bias, (bf16 *)d_out, npix, H, W, Ci,
+                                      Co, gather);
+    else
+        conv_launch<float, float, false>(&tmp, (const float *)d_in, (const bf16 *)d_w, nullptr, (float *)d_out, npix, H, W,
+                                         Ci, Co, gather);
     tmp.stream = nullptr;
     return (int)hipGetLastError();
 }

@@ -79,12 +79,19 @@ def _bf16_round(a):
     return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
 
 
+# (ci, co, B, H, W): W = 16 runs the gathering kernel only; W % 32 == 0 with
+# ci % 32 == 0 takes the LDS-tile kernel in modes 0 / 1 (all four tile shapes:
+# 4 or 8 rows x 32 or 64 channels), compared with the gathering kernel too.
+CONV_CASES = [(16, 32, 2, 16, 16), (32, 64, 2, 16, 16), (96, 32, 2, 16, 16), (384, 128, 2, 16, 16),
+              (32, 64, 2, 8, 64), (96, 32, 1, 16, 32), (384, 128, 1, 4, 32), (64, 64, 3, 12, 32),
+              (192, 64, 16, 32, 32), (32, 32, 4, 128, 128)]
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("ci,co", [(16, 32), (32, 64), (96, 32), (384, 128)])
-def test_gpu_conv3x3_matches_reference(gpu, ci, co):
+@pytest.mark.parametrize("ci,co,B,H,W", CONV_CASES)
+def test_gpu_conv3x3_matches_reference(gpu, ci, co, B, H, W):
     from lattibol_amd import _lib
-    rng = np.random.default_rng(ci * 1000 + co)
-    B, H, W = 2, 16, 16
+    rng = np.random.default_rng(ci * 1000 + co + H * W)
     x = _bf16_round(rng.standard_normal((B, H, W, ci)))
     w = _bf16_round(rng.standard_normal((co, 9, ci)) * 0.1)
     bias = rng.standard_normal(co).astype(np.float32) * 0.1
@@ -92,22 +99,30 @@ def test_gpu_conv3x3_matches_reference(gpu, ci, co):
     xd = _lib.DeviceArray.from_numpy(to_u16(x))
     wd = _lib.DeviceArray.from_numpy(to_u16(w))
     bd = _lib.DeviceArray.from_numpy(bias)
-    yd = _lib.DeviceArray((B, H, W, co), np.int16)
-    _lib.check(_lib.lib.lb_conv3x3_bf16(0, xd.ptr, wd.ptr, bd.ptr, yd.ptr, B, H, W, ci, co, None), "conv")
-    _lib.check(_lib.lib.lb_device_synchronize(), "sync")
-    y = torch.from_numpy(yd.numpy()).view(torch.bfloat16).to(torch.float32).numpy()
     ref = UO.conv3x3_ref(x, w, bias, relu=True)
-    # one BF16 rounding of an FP32-accumulated value: <= 2^-8 relative
-    err = np.abs(y - ref)
-    assert (err <= 2.0 ** -8 * np.abs(ref) + 1e-6).all(), err.max()
-    # FP32 in / FP32 out (the backward-data form)
     xf = rng.standard_normal((B, H, W, ci)).astype(np.float32)
     xfd = _lib.DeviceArray.from_numpy(xf)
-    yfd = _lib.DeviceArray((B, H, W, co), np.float32)
-    _lib.check(_lib.lib.lb_conv3x3_bf16(1, xfd.ptr, wd.ptr, None, yfd.ptr, B, H, W, ci, co, None), "conv")
-    _lib.check(_lib.lib.lb_device_synchronize(), "sync")
-    ref = UO.conv3x3_ref(_bf16_round(xf), w)
-    np.testing.assert_allclose(yfd.numpy(), ref, rtol=1e-5, atol=1e-5)
+    reff = UO.conv3x3_ref(_bf16_round(xf), w)
+    outs = {}
+    for mode in (0, 2):
+        yd = _lib.DeviceArray((B, H, W, co), np.int16)
+        _lib.check(_lib.lib.lb_conv3x3_bf16(mode, xd.ptr, wd.ptr, bd.ptr, yd.ptr, B, H, W, ci, co, None), "conv")
+        _lib.check(_lib.lib.lb_device_synchronize(), "sync")
+        y = torch.from_numpy(yd.numpy()).view(torch.bfloat16).to(torch.float32).numpy()
+        # one BF16 rounding of an FP32-accumulated value: <= 2^-8 relative
+        err = np.abs(y - ref)
+        assert (err <= 2.0 ** -8 * np.abs(ref) + 1e-6).all(), (mode, err.max())
+        outs[mode] = y
+    # FP32 in / FP32 out (the backward-data form)
+    for mode in (1, 3):
+        yfd = _lib.DeviceArray((B, H, W, co), np.float32)
+        _lib.check(_lib.lib.lb_conv3x3_bf16(mode, xfd.ptr, wd.ptr, None, yfd.ptr, B, H, W, ci, co, None), "conv")
+        _lib.check(_lib.lib.lb_device_synchronize(), "sync")
+        np.testing.assert_allclose(yfd.numpy(), reff, rtol=1e-5, atol=1e-5, err_msg="mode %d" % mode)
+        outs[mode] = yfd.numpy()
+    # the two kernels differ only in FP32 summation order
+    np.testing.assert_allclose(outs[1], outs[3], rtol=1e-5, atol=1e-5)
+    assert np.abs(outs[0] - outs[2]).max() <= 2.0 ** -7 * np.abs(ref).max()
 
 
 @pytest.mark.gpu
