@@ -19,6 +19,7 @@
 //   head: conv1x1(C -> out_ch)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -475,6 +476,152 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_cm(const bf16 *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Weight gradient through LDS tiles (W % 32 == 0, H % 8 == 0, Co % 32 == 0,
+// Ci % 8 == 0): block (32 output channels, 32 input channels, chunk of pixel
+// tiles).  A pixel tile is 8 image rows x 32 pixels.  Per tile the gradient
+// block (256 pixels x 32 channels, FP32 -> BF16) and the input window (10
+// rows x 36 pixels x 32 channels, zero outside the image) are written to LDS
+// channel-major, four pixels per 8-byte store, so that an MFMA operand (8
+// consecutive pixels of one channel) is one 16-byte LDS read.  Wavefront w
+// takes image rows 2w, 2w + 1 of the tile (four 16-pixel K-steps) and keeps
+// all nine taps' 32 x 32 accumulators; the column shift of a tap is a funnel
+// shift across an aligned 16-byte and 4-byte read of the window row.  The
+// next tile's global loads are issued into registers before the MFMAs of
+// the current one.  The four wavefronts' tiles are added in LDS in a fixed
+// order and written as the chunk's partial (partial[chunk][co][tap][ci]).
+// ---------------------------------------------------------------------------
+constexpr int kWgRows = 8;      // image rows per pixel tile
+constexpr int kWgWin = 40;      // LDS stride of a window row (36 pixels used)
+constexpr int kWgXS = 520;      // LDS stride of one input channel (10 rows; 260 dwords = 4 mod 64 banks)
+constexpr int kWgGS = 264;      // LDS stride of one output channel (256 pixels; 132 dwords = 4 mod 64 banks)
+
+__device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
+    return uint2{(uint32_t)a | ((uint32_t)b << 16), (uint32_t)c | ((uint32_t)d << 16)};
+}
+__device__ __forceinline__ float lane_of(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restrict__ dout, const bf16 *__restrict__ in,
+                                                            float *__restrict__ partial, int H, int Wd, int Ci, int Co,
+                                                            int ntiles, int tpw) {
+    __shared__ __attribute__((aligned(16))) uint16_t sX[32 * kWgXS];
+    __shared__ __attribute__((aligned(16))) uint16_t sG[32 * kWgGS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, kh = (lane >> 5) * 8;
+    const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
+    const int K = 9 * Ci;
+    const int segs = Wd >> 5, rblocks = H / kWgRows;
+    const int t0 = blockIdx.z * tpw, t1 = min(ntiles, t0 + tpw);
+    constexpr int NXQ = (kWgRows + 2) * 9 * 4;   // window: rows x nine 4-pixel groups x four 8-channel groups
+    constexpr int RX = (NXQ + 255) / 256;
+    u16x8 xq[RX][4];
+    float4 gq[4][2];
+    auto fetch = [&](int tile) {
+        const int seg = tile % segs, rest = tile / segs;
+        const int y0 = (rest % rblocks) * kWgRows, img = rest / rblocks, x0 = seg * 32;
+#pragma unroll
+        for (int j = 0; j < RX; j++) {
+            const int e = tid + 256 * j;
+#pragma unroll
+            for (int p = 0; p < 4; p++) xq[j][p] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            if (e < NXQ) {
+                const int q = e & 3, r = e >> 2;
+                const int iy = r / 9, qx = r - iy * 9;
+                const int yy = y0 + iy - 1, c = ci0 + 8 * q;
+                if (yy >= 0 && yy < H && c < Ci) {
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        const int xx = x0 + 4 * qx + p - 1;
+                        if (xx >= 0 && xx < Wd) xq[j][p] = *(const u16x8 *)(in + (((size_t)img * H + yy) * Wd + xx) * Ci + c);
+                    }
+                }
+            }
+        }
+        {   // gradient: 8 rows x eight 4-pixel groups x four 8-channel groups = one piece per thread
+            const int q = tid & 3, r = tid >> 2;
+            const int iy = r >> 3, qx = r & 7;
+            const float *g = dout + (((size_t)img * H + y0 + iy) * Wd + x0 + 4 * qx) * Co + co0 + 8 * q;
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                gq[p][0] = *(const float4 *)(g + (size_t)p * Co);
+                gq[p][1] = *(const float4 *)(g + (size_t)p * Co + 4);
+            }
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int j = 0; j < RX; j++) {
+            const int e = tid + 256 * j;
+            if (e < NXQ) {
+                const int q = e & 3, r = e >> 2;
+                const int iy = r / 9, qx = r - iy * 9;
+                const int base = iy * kWgWin + 4 * qx;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    *(uint2 *)&sX[(8 * q + k) * kWgXS + base] = pack4(xq[j][0][k], xq[j][1][k], xq[j][2][k], xq[j][3][k]);
+            }
+        }
+        {
+            const int q = tid & 3, r = tid >> 2;
+            const int base = (r >> 3) * 32 + 4 * (r & 7);
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int h = k >> 2, c = k & 3;
+                *(uint2 *)&sG[(8 * q + k) * kWgGS + base] =
+                    pack4(lb::f32_to_bf16(lane_of(gq[0][h], c)), lb::f32_to_bf16(lane_of(gq[1][h], c)),
+                          lb::f32_to_bf16(lane_of(gq[2][h], c)), lb::f32_to_bf16(lane_of(gq[3][h], c)));
+            }
+        }
+    };
+
+    f32x16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+        for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
+    if (t0 < t1) fetch(t0);
+    for (int t = t0; t < t1; t++) {
+        __syncthreads();                 // every wavefront is done with the previous tile
+        store();
+        __syncthreads();
+        if (t + 1 < t1) fetch(t + 1);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const int trow = 2 * wave + (s >> 1), pxb = (s & 1) * 16 + kh;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sG[col * kWgGS + trow * 32 + pxb]);
+#pragma unroll
+            for (int dy = 0; dy < 3; dy++) {
+                const uint16_t *xp = &sX[col * kWgXS + (trow + dy) * kWgWin + pxb];
+                const u32x4 d = *(const u32x4 *)xp;
+                const uint32_t d4 = *(const uint32_t *)(xp + 8);
+                const u32x4 b1 = u32x4{__builtin_amdgcn_alignbit(d[1], d[0], 16), __builtin_amdgcn_alignbit(d[2], d[1], 16),
+                                       __builtin_amdgcn_alignbit(d[3], d[2], 16), __builtin_amdgcn_alignbit(d4, d[3], 16)};
+                const u32x4 b2 = u32x4{d[1], d[2], d[3], d4};
+                acc[3 * dy] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, d), acc[3 * dy], 0, 0, 0);
+                acc[3 * dy + 1] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, b1), acc[3 * dy + 1], 0, 0, 0);
+                acc[3 * dy + 2] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, __builtin_bit_cast(bf16x8, b2), acc[3 * dy + 2], 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();
+    float *red = (float *)sX;            // 4 x 32 x 33 floats
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) {
+        for (int r = 0; r < 16; r++) {
+            const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            red[(wave * 32 + i) * 33 + col] = acc[tap][r];
+        }
+        __syncthreads();
+        for (int e = tid; e < 32 * 32; e += 256) {
+            const int i = e >> 5, j = e & 31;
+            const float sum = ((red[i * 33 + j] + red[(32 + i) * 33 + j]) + red[(64 + i) * 33 + j]) + red[(96 + i) * 33 + j];
+            if (ci0 + j < Ci) partial[((size_t)blockIdx.z * Co + co0 + i) * K + tap * Ci + ci0 + j] = sum;
+        }
+        __syncthreads();
+    }
+}
+
 // Sum of partials over chunks (fixed order) -> gradient.
 __global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *out) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -845,18 +992,40 @@ void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TO
     }
 }
 
+// Pixel-tile chunks of the LDS-tile weight gradient: tiles per workgroup
+// chosen for about 768 workgroups; 0 when the shape does not fit the kernel.
+struct WgradTiles { int ntiles, tpw, nchunk; };
+inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
+    if (Wd % 32 || H % kWgRows || Co % 32 || Ci % 8) return {0, 0, 0};
+    const int ntiles = npix / (kWgRows * 32);
+    const long base = (long)(Co / 32) * ((Ci + 31) / 32);
+    const int tpw = (int)std::max(1L, (ntiles * base + 767) / 768);
+    return {ntiles, tpw, (ntiles + tpw - 1) / tpw};
+}
+// Chunk count of the per-wavefront weight-gradient kernels (1,024 pixels each).
+inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
+
 // Weight gradient dW[co][tap][ci] from dout (FP32 NHWC) and the conv input
-// (BF16 NHWC).  With W % 16 == 0 and channel-major scratch (tA >= npix Co,
-// tB >= npix Ci elements) both operands are transposed first and the
-// vector-load kernel runs; otherwise the gathering kernel.
+// (BF16 NHWC).  Shapes the LDS-tile kernel supports take it (method < 0 or
+// 2); otherwise, with W % 16 == 0 and channel-major scratch (tA >= npix Co,
+// tB >= npix Ci elements), both operands are transposed first and the
+// vector-load kernel runs (method 1); otherwise the gathering kernel
+// (method 0).
 template <class TIn>
 void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
-                  bf16 *tA, bf16 *tB) {
+                  bf16 *tA, bf16 *tB, int method = -1) {
     const int K = 9 * Ci;
+    const WgradTiles wt = wgrad_tiles(npix, H, Wd, Ci, Co);
+    if ((method < 0 || method == 2) && wt.ntiles) {
+        hipLaunchKernelGGL(k_conv3x3_wgrad_tile, dim3(Co / 32, (Ci + 31) / 32, wt.nchunk), dim3(256), 0, u->stream, dout,
+                           (const bf16 *)in, u->partial, H, Wd, Ci, Co, wt.ntiles, wt.tpw);
+        sum_chunks(u, u->partial, wt.nchunk, (size_t)Co * K, dW);
+        return;
+    }
     const int chunk = 1024;
     const int nchunk = (npix + chunk - 1) / chunk;
     dim3 grd(Co / 32, (K + 31) / 32, nchunk);
-    if (tA && tB && Wd % 16 == 0) {
+    if (method != 0 && tA && tB && Wd % 16 == 0) {
         hipLaunchKernelGGL(k_to_cmajor<float>, dim3((npix + 63) / 64, (Co + 63) / 64), dim3(256), 0, u->stream, dout, tA,
                            npix, Co);
         hipLaunchKernelGGL(k_to_cmajor<TIn>, dim3((npix + 63) / 64, (Ci + 63) / 64), dim3(256), 0, u->stream, in, tB,
@@ -977,7 +1146,8 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         else if (ci < (size_t)2 * Lv) l = Lv - 1;
         else l = Lv - 2 - (int)(ci - 2 * Lv) / 2;
         size_t npix = (size_t)B * L[l].H * L[l].W;
-        size_t nchunk = (npix + 1023) / 1024;
+        const WgradTiles wt = wgrad_tiles((int)npix, L[l].H, L[l].W, u->convs[ci].ci, u->convs[ci].co);
+        size_t nchunk = std::max((size_t)wgrad_chunks_legacy((int)npix), (size_t)wt.nchunk);
         const size_t nw = (size_t)u->convs[ci].co * 9 * u->convs[ci].ci;
         u->partial_n = std::max(u->partial_n, nchunk * nw);
         // group sums of the two-level reductions (weights, bias)
@@ -1236,13 +1406,15 @@ extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, cons
 
 // Weight gradient of one 3x3 convolution (test and diagnostic entry point;
 // allocates its scratch).  method 0: gathering kernel, 1: channel-major
-// operands (needs W % 16 == 0).
+// operands (needs W % 16 == 0), 2: LDS tiles (needs W % 32 == 0, H % 8 == 0).
 extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,
                                      int Ci, int Co, lb_stream_t stream) {
-    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || method < 0 || method > 1) return LB_ERR_INVALID;
+    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || method < 0 || method > 2) return LB_ERR_INVALID;
     if (method == 1 && W % 16) return LB_ERR_UNSUPPORTED;
     const int npix = B * H * W;
-    const size_t nchunk = ((size_t)npix + 1023) / 1024;
+    const WgradTiles wt = wgrad_tiles(npix, H, W, Ci, Co);
+    if (method == 2 && !wt.ntiles) return LB_ERR_UNSUPPORTED;
+    const size_t nchunk = method == 2 ? (size_t)wt.nchunk : (size_t)wgrad_chunks_legacy(npix);
     lb_unet tmp;
     tmp.stream = (hipStream_t)stream;
     tmp.partial = tmp.alloc<float>(nchunk * Co * 9 * Ci);
@@ -1252,7 +1424,7 @@ extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void
     int rc = 0;
     if (!tmp.partial || !tmp.partial2 || (method && (!tA || !tB))) rc = LB_ERR_INVALID;
     if (!rc) {
-        wgrad_launch<bf16>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB);
+        wgrad_launch<bf16>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB, method);
         rc = (int)hipGetLastError();
     }
     if (!rc) rc = (int)hipStreamSynchronize(tmp.stream);
