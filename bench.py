@@ -293,7 +293,10 @@ def unet_flops(levels=4, base=32, H=128, W=128, B=16, out_ch=3):
 
 def bench_unet(args):
     """U-Net training iterations (batch 16 of 128x128, synthetic inputs):
-    forward + backward + Adam, all on the BF16 MFMA kernels."""
+    forward + backward + Adam, all on the BF16 MFMA kernels.  The loss is
+    computed on the device every iteration and read back after the timed
+    loop (a per-iteration read-back adds one host synchronisation, reported
+    as ms_per_step_loss_readback)."""
     import numpy as np
     from lattibol_amd.unet import UNet
     from lattibol_amd._lib import DeviceArray
@@ -309,9 +312,14 @@ def bench_unet(args):
     steps = max(1, args.steps // 100)
     t0 = time.perf_counter()
     for _ in range(steps):
-        net.forward(xd, 16); net.backward(td, md, 16); net.adam(1e-4)
+        net.forward(xd, 16); net.backward(td, md, 16, want_loss=False); net.adam(1e-4)
     net.sync()
     dt = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward(xd, 16); net.backward(td, md, 16); net.adam(1e-4)
+    net.sync()
+    dt_sync = (time.perf_counter() - t0) / steps
     t0 = time.perf_counter()
     for _ in range(steps):
         net.forward(xd, 16, yd)
@@ -323,6 +331,7 @@ def bench_unet(args):
                                                            "synthetic inputs; forward+backward+Adam"},
                   higher=False, train_tflops=round(ftrain / dt / 1e12, 2), train_gflop=round(ftrain / 1e9, 1),
                   forward_ms=round(dtf * 1e3, 3), forward_tflops=round(ffwd / dtf / 1e12, 2),
+                  ms_per_step_loss_readback=round(dt_sync * 1e3, 3),
                   mfma_peak_tflops=2500.0)
 
 

@@ -316,9 +316,10 @@ int64_t lb_batch_total_nodes(lb_batch *b);
 /*    No counterpart in the reference: the learned warm start.              */
 /* ---------------------------------------------------------------------- */
 typedef struct {
-    int levels;             /* resolution levels (4) */
-    int base;               /* channels at level 0 (32); doubles per level */
-    int in_ch, out_ch;      /* 3 -> 3 */
+    int levels;             /* resolution levels (4), 2..6 */
+    int base;               /* channels at level 0 (32); doubles per level;
+                               a multiple of 32, <= 256, base << (levels-1) <= 1024 */
+    int in_ch, out_ch;      /* 3 -> 3; in_ch <= 16, out_ch <= 8 */
     int height, width;      /* 128 x 128 */
     int max_batch;
 } lb_unet_desc;
@@ -337,7 +338,10 @@ int lb_unet_set_params(lb_unet *u, const float *host_params);
  * once y is written. */
 int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y);
 /* Masked MSE of the last forward vs d_t (B,H,W,out_ch) with weights d_mask
- * (B,H,W); fills every parameter gradient; *loss = mean squared error. */
+ * (B,H,W); fills every parameter gradient (asynchronous on the handle's
+ * stream; an all-zero mask gives zero gradients).  With loss non-NULL the
+ * call waits for the stream and sets *loss = mean squared error
+ * (LB_ERR_INVALID for an all-zero mask). */
 int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mask, int B, double *loss);
 int lb_unet_adam(lb_unet *u, float lr, float beta1, float beta2, float eps);
 int lb_unet_sync(lb_unet *u);

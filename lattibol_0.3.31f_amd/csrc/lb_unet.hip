@@ -631,139 +631,210 @@ __global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *
     out[e] = s;
 }
 
-// First pass of a two-level chunk sum: group g (blockIdx.y) adds chunks
-// [g S, min((g+1) S, nchunk)) in order into out[g][e].
-__global__ void k_sum_chunk_groups(const float *partial, int nchunk, int S, size_t n, float *out) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const int c0 = blockIdx.y * S, c1 = min(nchunk, c0 + S);
+// Fixed-order sum of many chunks in one pass: thread (element el of a
+// 16-element group, phase ph of 16) adds chunks ph, ph + 16, ph + 32, ... in
+// order; the 16 phase sums are then added in LDS in phase order.
+__global__ __launch_bounds__(256) void k_sum_chunks16(const float *__restrict__ partial, int nchunk, size_t n,
+                                                      float *__restrict__ out) {
+    const int el = threadIdx.x & 15, ph = threadIdx.x >> 4;
+    const size_t e = (size_t)blockIdx.x * 16 + el;
     float s = 0.f;
-    for (int c = c0; c < c1; c++) s += partial[(size_t)c * n + e];
-    out[(size_t)blockIdx.y * n + e] = s;
-}
-
-// Bias-gradient partials: db[co] = sum_m dout[m, co], block (32-channel
-// group, pixel chunk); 8 pixel lanes x 32 channels, fixed order, then
-// k_sum_chunks over the chunks.
-__global__ __launch_bounds__(256) void k_bias_grad(const float *dout, int npix, int Co, int chunk, float *partial) {
-    const int co = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int part = threadIdx.x >> 5;   // 8 pixel strides
-    const int pbeg = blockIdx.y * chunk, pend = min(npix, pbeg + chunk);
-    float s = 0.f;
-    if (co < Co)
-        for (int m = pbeg + part; m < pend; m += 8) s += dout[(size_t)m * Co + co];
-    __shared__ float sh[8][32];
-    sh[part][threadIdx.x & 31] = s;
+    if (e < n) {
+#pragma unroll 8
+        for (int c = ph; c < nchunk; c += 16) s += partial[(size_t)c * n + e];
+    }
+    __shared__ float sh[16][17];
+    sh[ph][el] = s;
     __syncthreads();
-    if (threadIdx.x < 32 && co < Co) {
-        float t = 0.f;
-        for (int p = 0; p < 8; p++) t += sh[p][threadIdx.x];
-        partial[(size_t)blockIdx.y * Co + co] = t;
+    if (threadIdx.x < 16 && e < n) {
+        float v = 0.f;
+        for (int p = 0; p < 16; p++) v += sh[p][el];
+        out[e] = v;
     }
 }
 
-// dpre = d * (act > 0): ReLU backward on the stored post-activation.
-__global__ void k_relu_bwd(const float *d, const bf16 *act, float *dpre, size_t n) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    dpre[e] = from_bf16(act[e]) > 0.f ? d[e] : 0.f;
+// Bias-gradient chunking of k_relu_bwd_bias: at most 512 chunks of >= 64 pixels.
+struct BiasChunks { int chunk, nchunk; };
+inline BiasChunks bias_chunks(int npix) {
+    const int chunk = npix > 64 * 512 ? (npix + 511) / 512 : 64;
+    return {chunk, (npix + chunk - 1) / chunk};
 }
 
+// ReLU backward on the stored post-activation, dpre = d * (act > 0), and the
+// bias-gradient partials of the same pass, partial[chunk][co] =
+// sum_{m in chunk} dpre[m, co].  Block = one pixel chunk x all Co channels
+// (Co % 4 == 0, Co <= 1024): thread (4-channel group, pixel phase) walks the
+// chunk with stride 256 / (Co / 4); the phases are added in LDS in a fixed
+// order.
+__global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__ d, const bf16 *__restrict__ act,
+                                                       float *__restrict__ dpre, int npix, int Co, int chunk,
+                                                       float *__restrict__ partial) {
+    const int tpp = Co >> 2, nph = 256 / tpp;
+    const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
+    const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (ph < nph) {
+        for (int m = pbeg + ph; m < pend; m += nph) {
+            const size_t o = (size_t)m * Co + 4 * cg;
+            const float4 g = *(const float4 *)(d + o);
+            const uint2 a = *(const uint2 *)(act + o);
+            float4 r;
+            r.x = lb::bf16_to_f32((uint16_t)(a.x & 0xffffu)) > 0.f ? g.x : 0.f;
+            r.y = lb::bf16_to_f32((uint16_t)(a.x >> 16)) > 0.f ? g.y : 0.f;
+            r.z = lb::bf16_to_f32((uint16_t)(a.y & 0xffffu)) > 0.f ? g.z : 0.f;
+            r.w = lb::bf16_to_f32((uint16_t)(a.y >> 16)) > 0.f ? g.w : 0.f;
+            *(float4 *)(dpre + o) = r;
+            s0 += r.x; s1 += r.y; s2 += r.z; s3 += r.w;
+        }
+    }
+    __shared__ float sh[1024];
+    if (ph < nph) {
+        float *q = sh + ph * Co + 4 * cg;
+        q[0] = s0; q[1] = s1; q[2] = s2; q[3] = s3;
+    }
+    __syncthreads();
+    if (t < Co) {
+        float v = 0.f;
+        for (int p = 0; p < nph; p++) v += sh[p * Co + t];
+        partial[(size_t)blockIdx.x * Co + t] = v;
+    }
+}
+
+// 2x2 average pooling, eight channels per thread (C % 8 == 0).
 __global__ void k_avgpool(const bf16 *in, bf16 *out, int B, int H, int Wd, int C) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int H2 = H / 2, W2 = Wd / 2;
-    size_t n = (size_t)B * H2 * W2 * C;
-    if (e >= n) return;
-    int c = e % C;
-    size_t p = e / C;
-    int x = p % W2, y = (p / W2) % H2, b = p / ((size_t)W2 * H2);
-    auto at = [&](int yy, int xx) { return from_bf16(in[(((size_t)b * H + yy) * Wd + xx) * C + c]); };
-    float s = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
-    out[e] = to_bf16(0.25f * s);
+    const unsigned C8 = C >> 3, H2 = H / 2, W2 = Wd / 2;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)B * H2 * W2 * C8) return;
+    const unsigned c = (e % C8) * 8, p = e / C8;
+    const unsigned x = p % W2, r = p / W2, y = r % H2, b = r / H2;
+    const bf16 *q = in + (((size_t)b * H + 2 * y) * Wd + 2 * x) * C + c;
+    const u16x8 v00 = *(const u16x8 *)q, v01 = *(const u16x8 *)(q + C);
+    const u16x8 v10 = *(const u16x8 *)(q + (size_t)Wd * C), v11 = *(const u16x8 *)(q + (size_t)Wd * C + C);
+    u16x8 o;
+    for (int k = 0; k < 8; k++) {
+        const float sum = (lb::bf16_to_f32(v00[k]) + lb::bf16_to_f32(v01[k])) +
+                          (lb::bf16_to_f32(v10[k]) + lb::bf16_to_f32(v11[k]));
+        o[k] = lb::f32_to_bf16(0.25f * sum);
+    }
+    *(u16x8 *)(out + (size_t)p * C + c) = o;
 }
 
-// d(in) (+)= d(out)/4 at each of the four children.
+// d(in) (+)= d(out)/4 at each of the four children, four channels per thread.
 __global__ void k_avgpool_bwd(const float *dout, float *din, int B, int H, int Wd, int C, int accumulate) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)B * H * Wd * C;
-    if (e >= n) return;
-    int c = e % C;
-    size_t p = e / C;
-    int x = p % Wd, y = (p / Wd) % H, b = p / ((size_t)Wd * H);
-    float g = 0.25f * dout[(((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * C + c];
-    din[e] = accumulate ? din[e] + g : g;
+    const unsigned C4 = C >> 2;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)B * H * Wd * C4) return;
+    const unsigned c = (e % C4) * 4, p = e / C4;
+    const unsigned x = p % Wd, r = p / Wd, y = r % H, b = r / H;
+    const float4 s = *(const float4 *)(dout + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * C + c);
+    const float4 g = float4{0.25f * s.x, 0.25f * s.y, 0.25f * s.z, 0.25f * s.w};
+    float4 *o = (float4 *)(din + (size_t)p * C + c);
+    if (accumulate) {
+        const float4 v = *o;
+        *o = float4{v.x + g.x, v.y + g.y, v.z + g.z, v.w + g.w};
+    } else {
+        *o = g;
+    }
 }
 
-// out[b, y, x, :] = [low[b, y/2, x/2, 0:Cl] | skip[b, y, x, 0:Cs]]
+// out[b, y, x, :] = [low[b, y/2, x/2, 0:Cl] | skip[b, y, x, 0:Cs]], eight
+// channels per thread (Cl % 8 == 0, Cs % 8 == 0).
 __global__ void k_upcat(const bf16 *low, const bf16 *skip, bf16 *out, int B, int H, int Wd, int Cl, int Cs) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int C = Cl + Cs;
-    size_t n = (size_t)B * H * Wd * C;
-    if (e >= n) return;
-    int c = e % C;
-    size_t p = e / C;
-    int x = p % Wd, y = (p / Wd) % H, b = p / ((size_t)Wd * H);
-    out[e] = c < Cl ? low[(((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * Cl + c]
-                    : skip[p * Cs + (c - Cl)];
+    const unsigned C8 = (Cl + Cs) >> 3;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)B * H * Wd * C8) return;
+    const unsigned c = (e % C8) * 8, p = e / C8;
+    const unsigned x = p % Wd, r = p / Wd, y = r % H, b = r / H;
+    const u16x8 v = c < (unsigned)Cl ? *(const u16x8 *)(low + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * Cl + c)
+                                     : *(const u16x8 *)(skip + (size_t)p * Cs + (c - Cl));
+    *(u16x8 *)(out + (size_t)p * (Cl + Cs) + c) = v;
 }
 
-// Backward of upcat: dlow = sum of the 4 children; dskip = slice.
+// Backward of upcat: dlow = sum of the 4 children; dskip = slice (four
+// channels per thread).
 __global__ void k_upcat_bwd_low(const float *dcat, float *dlow, int B, int H, int Wd, int Cl, int Cs) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int H2 = H / 2, W2 = Wd / 2, C = Cl + Cs;
-    size_t n = (size_t)B * H2 * W2 * Cl;
-    if (e >= n) return;
-    int c = e % Cl;
-    size_t p = e / Cl;
-    int x = p % W2, y = (p / W2) % H2, b = p / ((size_t)W2 * H2);
-    auto at = [&](int yy, int xx) { return dcat[(((size_t)b * H + yy) * Wd + xx) * C + c]; };
-    dlow[e] = (at(2 * y, 2 * x) + at(2 * y, 2 * x + 1)) + (at(2 * y + 1, 2 * x) + at(2 * y + 1, 2 * x + 1));
+    const unsigned C4 = Cl >> 2, H2 = H / 2, W2 = Wd / 2, C = Cl + Cs;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)B * H2 * W2 * C4) return;
+    const unsigned c = (e % C4) * 4, p = e / C4;
+    const unsigned x = p % W2, r = p / W2, y = r % H2, b = r / H2;
+    const float *q = dcat + (((size_t)b * H + 2 * y) * Wd + 2 * x) * C + c;
+    const float4 a = *(const float4 *)q, bb = *(const float4 *)(q + C);
+    const float4 cc = *(const float4 *)(q + (size_t)Wd * C), dd = *(const float4 *)(q + (size_t)Wd * C + C);
+    *(float4 *)(dlow + (size_t)p * Cl + c) = float4{(a.x + bb.x) + (cc.x + dd.x), (a.y + bb.y) + (cc.y + dd.y),
+                                                    (a.z + bb.z) + (cc.z + dd.z), (a.w + bb.w) + (cc.w + dd.w)};
 }
 __global__ void k_upcat_bwd_skip(const float *dcat, float *dskip, int B, int H, int Wd, int Cl, int Cs) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)B * H * Wd * Cs;
-    if (e >= n) return;
-    int c = e % Cs;
-    size_t p = e / Cs;
-    dskip[e] = dcat[p * (Cl + Cs) + Cl + c];
+    const unsigned C4 = Cs >> 2;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)B * H * Wd * C4) return;
+    const unsigned c = (e % C4) * 4, p = e / C4;
+    *(float4 *)(dskip + (size_t)p * Cs + c) = *(const float4 *)(dcat + (size_t)p * (Cl + Cs) + Cl + c);
 }
 
-// 1x1 output head (FP32): y[m, o] = sum_c x[m, c] Wh[o, c] + bh[o]
+// 1x1 output head (FP32), one pixel per thread (C % 8 == 0, O <= 8):
+// y[m, o] = sum_c x[m, c] Wh[o, c] + bh[o], c ascending.
 __global__ void k_head_fwd(const bf16 *x, const bf16 *Wh, const float *bh, float *y, int npix, int C, int O) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)npix * O) return;
-    int o = e % O;
-    size_t m = e / O;
-    float s = 0.f;
-    for (int c = 0; c < C; c++) s += from_bf16(x[m * C + c]) * from_bf16(Wh[o * C + c]);
-    y[e] = s + bh[o];
+    const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= (unsigned)npix) return;
+    float s[8];
+    for (int o = 0; o < 8; o++) s[o] = 0.f;
+    for (int c = 0; c < C; c += 8) {
+        const u16x8 v = *(const u16x8 *)(x + (size_t)m * C + c);
+        for (int k = 0; k < 8; k++) {
+            const float xv = lb::bf16_to_f32(v[k]);
+            for (int o = 0; o < O; o++) s[o] += xv * from_bf16(Wh[o * C + c + k]);
+        }
+    }
+    for (int o = 0; o < O; o++) y[(size_t)m * O + o] = s[o] + bh[o];
 }
+// dx[m, c] = sum_o dy[m, o] Wh[o, c], four channels per thread.
 __global__ void k_head_bwd_data(const float *dy, const bf16 *Wh, float *dx, int npix, int C, int O) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)npix * C) return;
-    int c = e % C;
-    size_t m = e / C;
-    float s = 0.f;
-    for (int o = 0; o < O; o++) s += dy[m * O + o] * from_bf16(Wh[o * C + c]);
-    dx[e] = s;
+    const unsigned C4 = C >> 2;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)npix * C4) return;
+    const unsigned c = (e % C4) * 4, m = e / C4;
+    float r[4];
+    for (int k = 0; k < 4; k++) {
+        float s = 0.f;
+        for (int o = 0; o < O; o++) s += dy[(size_t)m * O + o] * from_bf16(Wh[o * C + c + k]);
+        r[k] = s;
+    }
+    *(float4 *)(dx + (size_t)m * C + c) = float4{r[0], r[1], r[2], r[3]};
 }
-// dWh[o, c] and dbh[o] partials: block (pair (o, c) or bias, pixel chunk),
-// fixed-order tree; k_sum_chunks adds the chunks.
+// dWh[o, c] and dbh[o] partials of one pixel chunk (O <= 8): thread
+// (channel, pixel phase) accumulates all O outputs for its channel (and the
+// bias sums), phases added in LDS in a fixed order; partial[chunk][o][c],
+// with the bias at c = C; k_sum_chunks adds the chunks.
 __global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 *x, int npix, int C, int O, int chunk,
                                                     float *partial) {
-    const int o = blockIdx.x / (C + 1), c = blockIdx.x % (C + 1);
-    const int pbeg = blockIdx.y * chunk, pend = min(npix, pbeg + chunk);
-    float s = 0.f;
-    for (int m = pbeg + threadIdx.x; m < pend; m += 256)
-        s += dy[(size_t)m * O + o] * (c < C ? from_bf16(x[(size_t)m * C + c]) : 1.f);
-    __shared__ float sh[256];
-    sh[threadIdx.x] = s;
+    const int nph = 256 / C;
+    const int t = threadIdx.x, c = t % C, ph = t / C;
+    const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
+    float acc[8], accb[8];
+    for (int o = 0; o < 8; o++) acc[o] = accb[o] = 0.f;
+    if (ph < nph)
+        for (int m = pbeg + ph; m < pend; m += nph) {
+            const float xv = from_bf16(x[(size_t)m * C + c]);
+            for (int o = 0; o < O; o++) {
+                const float g = dy[(size_t)m * O + o];
+                acc[o] += g * xv;
+                accb[o] += g;
+            }
+        }
+    __shared__ float sh[256 * 9];
+    const int np = O * (C + 1);
+    if (ph < nph)
+        for (int o = 0; o < O; o++) {
+            sh[ph * np + o * (C + 1) + c] = acc[o];
+            if (c == 0) sh[ph * np + o * (C + 1) + C] = accb[o];
+        }
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-        __syncthreads();
+    for (int e = t; e < np; e += 256) {
+        float v = 0.f;
+        for (int p = 0; p < nph; p++) v += sh[p * np + e];
+        partial[(size_t)blockIdx.x * np + e] = v;
     }
-    if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sh[0];
 }
 // partial (o, c) index -> dW [o][c], bias (o, C) -> db[o]
 __global__ void k_head_scatter(const float *sum, int C, int O, float *dW, float *db) {
@@ -774,10 +845,27 @@ __global__ void k_head_scatter(const float *sum, int C, int O, float *dW, float 
     else db[o] = sum[e];
 }
 
-// Masked mean-squared error: dy = 2 (y - t) mask / count, loss partials.
+// Per-block sums (double) of the mask weights: partial[block].
+__global__ __launch_bounds__(256) void k_mask_sum(const float *mask, int npix, double *partial) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sh[256];
+    sh[threadIdx.x] = e < (size_t)npix ? (double)mask[e] : 0.0;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// Masked mean-squared error: dy = 2 (y - t) mask / count with count = O x
+// (sum of the mask weights, *wsum; zero gradients for an all-zero mask), loss
+// partials.
 __global__ __launch_bounds__(256) void k_mse(const float *y, const float *t, const float *mask, float *dy, int npix,
-                                             int O, float inv_count, double *partial) {
+                                             int O, const double *wsum, double *partial) {
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double cnt = *wsum * O;
+    const float inv_count = cnt > 0 ? (float)(1.0 / cnt) : 0.f;
     double l = 0;
     if (e < (size_t)npix * O) {
         float w = mask[e / O];
@@ -895,7 +983,7 @@ struct lb_unet {
     float *gA = nullptr, *gB = nullptr, *gC = nullptr;   // gradient scratch (largest tensor)
     bf16 *tA = nullptr, *tB = nullptr;                   // channel-major copies for the weight gradient
     std::vector<float *> gskip;    // gradient of each encoder output (skip)
-    float *partial = nullptr, *partial2 = nullptr;   // chunk partials, group sums
+    float *partial = nullptr;      // chunk partials of the weight, bias and head gradients
     size_t partial_n = 0;
     double *lpart = nullptr, *lsum = nullptr;
     std::vector<void *> allocs;
@@ -921,32 +1009,20 @@ struct Level { int H, W, C; };
 // Conv index layout: enc l (2 convs), bottom (2), dec l (2), l descending.
 inline int enc_conv(int l) { return 2 * l; }
 
-// Chunk count of the groups of the two-level sum over nchunk chunks.
-inline int chunk_groups(int nchunk) {
-    int g = 1;
-    while (g * g < nchunk) g++;
-    return g;
-}
-constexpr int kBiasChunk = 256;
+// Pixels per chunk of the head's weight-gradient partials.
+constexpr int kHeadChunk = 512;
 
-// Fixed-order sum of nchunk partial arrays of n floats: one pass for a few
-// chunks, else about sqrt(nchunk) groups in parallel, then the groups (the
-// long serial loops of a single pass leave most of the GPU idle when n is
-// small).
+// Fixed-order sum of nchunk partial arrays of n floats: one thread per
+// element for a few chunks, else 16 phases per element (the long serial
+// loops of one thread per element leave most of the GPU idle when n is small).
 void sum_chunks(lb_unet *u, const float *partial, int nchunk, size_t n, float *out) {
-    if (nchunk <= 8 || !u->partial2) {
+    if (nchunk <= 8)
         hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, partial, nchunk, n, out);
-        return;
-    }
-    const int G = chunk_groups(nchunk), S = (nchunk + G - 1) / G;
-    hipLaunchKernelGGL(k_sum_chunk_groups, dim3(nb256(n), G), dim3(256), 0, u->stream, partial, nchunk, S, n, u->partial2);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, u->partial2, G, n, out);
+    else
+        hipLaunchKernelGGL(k_sum_chunks16, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, u->stream, partial, nchunk, n,
+                           out);
 }
 
-// LDS-tile launch for eligible shapes; returns false (nothing launched) when
-// the shape does not fit the tile kernel.  Tile choice: the largest tile
-// (4 PM rows x 32 pixels x 32 NB channels) that still gives at least two
-// workgroups per CU, else the smallest.
 template <class TIn, class TOut, bool BR>
 bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
                       int Ci, int Co) {
@@ -1054,7 +1130,8 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
     *out = nullptr;
     if (lb_device_count() <= 0) return LB_ERR_NO_DEVICE;
     const int Lv = desc->levels;
-    if (Lv < 2 || Lv > 6 || desc->base % 32 || desc->in_ch < 1 || desc->in_ch > 16 || desc->out_ch < 1 ||
+    if (Lv < 2 || Lv > 6 || desc->base % 32 || desc->base > 256 || (desc->base << (Lv - 1)) > 1024 ||
+        desc->in_ch < 1 || desc->in_ch > 16 || desc->out_ch < 1 || desc->out_ch > 8 ||
         desc->max_batch < 1 || desc->height % (1 << (Lv - 1)) || desc->width % (1 << (Lv - 1)))
         return LB_ERR_INVALID;
     lb_unet *u = new (std::nothrow) lb_unet();
@@ -1139,7 +1216,6 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
     u->y = u->alloc<float>((size_t)B * L[0].H * L[0].W * desc->out_ch);
     u->dy = u->alloc<float>((size_t)B * L[0].H * L[0].W * desc->out_ch);
     // weight-gradient partials: max over convs of nchunk * Co * K
-    size_t p2 = 0;
     for (size_t ci = 0; ci < u->convs.size(); ci++) {
         int l;
         if (ci < (size_t)2 * (Lv - 1)) l = (int)ci / 2;
@@ -1150,21 +1226,20 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         size_t nchunk = std::max((size_t)wgrad_chunks_legacy((int)npix), (size_t)wt.nchunk);
         const size_t nw = (size_t)u->convs[ci].co * 9 * u->convs[ci].ci;
         u->partial_n = std::max(u->partial_n, nchunk * nw);
-        // group sums of the two-level reductions (weights, bias)
-        p2 = std::max(p2, (size_t)chunk_groups((int)nchunk) * nw);
-        p2 = std::max(p2, (size_t)chunk_groups((int)((npix + kBiasChunk - 1) / kBiasChunk)) * u->convs[ci].co);
+        u->partial_n = std::max(u->partial_n, (size_t)bias_chunks((int)npix).nchunk * u->convs[ci].co);
     }
     {
+        // head: chunk partials followed by their sum
         const size_t npix0 = (size_t)B * L[0].H * L[0].W;
-        p2 = std::max(p2, (size_t)chunk_groups((int)((npix0 + 4095) / 4096)) * desc->out_ch * (L[0].C + 1));
+        const size_t nchunk = (npix0 + kHeadChunk - 1) / kHeadChunk;
+        u->partial_n = std::max(u->partial_n, (nchunk + 1) * desc->out_ch * (L[0].C + 1));
     }
     u->partial = u->alloc<float>(u->partial_n);
-    u->partial2 = u->alloc<float>(p2);
     u->lpart = u->alloc<double>(nb256((size_t)B * L[0].H * L[0].W * desc->out_ch));
-    u->lsum = u->alloc<double>(1);
+    u->lsum = u->alloc<double>(2);
     bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->gA && u->gB && u->gC && u->tA && u->tB &&
               u->xin && u->y &&
-              u->dy && u->partial && u->partial2 && u->lpart && u->lsum;
+              u->dy && u->partial && u->lpart && u->lsum;
     for (auto p : u->act) ok = ok && p;
     for (auto p : u->pool) ok = ok && p;
     for (auto p : u->cat) ok = ok && p;
@@ -1238,14 +1313,14 @@ extern "C" int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y) 
             cur = u->act[ci];
         }
         if (l < Lv - 1) {
-            size_t n = (size_t)B * L[l + 1].H * L[l + 1].W * L[l].C;
+            size_t n = (size_t)B * L[l + 1].H * L[l + 1].W * L[l].C / 8;
             hipLaunchKernelGGL(k_avgpool, dim3(nb256(n)), dim3(256), 0, u->stream, cur, u->pool[l], B, L[l].H, L[l].W, L[l].C);
             cur = u->pool[l];
         }
     }
     for (int l = Lv - 2; l >= 0; l--) {
         const int npix = B * L[l].H * L[l].W;
-        size_t n = (size_t)npix * (L[l + 1].C + L[l].C);
+        size_t n = (size_t)npix * (L[l + 1].C + L[l].C) / 8;
         hipLaunchKernelGGL(k_upcat, dim3(nb256(n)), dim3(256), 0, u->stream, cur, u->act[enc_conv(l) + 1], u->cat[l], B,
                            L[l].H, L[l].W, L[l + 1].C, L[l].C);
         cur = u->cat[l];
@@ -1256,7 +1331,7 @@ extern "C" int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y) 
         }
     }
     float *y = d_y ? d_y : u->y;
-    hipLaunchKernelGGL(k_head_fwd, dim3(nb256(npix0 * u->d.out_ch)), dim3(256), 0, u->stream, cur, u->wb + u->head_woff,
+    hipLaunchKernelGGL(k_head_fwd, dim3(nb256(npix0)), dim3(256), 0, u->stream, cur, u->wb + u->head_woff,
                        u->w + u->head_boff, y, (int)npix0, L[0].C, u->d.out_ch);
     if (y != u->y)
         (void)hipMemcpyAsync(u->y, y, npix0 * u->d.out_ch * 4, hipMemcpyDeviceToDevice, u->stream);
@@ -1275,32 +1350,28 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     const int Lv = u->d.levels;
     const int O = u->d.out_ch;
     const int npix0 = B * L[0].H * L[0].W;
-    // count of weighted entries (host: masks are 0/1, count on the device side
-    // would need another reduction; the caller's mask sum is recomputed here)
-    std::vector<float> hmask(npix0);
-    int rc = (int)hipMemcpyAsync(hmask.data(), d_mask, npix0 * 4, hipMemcpyDeviceToHost, u->stream);
-    if (!rc) rc = (int)hipStreamSynchronize(u->stream);
-    if (rc) return rc;
-    double cnt = 0;
-    for (float v : hmask) cnt += v;
-    cnt *= O;
-    if (cnt <= 0) return LB_ERR_INVALID;
-    const unsigned nbl = nb256((size_t)npix0 * O);
-    hipLaunchKernelGGL(k_mse, dim3(nbl), dim3(256), 0, u->stream, u->y, d_t, d_mask, u->dy, npix0, O, (float)(1.0 / cnt), u->lpart);
+    // lsum[1] = sum of the mask weights, lsum[0] = sum of squared weighted
+    // errors, both reduced on the device (no host round trip before the
+    // backward pass)
+    const unsigned nbm = nb256((size_t)npix0), nbl = nb256((size_t)npix0 * O);
+    hipLaunchKernelGGL(k_mask_sum, dim3(nbm), dim3(256), 0, u->stream, d_mask, npix0, u->lpart);
+    hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(256), 0, u->stream, u->lpart, (int)nbm, u->lsum + 1);
+    hipLaunchKernelGGL(k_mse, dim3(nbl), dim3(256), 0, u->stream, u->y, d_t, d_mask, u->dy, npix0, O, u->lsum + 1, u->lpart);
     hipLaunchKernelGGL(k_sum_d, dim3(1), dim3(256), 0, u->stream, u->lpart, (int)nbl, u->lsum);
+    int rc = 0;
     // head
     const int C0 = L[0].C;
     const int last = (int)u->convs.size() - 1;
     {
         // partials in the weight-gradient buffer (free until the first wgrad_launch)
-        const int chunk = 4096, nchunk = (npix0 + chunk - 1) / chunk, np = O * (C0 + 1);
-        hipLaunchKernelGGL(k_head_bwd_w, dim3(np, nchunk), dim3(256), 0, u->stream, u->dy, u->act[last], npix0, C0, O, chunk,
+        const int chunk = kHeadChunk, nchunk = (npix0 + chunk - 1) / chunk, np = O * (C0 + 1);
+        hipLaunchKernelGGL(k_head_bwd_w, dim3(nchunk), dim3(256), 0, u->stream, u->dy, u->act[last], npix0, C0, O, chunk,
                            u->partial);
         sum_chunks(u, u->partial, nchunk, (size_t)np, u->partial + (size_t)nchunk * np);
         hipLaunchKernelGGL(k_head_scatter, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial + (size_t)nchunk * np, C0, O,
                            u->g + u->head_woff, u->g + u->head_boff);
     }
-    hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0)), dim3(256), 0, u->stream, u->dy, u->wb + u->head_woff,
+    hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0 / 4)), dim3(256), 0, u->stream, u->dy, u->wb + u->head_woff,
                        u->gA, npix0, C0, O);
     // gA = d(out of current conv), post-activation gradient
     float *gcur = u->gA, *gtmp = u->gB, *gpre = u->gC;
@@ -1308,14 +1379,13 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
-        size_t n = (size_t)npix * c.co;
-        hipLaunchKernelGGL(k_relu_bwd, dim3(nb256(n)), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, n);
         {
-            // the weight-gradient partial buffer is free until wgrad_launch below
-            const int chunk = kBiasChunk, nchunk = (npix + chunk - 1) / chunk;
-            hipLaunchKernelGGL(k_bias_grad, dim3((c.co + 31) / 32, nchunk), dim3(256), 0, u->stream, gpre, npix, c.co, chunk,
-                               u->partial);
-            sum_chunks(u, u->partial, nchunk, (size_t)c.co, u->g + c.boff);
+            // ReLU backward + bias-gradient partials (the weight-gradient
+            // partial buffer is free until wgrad_launch below)
+            const BiasChunks bc = bias_chunks(npix);
+            hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
+                               c.co, bc.chunk, u->partial);
+            sum_chunks(u, u->partial, bc.nchunk, (size_t)c.co, u->g + c.boff);
         }
         wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB);
         if (need_din)
@@ -1330,10 +1400,10 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         ci--;
         // gcur = d(cat[lev]); split into d(low) and d(skip)
         const int npix = B * L[lev].H * L[lev].W;
-        size_t nsk = (size_t)npix * L[lev].C;
+        size_t nsk = (size_t)npix * L[lev].C / 4;
         hipLaunchKernelGGL(k_upcat_bwd_skip, dim3(nb256(nsk)), dim3(256), 0, u->stream, gcur, u->gskip[lev], B, L[lev].H,
                            L[lev].W, L[lev + 1].C, L[lev].C);
-        size_t nlow = (size_t)B * L[lev + 1].H * L[lev + 1].W * L[lev + 1].C;
+        size_t nlow = (size_t)B * L[lev + 1].H * L[lev + 1].W * L[lev + 1].C / 4;
         hipLaunchKernelGGL(k_upcat_bwd_low, dim3(nb256(nlow)), dim3(256), 0, u->stream, gcur, gtmp, B, L[lev].H, L[lev].W,
                            L[lev + 1].C, L[lev].C);
         std::swap(gcur, gtmp);
@@ -1347,17 +1417,19 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         if (l > 0) {
             // gcur = d(pool[l-1]); add the pool backward into the skip gradient
             size_t n = (size_t)B * L[l - 1].H * L[l - 1].W * L[l - 1].C;
-            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb256(n)), dim3(256), 0, u->stream, gcur, u->gskip[l - 1], B, L[l - 1].H,
+            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb256(n / 4)), dim3(256), 0, u->stream, gcur, u->gskip[l - 1], B, L[l - 1].H,
                                L[l - 1].W, L[l - 1].C, 1);
             (void)hipMemcpyAsync(gcur, u->gskip[l - 1], n * 4, hipMemcpyDeviceToDevice, u->stream);
         }
     }
     rc = (int)hipGetLastError();
     if (!rc && loss) {
-        double s = 0;
-        rc = (int)hipMemcpyAsync(&s, u->lsum, 8, hipMemcpyDeviceToHost, u->stream);
+        double s[2] = {0, 0};
+        rc = (int)hipMemcpyAsync(s, u->lsum, 16, hipMemcpyDeviceToHost, u->stream);
         if (!rc) rc = (int)hipStreamSynchronize(u->stream);
-        *loss = s / cnt;
+        const double cnt = s[1] * O;
+        if (!rc && cnt <= 0) rc = LB_ERR_INVALID;
+        *loss = cnt > 0 ? s[0] / cnt : 0.0;
     }
     return rc;
 }
@@ -1418,11 +1490,10 @@ extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void
     lb_unet tmp;
     tmp.stream = (hipStream_t)stream;
     tmp.partial = tmp.alloc<float>(nchunk * Co * 9 * Ci);
-    tmp.partial2 = tmp.alloc<float>((size_t)chunk_groups((int)nchunk) * Co * 9 * Ci);
     bf16 *tA = method ? tmp.alloc<bf16>((size_t)npix * Co) : nullptr;
     bf16 *tB = method ? tmp.alloc<bf16>((size_t)npix * Ci) : nullptr;
     int rc = 0;
-    if (!tmp.partial || !tmp.partial2 || (method && (!tA || !tB))) rc = LB_ERR_INVALID;
+    if (!tmp.partial || (method && (!tA || !tB))) rc = LB_ERR_INVALID;
     if (!rc) {
         wgrad_launch<bf16>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB, method);
         rc = (int)hipGetLastError();

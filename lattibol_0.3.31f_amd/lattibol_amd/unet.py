@@ -98,7 +98,12 @@ class UNet:
         check(lib.lb_unet_forward(self._h, x_dev.ptr, int(B), y_dev.ptr if y_dev is not None else None),
               "lb_unet_forward")
 
-    def backward(self, t_dev, mask_dev, B):
+    def backward(self, t_dev, mask_dev, B, want_loss=True):
+        """Gradients of the masked MSE of the last forward; returns the loss
+        (waits for the GPU) or, with want_loss=False, None (asynchronous)."""
+        if not want_loss:
+            check(lib.lb_unet_backward(self._h, t_dev.ptr, mask_dev.ptr, int(B), None), "lb_unet_backward")
+            return None
         loss = C.c_double()
         check(lib.lb_unet_backward(self._h, t_dev.ptr, mask_dev.ptr, int(B), C.byref(loss)), "lb_unet_backward")
         return loss.value

@@ -231,3 +231,35 @@ def test_gpu_unet_training_reduces_loss(gpu):
     net.sync()
     assert losses[-1] < 0.3 * losses[0], (losses[0], losses[-1])
     net.close()
+
+
+@pytest.mark.gpu
+def test_gpu_unet_backward_without_loss_readback(gpu):
+    # the asynchronous form (no loss read-back, mask weight count reduced on
+    # the device) gives the same gradients as the synchronous one; an
+    # all-zero mask gives zero gradients and an error when the loss is asked for
+    from lattibol_amd import _lib
+    from lattibol_amd.unet import UNet
+    B, H = 2, 32
+    net = UNet(**SMALL, height=H, width=H, max_batch=B, seed=3)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    t = rng.standard_normal((B, H, H, 3)).astype(np.float32)
+    m = (rng.uniform(size=(B, H, H)) > 0.3).astype(np.float32)
+    xd, td, md = (_lib.DeviceArray.from_numpy(a) for a in (x, t, m))
+    net.forward(xd, B)
+    assert net.backward(td, md, B, want_loss=False) is None
+    _, g_async = net.params(grads=True)
+    net.forward(xd, B)
+    loss = net.backward(td, md, B)
+    _, g_sync = net.params(grads=True)
+    assert np.array_equal(g_async, g_sync)
+    assert loss > 0
+    zd = _lib.DeviceArray.from_numpy(np.zeros((B, H, H), np.float32))
+    net.forward(xd, B)
+    net.backward(td, zd, B, want_loss=False)
+    _, g0 = net.params(grads=True)
+    assert not np.any(g0)
+    with pytest.raises(_lib.LatticeError):
+        net.backward(td, zd, B)
+    net.close()
