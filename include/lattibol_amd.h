@@ -363,9 +363,9 @@ int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_
  * times the shifted input d_in (BF16 NHWC).  method 0: per-element gathers;
  * 1: channel-major transposed operands, 16-byte loads (W % 16 == 0, else
  * LB_ERR_UNSUPPORTED).  Both sum in the same order (identical results).
- * 2: LDS tiles of 8 rows x 32 pixels (W % 32 == 0 and H % 8 == 0, else
- * LB_ERR_UNSUPPORTED), the kernel the U-Net takes for such shapes; its FP32
- * summation order differs from 0 / 1.
+ * 2: LDS tiles of 8 rows x 32 pixels (W % 32 == 0 and H % 8 == 0) or 16 x
+ * 16 (W % 16 == 0 and H % 16 == 0), else LB_ERR_UNSUPPORTED; the kernel the
+ * U-Net takes for such shapes; its FP32 summation order differs from 0 / 1.
  * Allocates its scratch and synchronises: a test and diagnostic entry point;
  * lb_unet_backward uses the same kernels on the handle's buffers. */
 int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,

@@ -477,23 +477,22 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_cm(const bf16 *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient through LDS tiles (W % 32 == 0, H % 8 == 0, Co % 32 == 0,
-// Ci % 8 == 0): block (32 output channels, 32 input channels, chunk of pixel
-// tiles).  A pixel tile is 8 image rows x 32 pixels.  Per tile the gradient
-// block (256 pixels x 32 channels, FP32 -> BF16) and the input window (10
-// rows x 36 pixels x 32 channels, zero outside the image) are written to LDS
+// Weight gradient through LDS tiles (Co % 32 == 0, Ci % 8 == 0; TW = 32:
+// W % 32 == 0, H % 8 == 0; TW = 16: W % 16 == 0, H % 16 == 0): block (32
+// output channels, 32 input channels, chunk of pixel tiles).  A pixel tile
+// is 256 / TW image rows x TW pixels.  Per tile the gradient block (256
+// pixels x 32 channels, FP32 -> BF16) and the input window (tile rows + 2 x
+// TW + 4 pixels x 32 channels, zero outside the image) are written to LDS
 // channel-major, four pixels per 8-byte store, so that an MFMA operand (8
 // consecutive pixels of one channel) is one 16-byte LDS read.  Wavefront w
-// takes image rows 2w, 2w + 1 of the tile (four 16-pixel K-steps) and keeps
-// all nine taps' 32 x 32 accumulators; the column shift of a tap is a funnel
+// takes a quarter of the tile's rows (four 16-pixel K-steps) and keeps all
+// nine taps' 32 x 32 accumulators; the column shift of a tap is a funnel
 // shift across an aligned 16-byte and 4-byte read of the window row.  The
 // next tile's global loads are issued into registers before the MFMAs of
 // the current one.  The four wavefronts' tiles are added in LDS in a fixed
 // order and written as the chunk's partial (partial[chunk][co][tap][ci]).
 // ---------------------------------------------------------------------------
-constexpr int kWgRows = 8;      // image rows per pixel tile
-constexpr int kWgWin = 40;      // LDS stride of a window row (36 pixels used)
-constexpr int kWgXS = 520;      // LDS stride of one input channel (10 rows; 260 dwords = 4 mod 64 banks)
+constexpr int kWgXS = 520;      // LDS stride of one input channel (<= 18 window rows; 260 dwords = 4 mod 64 banks)
 constexpr int kWgGS = 264;      // LDS stride of one output channel (256 pixels; 132 dwords = 4 mod 64 banks)
 
 __device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
@@ -501,6 +500,7 @@ __device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint1
 }
 __device__ __forceinline__ float lane_of(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
+template <int TW>
 __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restrict__ dout, const bf16 *__restrict__ in,
                                                             float *__restrict__ partial, int H, int Wd, int Ci, int Co,
                                                             int ntiles, int tpw) {
@@ -510,15 +510,20 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
     const int col = lane & 31, kh = (lane >> 5) * 8;
     const int co0 = blockIdx.x * 32, ci0 = blockIdx.y * 32;
     const int K = 9 * Ci;
-    const int segs = Wd >> 5, rblocks = H / kWgRows;
+    constexpr int TR = 256 / TW;                 // tile rows
+    constexpr int WIN = TW == 32 ? 40 : 24;      // LDS stride of a window row (TW + 4 pixels used)
+    constexpr int NQX = (TW + 4) / 4;            // 4-pixel groups of a window row
+    constexpr int QPR = TW / 4;                  // 4-pixel groups of a tile row
+    constexpr int KPR = TW / 16;                 // 16-pixel K-steps of a tile row
+    const int segs = Wd / TW, rblocks = H / TR;
     const int t0 = blockIdx.z * tpw, t1 = min(ntiles, t0 + tpw);
-    constexpr int NXQ = (kWgRows + 2) * 9 * 4;   // window: rows x nine 4-pixel groups x four 8-channel groups
+    constexpr int NXQ = (TR + 2) * NQX * 4;      // window: rows x 4-pixel groups x four 8-channel groups
     constexpr int RX = (NXQ + 255) / 256;
     u16x8 xq[RX][4];
     float4 gq[4][2];
     auto fetch = [&](int tile) {
         const int seg = tile % segs, rest = tile / segs;
-        const int y0 = (rest % rblocks) * kWgRows, img = rest / rblocks, x0 = seg * 32;
+        const int y0 = (rest % rblocks) * TR, img = rest / rblocks, x0 = seg * TW;
 #pragma unroll
         for (int j = 0; j < RX; j++) {
             const int e = tid + 256 * j;
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
             for (int p = 0; p < 4; p++) xq[j][p] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
             if (e < NXQ) {
                 const int q = e & 3, r = e >> 2;
-                const int iy = r / 9, qx = r - iy * 9;
+                const int iy = r / NQX, qx = r - iy * NQX;
                 const int yy = y0 + iy - 1, c = ci0 + 8 * q;
                 if (yy >= 0 && yy < H && c < Ci) {
 #pragma unroll
@@ -537,9 +542,9 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
                 }
             }
         }
-        {   // gradient: 8 rows x eight 4-pixel groups x four 8-channel groups = one piece per thread
+        {   // gradient: 64 4-pixel groups x four 8-channel groups = one piece per thread
             const int q = tid & 3, r = tid >> 2;
-            const int iy = r >> 3, qx = r & 7;
+            const int iy = r / QPR, qx = r % QPR;
             const float *g = dout + (((size_t)img * H + y0 + iy) * Wd + x0 + 4 * qx) * Co + co0 + 8 * q;
 #pragma unroll
             for (int p = 0; p < 4; p++) {
@@ -554,8 +559,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
             const int e = tid + 256 * j;
             if (e < NXQ) {
                 const int q = e & 3, r = e >> 2;
-                const int iy = r / 9, qx = r - iy * 9;
-                const int base = iy * kWgWin + 4 * qx;
+                const int iy = r / NQX, qx = r - iy * NQX;
+                const int base = iy * WIN + 4 * qx;
 #pragma unroll
                 for (int k = 0; k < 8; k++)
                     *(uint2 *)&sX[(8 * q + k) * kWgXS + base] = pack4(xq[j][0][k], xq[j][1][k], xq[j][2][k], xq[j][3][k]);
@@ -563,7 +568,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
         }
         {
             const int q = tid & 3, r = tid >> 2;
-            const int base = (r >> 3) * 32 + 4 * (r & 7);
+            const int base = 4 * r;              // tile pixel (r / QPR) TW + 4 (r % QPR)
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int h = k >> 2, c = k & 3;
@@ -586,11 +591,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
         if (t + 1 < t1) fetch(t + 1);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            const int trow = 2 * wave + (s >> 1), pxb = (s & 1) * 16 + kh;
-            const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sG[col * kWgGS + trow * 32 + pxb]);
+            const int trow = wave * (4 / KPR) + s / KPR, pxb = (s % KPR) * 16 + kh;
+            const bf16x8 a = __builtin_bit_cast(bf16x8, *(const u16x8 *)&sG[col * kWgGS + trow * TW + pxb]);
 #pragma unroll
             for (int dy = 0; dy < 3; dy++) {
-                const uint16_t *xp = &sX[col * kWgXS + (trow + dy) * kWgWin + pxb];
+                const uint16_t *xp = &sX[col * kWgXS + (trow + dy) * WIN + pxb];
                 const u32x4 d = *(const u32x4 *)xp;
                 const uint32_t d4 = *(const uint32_t *)(xp + 8);
                 const u32x4 b1 = u32x4{__builtin_amdgcn_alignbit(d[1], d[0], 16), __builtin_amdgcn_alignbit(d[2], d[1], 16),
@@ -674,6 +679,7 @@ __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__
     const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     if (ph < nph) {
+#pragma unroll 4
         for (int m = pbeg + ph; m < pend; m += nph) {
             const size_t o = (size_t)m * Co + 4 * cg;
             const float4 g = *(const float4 *)(d + o);
@@ -814,6 +820,7 @@ __global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 
     float acc[8], accb[8];
     for (int o = 0; o < 8; o++) acc[o] = accb[o] = 0.f;
     if (ph < nph)
+#pragma unroll 4
         for (int m = pbeg + ph; m < pend; m += nph) {
             const float xv = from_bf16(x[(size_t)m * C + c]);
             for (int o = 0; o < O; o++) {
@@ -918,15 +925,22 @@ __global__ void k_to_bf16(const float *w, bf16 *wb, size_t n) {
     if (e < n) wb[e] = to_bf16(w[e]);
 }
 
-// W'[ci, tap', co] = W[co, 8 - tap', ci] (flipped, transposed: backward data)
-__global__ void k_flip_transpose(const bf16 *W, bf16 *Wt, int Ci, int Co) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)Ci * 9 * Co;
-    if (e >= n) return;
-    int co = e % Co;
-    size_t r = e / Co;
-    int tap = r % 9, ci = r / 9;
-    Wt[e] = W[((size_t)co * 9 + (8 - tap)) * Ci + ci];
+// W'[ci, tap', co] = W[co, 8 - tap', ci] (flipped, transposed: backward
+// data) for every conv in one launch.  tab (device memory) holds per conv
+// {wt offset, wb offset, Ci, Co}; element e of the packed flipped copies
+// belongs to the last conv whose wt offset is <= e.
+__global__ void k_flip_transpose_all(const bf16 *wb, bf16 *wt, const long long *tab, int nconv, size_t total) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= total) return;
+    int j = 0;
+    while (j + 1 < nconv && (size_t)tab[4 * (j + 1)] <= e) j++;
+    const size_t local = e - (size_t)tab[4 * j];
+    const size_t woff = (size_t)tab[4 * j + 1];
+    const int Ci = (int)tab[4 * j + 2], Co = (int)tab[4 * j + 3];
+    const int co = (int)(local % Co);
+    const size_t r = local / Co;
+    const int tap = (int)(r % 9), ci = (int)(r / 9);
+    wt[e] = wb[woff + ((size_t)co * 9 + (8 - tap)) * Ci + ci];
 }
 
 // Input tensor: fp32 (B, H, W, cin) -> bf16 (B, H, W, cpad), zero padded.
@@ -974,6 +988,8 @@ struct lb_unet {
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // FP32 master / grad / Adam
     bf16 *wb = nullptr, *wt = nullptr;                                // BF16 weights, flipped copies
     std::vector<size_t> wt_off;
+    size_t wt_total = 0;
+    long long *flip_tab = nullptr;   // device: per conv {wt offset, wb offset, Ci, Co}
     // activations (bf16) and gradients (fp32)
     std::vector<bf16 *> act;       // per conv output
     std::vector<bf16 *> pool;      // per encoder level
@@ -1070,13 +1086,15 @@ void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TO
 
 // Pixel-tile chunks of the LDS-tile weight gradient: tiles per workgroup
 // chosen for about 768 workgroups; 0 when the shape does not fit the kernel.
-struct WgradTiles { int ntiles, tpw, nchunk; };
+struct WgradTiles { int ntiles, tpw, nchunk, tw; };
 inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
-    if (Wd % 32 || H % kWgRows || Co % 32 || Ci % 8) return {0, 0, 0};
-    const int ntiles = npix / (kWgRows * 32);
+    if (Co % 32 || Ci % 8) return {0, 0, 0, 0};
+    const int tw = (Wd % 32 == 0 && H % 8 == 0) ? 32 : (Wd % 16 == 0 && H % 16 == 0) ? 16 : 0;
+    if (!tw) return {0, 0, 0, 0};
+    const int ntiles = npix / 256;
     const long base = (long)(Co / 32) * ((Ci + 31) / 32);
     const int tpw = (int)std::max(1L, (ntiles * base + 767) / 768);
-    return {ntiles, tpw, (ntiles + tpw - 1) / tpw};
+    return {ntiles, tpw, (ntiles + tpw - 1) / tpw, tw};
 }
 // Chunk count of the per-wavefront weight-gradient kernels (1,024 pixels each).
 inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
@@ -1093,8 +1111,13 @@ void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int n
     const int K = 9 * Ci;
     const WgradTiles wt = wgrad_tiles(npix, H, Wd, Ci, Co);
     if ((method < 0 || method == 2) && wt.ntiles) {
-        hipLaunchKernelGGL(k_conv3x3_wgrad_tile, dim3(Co / 32, (Ci + 31) / 32, wt.nchunk), dim3(256), 0, u->stream, dout,
-                           (const bf16 *)in, u->partial, H, Wd, Ci, Co, wt.ntiles, wt.tpw);
+        const dim3 grd(Co / 32, (Ci + 31) / 32, wt.nchunk);
+        if (wt.tw == 32)
+            hipLaunchKernelGGL(k_conv3x3_wgrad_tile<32>, grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+                               Wd, Ci, Co, wt.ntiles, wt.tpw);
+        else
+            hipLaunchKernelGGL(k_conv3x3_wgrad_tile<16>, grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+                               Wd, Ci, Co, wt.ntiles, wt.tpw);
         sum_chunks(u, u->partial, wt.nchunk, (size_t)Co * K, dW);
         return;
     }
@@ -1165,8 +1188,17 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
     u->w = u->alloc<float>(P); u->g = u->alloc<float>(P); u->m = u->alloc<float>(P); u->v = u->alloc<float>(P);
     u->wb = u->alloc<bf16>(P);
     size_t wt_total = 0;
-    for (auto &c : u->convs) { u->wt_off.push_back(wt_total); wt_total += (size_t)c.co * 9 * c.ci; }
+    std::vector<long long> htab;
+    for (auto &c : u->convs) {
+        u->wt_off.push_back(wt_total);
+        htab.insert(htab.end(), {(long long)wt_total, (long long)c.woff, (long long)c.ci, (long long)c.co});
+        wt_total += (size_t)c.co * 9 * c.ci;
+    }
+    u->wt_total = wt_total;
     u->wt = u->alloc<bf16>(wt_total);
+    u->flip_tab = u->alloc<long long>(htab.size());
+    if (!rc && u->flip_tab)
+        rc = (int)hipMemcpyAsync(u->flip_tab, htab.data(), htab.size() * sizeof(long long), hipMemcpyHostToDevice, u->stream);
     // He-uniform initialisation from SplitMix64 (host), zero biases
     std::vector<float> hw(P, 0.f);
     uint64_t st = seed;
@@ -1237,7 +1269,7 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
     u->partial = u->alloc<float>(u->partial_n);
     u->lpart = u->alloc<double>(nb256((size_t)B * L[0].H * L[0].W * desc->out_ch));
     u->lsum = u->alloc<double>(2);
-    bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->gA && u->gB && u->gC && u->tA && u->tB &&
+    bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->flip_tab && u->gA && u->gB && u->gC && u->tA && u->tB &&
               u->xin && u->y &&
               u->dy && u->partial && u->lpart && u->lsum;
     for (auto p : u->act) ok = ok && p;
@@ -1260,12 +1292,8 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
 // BF16 copies and flipped/transposed copies from the FP32 masters.
 int lb_unet_refresh(lb_unet *u) {
     hipLaunchKernelGGL(k_to_bf16, dim3(nb256(u->n_params)), dim3(256), 0, u->stream, u->w, u->wb, u->n_params);
-    for (size_t ci = 0; ci < u->convs.size(); ci++) {
-        const UConv &c = u->convs[ci];
-        size_t n = (size_t)c.ci * 9 * c.co;
-        hipLaunchKernelGGL(k_flip_transpose, dim3(nb256(n)), dim3(256), 0, u->stream, u->wb + c.woff, u->wt + u->wt_off[ci],
-                           c.ci, c.co);
-    }
+    hipLaunchKernelGGL(k_flip_transpose_all, dim3(nb256(u->wt_total)), dim3(256), 0, u->stream, u->wb, u->wt, u->flip_tab,
+                       (int)u->convs.size(), u->wt_total);
     return (int)hipGetLastError();
 }
 
@@ -1440,12 +1468,8 @@ extern "C" int lb_unet_adam(lb_unet *u, float lr, float beta1, float beta2, floa
     float c1 = 1.f - std::pow(beta1, (float)u->steps), c2 = 1.f - std::pow(beta2, (float)u->steps);
     hipLaunchKernelGGL(k_adam, dim3(nb256(u->n_params)), dim3(256), 0, u->stream, u->w, u->g, u->m, u->v, u->wb, u->n_params,
                        lr, beta1, beta2, eps, c1, c2);
-    for (size_t ci = 0; ci < u->convs.size(); ci++) {
-        const UConv &c = u->convs[ci];
-        size_t n = (size_t)c.ci * 9 * c.co;
-        hipLaunchKernelGGL(k_flip_transpose, dim3(nb256(n)), dim3(256), 0, u->stream, u->wb + c.woff, u->wt + u->wt_off[ci],
-                           c.ci, c.co);
-    }
+    hipLaunchKernelGGL(k_flip_transpose_all, dim3(nb256(u->wt_total)), dim3(256), 0, u->stream, u->wb, u->wt, u->flip_tab,
+                       (int)u->convs.size(), u->wt_total);
     return (int)hipGetLastError();
 }
 
@@ -1478,7 +1502,8 @@ extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, cons
 
 // Weight gradient of one 3x3 convolution (test and diagnostic entry point;
 // allocates its scratch).  method 0: gathering kernel, 1: channel-major
-// operands (needs W % 16 == 0), 2: LDS tiles (needs W % 32 == 0, H % 8 == 0).
+// operands (needs W % 16 == 0), 2: LDS tiles (needs W % 32 == 0 and H % 8 == 0,
+// or W % 16 == 0 and H % 16 == 0).
 extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, float *d_dw, int B, int H, int W,
                                      int Ci, int Co, lb_stream_t stream) {
     if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || method < 0 || method > 2) return LB_ERR_INVALID;

@@ -139,7 +139,8 @@ def test_gpu_bf16_conversion_instruction_matches_bit_rounding(gpu):
 @pytest.mark.gpu
 @pytest.mark.parametrize("ci,co,H,W", [(16, 32, 16, 16), (32, 64, 8, 32), (96, 32, 16, 16), (384, 128, 16, 16),
                                        (64, 32, 8, 8), (16, 32, 8, 64), (96, 32, 16, 32), (384, 128, 8, 32),
-                                       (64, 64, 24, 32), (32, 32, 128, 128)])
+                                       (64, 64, 24, 32), (32, 32, 128, 128), (256, 256, 16, 16), (16, 32, 32, 48),
+                                       (64, 32, 16, 8)])
 def test_gpu_conv3x3_wgrad_matches_reference(gpu, ci, co, H, W):
     from lattibol_amd import _lib
     rng = np.random.default_rng(ci * 7 + co + H)
@@ -153,7 +154,8 @@ def test_gpu_conv3x3_wgrad_matches_reference(gpu, ci, co, H, W):
     for method in (0, 1, 2):
         dwd = _lib.DeviceArray((co, 9, ci), np.float32)
         rc = _lib.lib.lb_conv3x3_wgrad_bf16(method, gd.ptr, xd.ptr, dwd.ptr, B, H, W, ci, co, None)
-        if (method == 1 and W % 16) or (method == 2 and (W % 32 or H % 8)):
+        tile_ok = (W % 32 == 0 and H % 8 == 0) or (W % 16 == 0 and H % 16 == 0)
+        if (method == 1 and W % 16) or (method == 2 and not tile_ok):
             assert rc == _lib.LB_ERR_UNSUPPORTED
             continue
         _lib.check(rc, "wgrad")
