@@ -354,7 +354,9 @@ int lb_unet_to_fields(lb_unet *u, const float *d_y, int B, const float *d_uin, c
 /* One 3x3 convolution (pad 1) on device NHWC tensors, BF16 matrix cores:
  * mode 0: BF16 in/out, bias + ReLU; mode 1: FP32 in/out, no bias.
  * Modes 2 / 3: the same on the per-wavefront gathering kernel only (0 / 1
- * take the LDS-tile kernel when W % 32 == 0, Ci % 32 == 0 and H % 4 == 0).
+ * take the LDS-tile kernel when Ci % 32 == 0 and W % 32 == 0, H % 4 == 0 or
+ * W % 16 == 0, H % 8 == 0; a K-split tile launch allocates its scratch and
+ * the call then waits for the stream).
  * W is BF16 [Co][9][Ci]; Ci % 16 == 0, Co % 32 == 0. */
 int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B, int H,
                     int W, int Ci, int Co, lb_stream_t stream);

@@ -160,10 +160,14 @@ __global__ __launch_bounds__(256) void k_conv3x3(const TIn *__restrict__ in, con
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution through LDS tiles (W % 32 == 0, Ci % 32 == 0,
-// Co % (32 NB) == 0, H % (4 PM) == 0).  A workgroup computes 4 PM image rows
-// x 32 pixels x 32 NB output channels: wavefront w owns rows w PM .. w PM +
-// PM - 1 of the tile, a 32-pixel row segment per MFMA tile.  Input channels
+// 3x3 convolution through LDS tiles (W % SW == 0, Ci % 32 == 0,
+// Co % (32 NB) == 0, H % (4 PM 32 / SW) == 0).  A workgroup computes 4 PM
+// MFMA pixel tiles x 32 NB output channels; an MFMA pixel tile is 32 / SW
+// image rows x SW pixels (SW = 32: one row segment; SW = 16: two rows), and
+// wavefront w owns tiles w PM .. w PM + PM - 1.  SPLIT: blockIdx.z takes a
+// share of the input-channel stages and writes its FP32 partial tile to
+// part[z][pixel][co] (k_split_sum adds the shares in order, then bias and
+// ReLU), for layers with too few pixels to fill the GPU.  Input channels
 // go in stages of 32: the stage's input window (4 PM + 2 rows x 34 pixels,
 // zero outside the image) and weight block (32 NB channels x 9 taps x 32)
 // are written to LDS once, converted to BF16 on the way (FP32 input of the
@@ -199,11 +203,12 @@ template <> struct Piece8<float> {
     }
 };
 
-template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU>
+template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU, int SW = 32, bool SPLIT = false>
 __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in, const bf16 *__restrict__ W,
-                                                      const float *__restrict__ bias, TOut *__restrict__ out, int H,
-                                                      int Wd, int Ci, int Co) {
-    constexpr int ROWS = 4 * PM, IR = ROWS + 2, IC = 34;
+                                                      const float *__restrict__ bias, TOut *__restrict__ out,
+                                                      float *__restrict__ part, int npix, int H, int Wd, int Ci, int Co) {
+    constexpr int RPT = 32 / SW;                 // image rows per MFMA pixel tile
+    constexpr int ROWS = 4 * PM * RPT, IR = ROWS + 2, IC = SW + 2;
     constexpr int NCO = 32 * NB;
     constexpr int NIN = IR * IC * 4;             // 16-byte pieces of the input window
     constexpr int NW = NCO * 9 * 4;              // 16-byte pieces of the weight block
@@ -212,14 +217,17 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
     __shared__ __attribute__((aligned(16))) uint16_t s_w[NCO * kTileWS];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int row = lane & 31, kh = (lane >> 5) * 8;
-    const int segs = Wd >> 5, rblocks = H / ROWS;
+    const int segs = Wd / SW, rblocks = H / ROWS;
     int t = blockIdx.x;
     const int seg = t % segs;
     t /= segs;
     const int rb = t % rblocks, img = t / rblocks;
-    const int x0 = seg * 32, y0 = rb * ROWS;
+    const int x0 = seg * SW, y0 = rb * ROWS;
     const int n0 = blockIdx.y * NCO;
-    const int nstage = Ci / kTileCK;
+    const int nst_all = Ci / kTileCK;
+    const int sbeg = SPLIT ? (int)(blockIdx.z * nst_all / gridDim.z) : 0;
+    const int send = SPLIT ? (int)((blockIdx.z + 1) * nst_all / gridDim.z) : nst_all;
+    const int ry = row / SW, rx = row % SW;      // this lane's pixel in its MFMA tile
 
     // register prefetch of one stage
     Piece8<TIn> pin[RIN];
@@ -272,12 +280,12 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
         for (int b = 0; b < NB; b++)
             for (int r = 0; r < 16; r++) acc[j][b][r] = 0.f;
 
-    fetch(0);
-    for (int s = 0; s < nstage; s++) {
+    if (sbeg < send) fetch(sbeg * kTileCK);
+    for (int s = sbeg; s < send; s++) {
         __syncthreads();                 // every wavefront is done with the previous stage
         store();
         __syncthreads();
-        if (s + 1 < nstage) fetch((s + 1) * kTileCK);
+        if (s + 1 < send) fetch((s + 1) * kTileCK);
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;
@@ -287,7 +295,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
 #pragma unroll
                 for (int j = 0; j < PM; j++)
                     a[j] = __builtin_bit_cast(
-                        bf16x8, *(const u16x8 *)&s_in[((wave * PM + j + dy) * IC + row + dx) * kTilePS + kk * 16 + kh]);
+                        bf16x8, *(const u16x8 *)&s_in[(((wave * PM + j) * RPT + ry + dy) * IC + rx + dx) * kTilePS +
+                                                      kk * 16 + kh]);
 #pragma unroll
                 for (int b = 0; b < NB; b++)
                     bw[b] = __builtin_bit_cast(
@@ -302,18 +311,44 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
     }
 #pragma unroll
     for (int j = 0; j < PM; j++) {
-        const size_t pix0 = ((size_t)img * H + y0 + wave * PM + j) * Wd + x0;
+        const int ty0 = y0 + (wave * PM + j) * RPT;
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             const int co = n0 + 32 * b + row;
-            const float bv = BIAS_RELU ? bias[co] : 0.f;
+            const float bv = (BIAS_RELU && !SPLIT) ? bias[co] : 0.f;
             for (int r = 0; r < 16; r++) {
                 const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const size_t pix = ((size_t)img * H + ty0 + i / SW) * Wd + x0 + i % SW;
                 float v = acc[j][b][r];
-                if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
-                out[(pix0 + i) * Co + co] = from_f<TOut>(v);
+                if (SPLIT) {
+                    part[((size_t)blockIdx.z * npix + pix) * Co + co] = v;
+                } else {
+                    if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
+                    out[pix * Co + co] = from_f<TOut>(v);
+                }
             }
         }
+    }
+}
+
+// Sum of the K-split shares of k_conv3x3_tile in order, then bias + ReLU
+// (BIAS_RELU) and the output type; four channels per thread.
+template <class TOut, bool BIAS_RELU>
+__global__ void k_split_sum(const float *__restrict__ part, int ks, size_t n, int Co, const float *__restrict__ bias,
+                            TOut *__restrict__ out) {
+    const size_t e = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+    if (e >= n) return;
+    float4 v = *(const float4 *)(part + e);
+    for (int z = 1; z < ks; z++) {
+        const float4 w = *(const float4 *)(part + (size_t)z * n + e);
+        v.x += w.x; v.y += w.y; v.z += w.z; v.w += w.w;
+    }
+    float r[4] = {v.x, v.y, v.z, v.w};
+    const int co = (int)(e % Co);
+    for (int k = 0; k < 4; k++) {
+        float x = r[k];
+        if (BIAS_RELU) { x += bias[co + k]; x = x > 0.f ? x : 0.f; }
+        out[e + k] = from_f<TOut>(x);
     }
 }
 
@@ -487,9 +522,10 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_cm(const bf16 *__restrict
 // consecutive pixels of one channel) is one 16-byte LDS read.  Wavefront w
 // takes a quarter of the tile's rows (four 16-pixel K-steps) and keeps all
 // nine taps' 32 x 32 accumulators; the column shift of a tap is a funnel
-// shift across an aligned 16-byte and 4-byte read of the window row.  The
-// next tile's global loads are issued into registers before the MFMAs of
-// the current one.  The four wavefronts' tiles are added in LDS in a fixed
+// shift across an aligned 16-byte and 4-byte read of the window row.  Two
+// workgroups share a CU (at most 256 registers per lane), so one stages its
+// next tile while the other runs its MFMAs.  The four wavefronts' tiles are
+// added in LDS in a fixed
 // order and written as the chunk's partial (partial[chunk][co][tap][ci]).
 // ---------------------------------------------------------------------------
 constexpr int kWgXS = 520;      // LDS stride of one input channel (<= 18 window rows; 260 dwords = 4 mod 64 banks)
@@ -501,7 +537,7 @@ __device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint1
 __device__ __forceinline__ float lane_of(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
 template <int TW>
-__global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restrict__ dout, const bf16 *__restrict__ in,
+__global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const float *__restrict__ dout, const bf16 *__restrict__ in,
                                                             float *__restrict__ partial, int H, int Wd, int Ci, int Co,
                                                             int ntiles, int tpw) {
     __shared__ __attribute__((aligned(16))) uint16_t sX[32 * kWgXS];
@@ -583,12 +619,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
 #pragma unroll
     for (int t = 0; t < 9; t++)
         for (int r = 0; r < 16; r++) acc[t][r] = 0.f;
-    if (t0 < t1) fetch(t0);
     for (int t = t0; t < t1; t++) {
         __syncthreads();                 // every wavefront is done with the previous tile
+        fetch(t);
         store();
         __syncthreads();
-        if (t + 1 < t1) fetch(t + 1);
 #pragma unroll
         for (int s = 0; s < 4; s++) {
             const int trow = wave * (4 / KPR) + s / KPR, pxb = (s % KPR) * 16 + kh;
@@ -627,22 +662,27 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad_tile(const float *__restr
     }
 }
 
-// Sum of partials over chunks (fixed order) -> gradient.
-__global__ void k_sum_chunks(const float *partial, int nchunk, size_t n, float *out) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    float s = 0.f;
-    for (int c = 0; c < nchunk; c++) s += partial[(size_t)c * n + e];
-    out[e] = s;
-}
 
-// Fixed-order sum of many chunks in one pass: thread (element el of a
-// 16-element group, phase ph of 16) adds chunks ph, ph + 16, ph + 32, ... in
-// order; the 16 phase sums are then added in LDS in phase order.
-__global__ __launch_bounds__(256) void k_sum_chunks16(const float *__restrict__ partial, int nchunk, size_t n,
-                                                      float *__restrict__ out) {
+// Fixed-order sum of many chunks in one pass, for one or two independent
+// arrays (segment a: blocks [0, nblk_a), segment b: the rest): thread
+// (element el of a 16-element group, phase ph of 16) adds chunks ph, ph + 16,
+// ph + 32, ... in order; the 16 phase sums are then added in LDS in phase
+// order (for nchunk <= 16 this is the plain in-order sum).
+struct SumSeg {
+    const float *partial;
+    int nchunk;
+    size_t n;
+    float *out;
+};
+__global__ __launch_bounds__(256) void k_sum_chunks16(SumSeg a, SumSeg b, unsigned nblk_a) {
+    const bool first = blockIdx.x < nblk_a;
+    const float *partial = first ? a.partial : b.partial;
+    const int nchunk = first ? a.nchunk : b.nchunk;
+    const size_t n = first ? a.n : b.n;
+    float *out = first ? a.out : b.out;
+    const unsigned blk = first ? blockIdx.x : blockIdx.x - nblk_a;
     const int el = threadIdx.x & 15, ph = threadIdx.x >> 4;
-    const size_t e = (size_t)blockIdx.x * 16 + el;
+    const size_t e = (size_t)blk * 16 + el;
     float s = 0.f;
     if (e < n) {
 #pragma unroll 8
@@ -808,34 +848,42 @@ __global__ void k_head_bwd_data(const float *dy, const bf16 *Wh, float *dx, int 
     }
     *(float4 *)(dx + (size_t)m * C + c) = float4{r[0], r[1], r[2], r[3]};
 }
-// dWh[o, c] and dbh[o] partials of one pixel chunk (O <= 8): thread
-// (channel, pixel phase) accumulates all O outputs for its channel (and the
-// bias sums), phases added in LDS in a fixed order; partial[chunk][o][c],
-// with the bias at c = C; k_sum_chunks adds the chunks.
+// dWh[o, c] and dbh[o] partials of one pixel chunk (C % 8 == 0, C <= 256,
+// O <= 8): thread (8-channel group, pixel phase) loads 16 bytes of the input
+// per pixel and accumulates all O outputs for its channels (and the bias
+// sums); the phases are added in LDS in a fixed order; partial[chunk][o][c],
+// with the bias at c = C; k_sum_chunks16 adds the chunks.
 __global__ __launch_bounds__(256) void k_head_bwd_w(const float *dy, const bf16 *x, int npix, int C, int O, int chunk,
                                                     float *partial) {
-    const int nph = 256 / C;
-    const int t = threadIdx.x, c = t % C, ph = t / C;
+    const int tpp = C >> 3, nph = 256 / tpp;
+    const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
     const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
-    float acc[8], accb[8];
-    for (int o = 0; o < 8; o++) acc[o] = accb[o] = 0.f;
+    float acc[8][8], accb[8];
+    for (int o = 0; o < 8; o++) {
+        accb[o] = 0.f;
+        for (int k = 0; k < 8; k++) acc[o][k] = 0.f;
+    }
     if (ph < nph)
-#pragma unroll 4
+#pragma unroll 2
         for (int m = pbeg + ph; m < pend; m += nph) {
-            const float xv = from_bf16(x[(size_t)m * C + c]);
-            for (int o = 0; o < O; o++) {
-                const float g = dy[(size_t)m * O + o];
-                acc[o] += g * xv;
-                accb[o] += g;
-            }
+            const u16x8 v = *(const u16x8 *)(x + (size_t)m * C + 8 * cg);
+            float xv[8];
+            for (int k = 0; k < 8; k++) xv[k] = lb::bf16_to_f32(v[k]);
+            for (int o = 0; o < 8; o++)
+                if (o < O) {
+                    const float g = dy[(size_t)m * O + o];
+                    for (int k = 0; k < 8; k++) acc[o][k] += g * xv[k];
+                    accb[o] += g;
+                }
         }
-    __shared__ float sh[256 * 9];
+    __shared__ float sh[64 * 8 * 33];            // nph x O x (C + 1) <= 2048 / C x 8 x (C + 1)
     const int np = O * (C + 1);
     if (ph < nph)
-        for (int o = 0; o < O; o++) {
-            sh[ph * np + o * (C + 1) + c] = acc[o];
-            if (c == 0) sh[ph * np + o * (C + 1) + C] = accb[o];
-        }
+        for (int o = 0; o < 8; o++)
+            if (o < O) {
+                for (int k = 0; k < 8; k++) sh[ph * np + o * (C + 1) + 8 * cg + k] = acc[o][k];
+                if (cg == 0) sh[ph * np + o * (C + 1) + C] = accb[o];
+            }
     __syncthreads();
     for (int e = t; e < np; e += 256) {
         float v = 0.f;
@@ -999,7 +1047,9 @@ struct lb_unet {
     float *gA = nullptr, *gB = nullptr, *gC = nullptr;   // gradient scratch (largest tensor)
     bf16 *tA = nullptr, *tB = nullptr;                   // channel-major copies for the weight gradient
     std::vector<float *> gskip;    // gradient of each encoder output (skip)
-    float *partial = nullptr;      // chunk partials of the weight, bias and head gradients
+    float *partial = nullptr;      // chunk partials of the weight and head gradients, K-split conv shares
+    float *bpart = nullptr;        // chunk partials of a bias gradient
+    size_t bpart_n = 0;
     size_t partial_n = 0;
     double *lpart = nullptr, *lsum = nullptr;
     std::vector<void *> allocs;
@@ -1028,33 +1078,92 @@ inline int enc_conv(int l) { return 2 * l; }
 // Pixels per chunk of the head's weight-gradient partials.
 constexpr int kHeadChunk = 512;
 
-// Fixed-order sum of nchunk partial arrays of n floats: one thread per
-// element for a few chunks, else 16 phases per element (the long serial
-// loops of one thread per element leave most of the GPU idle when n is small).
+// Fixed-order sum of nchunk partial arrays of n floats (and, in the same
+// launch, of a second set when b.n > 0).
+void sum_chunks2(lb_unet *u, SumSeg a, SumSeg b) {
+    const unsigned na = (unsigned)((a.n + 15) / 16), nb = (unsigned)((b.n + 15) / 16);
+    if (na + nb == 0) return;
+    hipLaunchKernelGGL(k_sum_chunks16, dim3(na + nb), dim3(256), 0, u->stream, a, b, na);
+}
 void sum_chunks(lb_unet *u, const float *partial, int nchunk, size_t n, float *out) {
-    if (nchunk <= 8)
-        hipLaunchKernelGGL(k_sum_chunks, dim3(nb256(n)), dim3(256), 0, u->stream, partial, nchunk, n, out);
-    else
-        hipLaunchKernelGGL(k_sum_chunks16, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, u->stream, partial, nchunk, n,
-                           out);
+    sum_chunks2(u, SumSeg{partial, nchunk, n, out}, SumSeg{nullptr, 0, 0, nullptr});
 }
 
+// Launch plan of the LDS-tile conv: segment width (32, 16; 0 = shape not
+// supported), MFMA tiles per wavefront, 32-channel groups per workgroup and
+// K-split shares.  32-pixel segments take the largest tile that still gives
+// at least two workgroups per CU; 16-pixel segments (small images) split the
+// input-channel stages over up to enough workgroups to fill the GPU.
+struct ConvPlan { int sw, pm, nb, ks; };
+inline ConvPlan conv_plan(int npix, int H, int Wd, int Ci, int Co) {
+    if (Ci % kTileCK || Co % 32 || H < 1 || Wd < 1) return {0, 0, 0, 0};
+    const int B = npix / (H * Wd);
+    const int nb = Co % 64 == 0 ? 2 : 1;
+    if (Wd % 32 == 0 && H % 4 == 0) {
+        const int pm = (H % 8 == 0 && (size_t)B * (H / 8) * (Wd / 32) * (Co / (32 * nb)) >= 512) ? 2 : 1;
+        return {32, pm, nb, 1};
+    }
+    if (Wd % 16 == 0 && H % 8 == 0) {
+        const long wgs = (long)B * (H / 8) * (Wd / 16) * (Co / (32 * nb));
+        const int nst = Ci / kTileCK;
+        int ks = 1;
+        while (ks * 2 <= nst && wgs * ks < 512) ks *= 2;
+        return {16, 1, nb, ks};
+    }
+    return {0, 0, 0, 0};
+}
+// FP32 scratch of a K-split conv (elements).
+inline size_t conv_split_scratch(int npix, int H, int Wd, int Ci, int Co) {
+    const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
+    return pl.ks > 1 ? (size_t)pl.ks * npix * Co : 0;
+}
+
+// LDS-tile launch for eligible shapes; returns false (nothing launched) when
+// the shape does not fit the tile kernel or a K-split finds no scratch
+// (u->partial, u->partial_n elements).
 template <class TIn, class TOut, bool BR>
 bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
                       int Ci, int Co) {
-    if (Wd % 32 || Ci % kTileCK || Co % 32 || H % 4) return false;
+    const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
+    if (!pl.sw) return false;
+    float *part = nullptr;
+    if (pl.ks > 1) {
+        if (!u->partial || u->partial_n < (size_t)pl.ks * npix * Co) return false;
+        part = u->partial;
+    }
     const int B = npix / (H * Wd);
-    const int NB = Co % 64 == 0 ? 2 : 1;
-    const int PM = (H % 8 == 0 && (size_t)B * (H / 8) * (Wd / 32) * (Co / (32 * NB)) >= 512) ? 2 : 1;
-    dim3 grd((unsigned)(B * (H / (4 * PM)) * (Wd / 32)), (unsigned)(Co / (32 * NB))), blk(256);
-    if (NB == 2 && PM == 2)
-        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
-    else if (NB == 2)
-        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
-    else if (PM == 2)
-        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
-    else
-        hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, H, Wd, Ci, Co);
+    const int rows = 4 * pl.pm * (32 / pl.sw);
+    const dim3 grd((unsigned)(B * (H / rows) * (Wd / pl.sw)), (unsigned)(Co / (32 * pl.nb)), (unsigned)pl.ks), blk(256);
+    if (pl.sw == 32) {
+        if (pl.nb == 2 && pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+                               Wd, Ci, Co);
+        else if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+                               Wd, Ci, Co);
+        else if (pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+                               Wd, Ci, Co);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+                               Wd, Ci, Co);
+    } else if (pl.ks > 1) {
+        if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16, true>), grd, blk, 0, u->stream, in, W, bias, out, part,
+                               npix, H, Wd, Ci, Co);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16, true>), grd, blk, 0, u->stream, in, W, bias, out, part,
+                               npix, H, Wd, Ci, Co);
+        const size_t n = (size_t)npix * Co;
+        hipLaunchKernelGGL((k_split_sum<TOut, BR>), dim3(nb256(n / 4)), blk, 0, u->stream, part, pl.ks, n, Co, bias, out);
+    } else {
+        if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16>), grd, blk, 0, u->stream, in, W, bias, out, part, npix,
+                               H, Wd, Ci, Co);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16>), grd, blk, 0, u->stream, in, W, bias, out, part, npix,
+                               H, Wd, Ci, Co);
+    }
     return true;
 }
 
@@ -1085,7 +1194,8 @@ void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TO
 }
 
 // Pixel-tile chunks of the LDS-tile weight gradient: tiles per workgroup
-// chosen for about 768 workgroups; 0 when the shape does not fit the kernel.
+// chosen for about 512 workgroups (two per CU); 0 when the shape does not fit
+// the kernel.
 struct WgradTiles { int ntiles, tpw, nchunk, tw; };
 inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
     if (Co % 32 || Ci % 8) return {0, 0, 0, 0};
@@ -1093,7 +1203,7 @@ inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
     if (!tw) return {0, 0, 0, 0};
     const int ntiles = npix / 256;
     const long base = (long)(Co / 32) * ((Ci + 31) / 32);
-    const int tpw = (int)std::max(1L, (ntiles * base + 767) / 768);
+    const int tpw = (int)std::max(1L, (ntiles * base + 511) / 512);
     return {ntiles, tpw, (ntiles + tpw - 1) / tpw, tw};
 }
 // Chunk count of the per-wavefront weight-gradient kernels (1,024 pixels each).
@@ -1105,9 +1215,11 @@ inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
 // tB >= npix Ci elements), both operands are transposed first and the
 // vector-load kernel runs (method 1); otherwise the gathering kernel
 // (method 0).
+// extra: a second chunk sum (the bias gradient) done in the same launch as
+// the weight gradient's.
 template <class TIn>
 void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
-                  bf16 *tA, bf16 *tB, int method = -1) {
+                  bf16 *tA, bf16 *tB, int method = -1, SumSeg extra = SumSeg{nullptr, 0, 0, nullptr}) {
     const int K = 9 * Ci;
     const WgradTiles wt = wgrad_tiles(npix, H, Wd, Ci, Co);
     if ((method < 0 || method == 2) && wt.ntiles) {
@@ -1118,7 +1230,7 @@ void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int n
         else
             hipLaunchKernelGGL(k_conv3x3_wgrad_tile<16>, grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
                                Wd, Ci, Co, wt.ntiles, wt.tpw);
-        sum_chunks(u, u->partial, wt.nchunk, (size_t)Co * K, dW);
+        sum_chunks2(u, SumSeg{u->partial, wt.nchunk, (size_t)Co * K, dW}, extra);
         return;
     }
     const int chunk = 1024;
@@ -1135,7 +1247,7 @@ void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int n
         hipLaunchKernelGGL((k_conv3x3_wgrad<TIn>), grd, dim3(256), 0, u->stream, dout, in, u->partial, npix, H, Wd, Ci, Co,
                            chunk);
     }
-    sum_chunks(u, u->partial, nchunk, (size_t)Co * K, dW);
+    sum_chunks2(u, SumSeg{u->partial, nchunk, (size_t)Co * K, dW}, extra);
 }
 
 std::vector<Level> levels(const lb_unet *u) {
@@ -1258,7 +1370,10 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         size_t nchunk = std::max((size_t)wgrad_chunks_legacy((int)npix), (size_t)wt.nchunk);
         const size_t nw = (size_t)u->convs[ci].co * 9 * u->convs[ci].ci;
         u->partial_n = std::max(u->partial_n, nchunk * nw);
-        u->partial_n = std::max(u->partial_n, (size_t)bias_chunks((int)npix).nchunk * u->convs[ci].co);
+        u->bpart_n = std::max(u->bpart_n, (size_t)bias_chunks((int)npix).nchunk * u->convs[ci].co);
+        // K-split scratch of the forward and input-gradient convs
+        u->partial_n = std::max(u->partial_n, conv_split_scratch((int)npix, L[l].H, L[l].W, u->convs[ci].ci, u->convs[ci].co));
+        u->partial_n = std::max(u->partial_n, conv_split_scratch((int)npix, L[l].H, L[l].W, u->convs[ci].co, u->convs[ci].ci));
     }
     {
         // head: chunk partials followed by their sum
@@ -1267,11 +1382,12 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         u->partial_n = std::max(u->partial_n, (nchunk + 1) * desc->out_ch * (L[0].C + 1));
     }
     u->partial = u->alloc<float>(u->partial_n);
+    u->bpart = u->alloc<float>(u->bpart_n);
     u->lpart = u->alloc<double>(nb256((size_t)B * L[0].H * L[0].W * desc->out_ch));
     u->lsum = u->alloc<double>(2);
     bool ok = !rc && u->w && u->g && u->m && u->v && u->wb && u->wt && u->flip_tab && u->gA && u->gB && u->gC && u->tA && u->tB &&
               u->xin && u->y &&
-              u->dy && u->partial && u->lpart && u->lsum;
+              u->dy && u->partial && u->bpart && u->lpart && u->lsum;
     for (auto p : u->act) ok = ok && p;
     for (auto p : u->pool) ok = ok && p;
     for (auto p : u->cat) ok = ok && p;
@@ -1407,15 +1523,13 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
-        {
-            // ReLU backward + bias-gradient partials (the weight-gradient
-            // partial buffer is free until wgrad_launch below)
-            const BiasChunks bc = bias_chunks(npix);
-            hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
-                               c.co, bc.chunk, u->partial);
-            sum_chunks(u, u->partial, bc.nchunk, (size_t)c.co, u->g + c.boff);
-        }
-        wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB);
+        // ReLU backward + bias-gradient partials, summed in the weight
+        // gradient's chunk-sum launch
+        const BiasChunks bc = bias_chunks(npix);
+        hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
+                           c.co, bc.chunk, u->bpart);
+        wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
+                           SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff});
         if (need_din)
             conv_launch<float, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
         std::swap(gcur, gtmp);
@@ -1490,14 +1604,23 @@ extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, cons
     tmp.stream = (hipStream_t)stream;
     const int npix = B * H * W;
     const bool gather = mode >= 2;
+    // a K-split plan needs scratch: allocated here (and the call then waits)
+    const size_t scratch = gather ? 0 : conv_split_scratch(npix, H, W, Ci, Co);
+    if (scratch) {
+        tmp.partial = tmp.alloc<float>(scratch);
+        tmp.partial_n = scratch;
+        if (!tmp.partial) { tmp.stream = nullptr; return LB_ERR_INVALID; }
+    }
     if (mode % 2 == 0)
         conv_launch<bf16, bf16, true>(&tmp, (const bf16 *)d_in, (const bf16 *)d_w, d_bias, (bf16 *)d_out, npix, H, W, Ci,
                                       Co, gather);
     else
         conv_launch<float, float, false>(&tmp, (const float *)d_in, (const bf16 *)d_w, nullptr, (float *)d_out, npix, H, W,
                                          Ci, Co, gather);
+    int rc = (int)hipGetLastError();
+    if (!rc && scratch) rc = (int)hipStreamSynchronize(tmp.stream);
     tmp.stream = nullptr;
-    return (int)hipGetLastError();
+    return rc;
 }
 
 // Weight gradient of one 3x3 convolution (test and diagnostic entry point;

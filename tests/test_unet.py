@@ -79,12 +79,15 @@ def _bf16_round(a):
     return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float32).numpy()
 
 
-# (ci, co, B, H, W): W = 16 runs the gathering kernel only; W % 32 == 0 with
-# ci % 32 == 0 takes the LDS-tile kernel in modes 0 / 1 (all four tile shapes:
-# 4 or 8 rows x 32 or 64 channels), compared with the gathering kernel too.
+# (ci, co, B, H, W): ci % 32 == 0 takes the LDS-tile kernel in modes 0 / 1,
+# compared with the gathering kernel (modes 2 / 3) too: W % 32 == 0 in all
+# four tile shapes (4 or 8 rows x 32 or 64 channels); W % 16 == 0 with
+# 16-pixel segments, K split over 2, 8 (uneven shares of 3 / 12 stages) or
+# no workgroups; ci = 16 runs the gathering kernel only.
 CONV_CASES = [(16, 32, 2, 16, 16), (32, 64, 2, 16, 16), (96, 32, 2, 16, 16), (384, 128, 2, 16, 16),
               (32, 64, 2, 8, 64), (96, 32, 1, 16, 32), (384, 128, 1, 4, 32), (64, 64, 3, 12, 32),
-              (192, 64, 16, 32, 32), (32, 32, 4, 128, 128)]
+              (192, 64, 16, 32, 32), (32, 32, 4, 128, 128), (256, 256, 16, 16, 16), (128, 256, 16, 16, 16),
+              (256, 128, 16, 16, 16), (64, 96, 2, 8, 48)]
 
 
 @pytest.mark.gpu
