@@ -178,6 +178,10 @@ def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128,
         step_ratio=float(warm["steps"][both].sum() / max(plain["steps"][both].sum(), 1)),
         wall_plain=plain["wall"], wall_warm=warm["wall"], wall_ratio=float(warm["wall"] / plain["wall"]),
         inference_s=warm["t_init"],
+        # channels steady from rest that did not reach tol from the warm start:
+        # residual +inf = retired by the divergence guard, finite = step cap
+        warm_not_converged=[dict(index=int(i), steps=int(warm["steps"][i]), residual=float(warm["res"][i]))
+                            for i in np.where((plain["res"] < tol) & ~(warm["res"] < tol))[0]],
         l2_max=float(l2[both].max()) if both.any() else None, l2_median=float(np.median(l2[both])) if both.any() else None,
         pred_err_median=float(np.median(pred_err[both])) if both.any() else None,
         per_lattice=dict(steps_plain=plain["steps"].tolist(), steps_warm=warm["steps"].tolist(),
