@@ -245,7 +245,11 @@ def bench_ensemble(args):
 
 
 def bench_warmstart(args):
-    """C5: U-Net warm start vs plain LBM from rest, time to residual < 1e-6."""
+    """C5: U-Net warm start vs plain LBM from rest, time to residual < 1e-6.
+    value: wall-time ratio of the warm-start pipeline with its plain fallback
+    (channels the prediction does not bring to the tolerance are run again
+    from rest) over the plain runs, on every channel that is steady from
+    rest; evaluation.wall_ratio is the warm start alone."""
     from lattibol_amd import warmstart
     from lattibol_amd.unet import UNet
     t0 = time.perf_counter()
@@ -257,7 +261,8 @@ def bench_warmstart(args):
     ev = warmstart.evaluate(net, n=args.heldout, seed=777, max_steps=args.max_steady_steps, init_mode=0,
                             extra_modes=(1,))
     ev.pop("per_lattice")
-    return common("time-to-steady-state, warm-start / plain LBM (wall-time ratio)", round(ev["wall_ratio"], 4),
+    return common("time-to-steady-state, warm-start / plain LBM (wall-time ratio)",
+                  round(ev["with_fallback"]["wall_ratio"], 4),
                   "ratio", args, 0, None, "fp32 LBM, bf16 U-Net",
                   {"workload": "D2Q9 channels 128x128 (1-4 circles, u_in in [0.02,0.08], tau in [0.52,0.9]); "
                                "4-level 32-channel U-Net trained on the batched solver's steady fields; "

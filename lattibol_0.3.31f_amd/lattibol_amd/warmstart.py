@@ -15,9 +15,11 @@ from .batch import Batch
 from .unet import UNet, make_inputs, make_targets
 
 
-def solve_ensemble(specs, tol=1e-6, k=100, max_steps=300000, unet=None, init_mode=0):
+def solve_ensemble(specs, tol=1e-6, k=100, max_steps=300000, unet=None, init_mode=0, fallback_plain=True):
     """Run every channel to steady state; returns per-lattice steps, residuals,
-    macros and the wall time (including an optional U-Net inference)."""
+    macros and the wall time (including an optional U-Net inference and, with
+    fallback_plain, the plain runs from rest of channels the warm start did
+    not bring to tol)."""
     b = Batch(specs)
     t0 = time.perf_counter()
     if unet is not None:
@@ -44,10 +46,24 @@ def solve_ensemble(specs, tol=1e-6, k=100, max_steps=300000, unet=None, init_mod
         b.init_fields_device(fields.ptr.value, init_mode)
     t_init = time.perf_counter() - t0
     steps, res = b.run_to_steady(tol=tol, k=k, max_steps=max_steps)
-    wall = time.perf_counter() - t0
     macros = [b.macros(i) for i in range(len(specs))]
     b.close()
-    return dict(steps=steps, res=res, macros=macros, wall=wall, t_init=t_init)
+    fallback = []
+    if unet is not None and fallback_plain:
+        # the plain LBM is the fallback: a channel that did not reach tol from
+        # the predicted field (diverged, or still moving at max_steps) is run
+        # again from rest; its steps and wall time count for the warm start
+        fallback = [int(i) for i in np.where(~(res < tol))[0]]
+        if fallback:
+            again = solve_ensemble([specs[i] for i in fallback], tol=tol, k=k, max_steps=max_steps)
+            steps = np.array(steps, copy=True)
+            res = np.array(res, copy=True)
+            for j, i in enumerate(fallback):
+                steps[i] += again["steps"][j]
+                res[i] = again["res"][j]
+                macros[i] = again["macros"][j]
+    wall = time.perf_counter() - t0
+    return dict(steps=steps, res=res, macros=macros, wall=wall, t_init=t_init, fallback=fallback)
 
 
 class _Offset:
@@ -148,7 +164,9 @@ def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128,
     specs = [specs_all[i] for i in steady]
     m = len(specs)
     plain = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps)
-    warm = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=init_mode)
+    warm = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=init_mode,
+                          fallback_plain=False)
+    warm_fb = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=init_mode)
     both = (plain["res"] < tol) & (warm["res"] < tol)
     l2 = np.array([velocity_l2(warm["macros"][i], plain["macros"][i], 1 - specs[i]["solid"]) for i in range(m)])
     # prediction error of the network itself (before relaxation)
@@ -162,7 +180,7 @@ def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128,
     pred_err = np.array(pred_err)
     extra = {}
     for mode in extra_modes:
-        w2 = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=mode)
+        w2 = solve_ensemble(specs, tol=tol, k=k, max_steps=max_steps, unet=net, init_mode=mode, fallback_plain=False)
         b2 = (plain["res"] < tol) & (w2["res"] < tol)
         l2b = np.array([velocity_l2(w2["macros"][i], plain["macros"][i], 1 - specs[i]["solid"]) for i in range(m)])
         extra["init_mode_%d" % mode] = dict(
@@ -182,6 +200,11 @@ def evaluate(net, n=256, seed=777, tol=1e-6, k=100, max_steps=300000, size=(128,
         # residual +inf = retired by the divergence guard, finite = step cap
         warm_not_converged=[dict(index=int(i), steps=int(warm["steps"][i]), residual=float(warm["res"][i]))
                             for i in np.where((plain["res"] < tol) & ~(warm["res"] < tol))[0]],
+        # the warm start with the plain fallback, over every steady channel
+        with_fallback=dict(fallback=len(warm_fb["fallback"]), converged=int((warm_fb["res"] < tol).sum()),
+                           step_ratio=float(warm_fb["steps"][plain["res"] < tol].sum() /
+                                            max(plain["steps"][plain["res"] < tol].sum(), 1)),
+                           wall_warm=warm_fb["wall"], wall_ratio=float(warm_fb["wall"] / plain["wall"])),
         l2_max=float(l2[both].max()) if both.any() else None, l2_median=float(np.median(l2[both])) if both.any() else None,
         pred_err_median=float(np.median(pred_err[both])) if both.any() else None,
         per_lattice=dict(steps_plain=plain["steps"].tolist(), steps_warm=warm["steps"].tolist(),
