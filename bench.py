@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--heldout", type=int, default=256)
     ap.add_argument("--max-steady-steps", type=int, default=300000)
+    ap.add_argument("--spinodal-storages", default="f64,f32,bf16,f16,bf16_sr,f16_sr",
+                    help="storages of the spinodal workload (comma list; f64 is the reference of the error columns)")
     return ap.parse_args()
 
 
@@ -142,7 +144,11 @@ def bench_spinodal(args):
     steps = args.steps
     out = {}
     fields = {}
-    runs = (("f64", 0), ("f32", 0), ("bf16", 0), ("f16", 0), ("bf16", 1), ("f16", 1))
+    want = set(args.spinodal_storages.split(","))
+    runs = [(st, rm) for st, rm in (("f64", 0), ("f32", 0), ("bf16", 0), ("f16", 0), ("bf16", 1), ("f16", 1))
+            if st + ("_sr" if rm else "") in want]
+    if "f32" not in want:
+        raise SystemExit("the spinodal workload reports the f32 run; include f32 in --spinodal-storages")
     for st, rm in runs:
         key = st + ("_sr" if rm else "")
         cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm)
@@ -165,9 +171,9 @@ def bench_spinodal(args):
         log("%s: %.1f MLUPS, drift %.2e, rho p5/p95 %.4f %.4f" % (key, out[key]["mlups"], out[key]["mass_drift_rel"],
                                                                    p5, p95))
         sol.close()
-    ref = fields["f64"]
+    ref = fields.get("f64")
     for key in out:
-        if key == "f64":
+        if key == "f64" or ref is None:
             continue
         d = fields[key] - ref
         out[key]["rho_err_l2_rel_vs_fp64"] = float(np.linalg.norm(d) / np.linalg.norm(ref))

@@ -360,7 +360,10 @@ struct lb_ctx {
     int mem = 0;                      // population load/store hints (see ldm/stm)
     int layout = LB_LAYOUT_SOA;       // population layout (LB_LAYOUT_*)
     int tb = 0;                       // steps per temporally blocked launch (0: off)
-    void *dargs = nullptr;            // two argument blocks (source buffer 0 / 1)
+    // launch geometry and load hints of the Shan-Chen density pass, tuned on
+    // their own (its traffic is 19 loads and one store per node)
+    int dbx_shift = 6, dby = 4, dxcd = 0, dmem = 0;
+    void *dargs = nullptr;            // argument blocks: sweep from buffer 0 / 1, density pass from 0 / 1
     static constexpr size_t kArgBytes = 512;
 
     ~lb_ctx() {
@@ -380,7 +383,9 @@ namespace {
 
 int ilog2(int v) { int s = 0; while ((1 << s) < v) s++; return s; }
 
-template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
+// Argument block of the sweep (dens = false) or of the density pass from
+// buffer src; the two differ only in launch geometry.
+template <class S> StepArgs<S> make_args(const lb_ctx *c, int src, bool dens = false) {
     StepArgs<S> a{};
     a.A = (const typename S::raw *)c->buf[src];
     a.B = (typename S::raw *)c->buf[1 - src];
@@ -412,22 +417,23 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src) {
     a.ctr = c->ctr;
     a.halo = c->cfg.halo_z && c->Q == 19;
     a.zh_ux = c->cfg.zh_ux_in; a.zh_rho = c->cfg.zh_rho_out;
-    a.bx_shift = c->bx_shift; a.by = c->by;
-    a.ntx = (c->cfg.nx + (1 << c->bx_shift) - 1) >> c->bx_shift;
-    a.nty = (c->cfg.ny + c->by - 1) / c->by;
+    a.bx_shift = dens ? c->dbx_shift : c->bx_shift;
+    a.by = dens ? c->dby : c->by;
+    a.ntx = (c->cfg.nx + (1 << a.bx_shift) - 1) >> a.bx_shift;
+    a.nty = (c->cfg.ny + a.by - 1) / a.by;
     a.nblocks = a.ntx * a.nty * c->cfg.nz;
-    a.xcd = c->xcd;
-    if (c->Q == 9) fast_path_args<9>(a, 1 << c->bx_shift, c->by);
-    else fast_path_args<19>(a, 1 << c->bx_shift, c->by);
+    a.xcd = dens ? c->dxcd : c->xcd;
+    if (c->Q == 9) fast_path_args<9>(a, 1 << a.bx_shift, a.by);
+    else fast_path_args<19>(a, 1 << a.bx_shift, a.by);
     return a;
 }
 
 template <class S> int upload_args_t(lb_ctx *c) {
     static_assert(sizeof(StepArgs<S>) <= lb_ctx::kArgBytes, "argument block size");
-    StepArgs<S> h[2] = {make_args<S>(c, 0), make_args<S>(c, 1)};
+    StepArgs<S> h[4] = {make_args<S>(c, 0), make_args<S>(c, 1), make_args<S>(c, 0, true), make_args<S>(c, 1, true)};
     int rc = (int)hipStreamSynchronize(c->stream);
-    if (!rc) rc = (int)hipMemcpy(c->dargs, &h[0], sizeof(StepArgs<S>), hipMemcpyHostToDevice);
-    if (!rc) rc = (int)hipMemcpy((char *)c->dargs + lb_ctx::kArgBytes, &h[1], sizeof(StepArgs<S>), hipMemcpyHostToDevice);
+    for (int k = 0; k < 4 && !rc; k++)
+        rc = (int)hipMemcpy((char *)c->dargs + k * lb_ctx::kArgBytes, &h[k], sizeof(StepArgs<S>), hipMemcpyHostToDevice);
     return rc;
 }
 
@@ -442,6 +448,14 @@ int upload_args(lb_ctx *c) {
     }
 }
 
+// The Shan-Chen density pass from buffer src (psi of every node).
+template <int Q, class S> void launch_density_t(lb_ctx *c, int src) {
+    const StepArgs<S> a = make_args<S>(c, src, true);   // host copy: launch geometry only
+    const StepArgs<S> *pd = (const StepArgs<S> *)((const char *)c->dargs + (2 + src) * lb_ctx::kArgBytes);
+    if (c->dmem == 2) hipLaunchKernelGGL((k_density<Q, S, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pd);
+    else hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pd);
+}
+
 template <int Q, class S>
 int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     StepArgs<S> a = make_args<S>(c, src);   // host copy: launch geometry only
@@ -450,8 +464,7 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     size_t lds = 0;
     if (sc) {
         lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
-        if (c->mem == 2) hipLaunchKernelGGL((k_density<Q, S, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
-        else hipLaunchKernelGGL((k_density<Q, S>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa);
+        launch_density_t<Q, S>(c, src);
     }
     if (sc) {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
@@ -486,6 +499,17 @@ template <int Q> int launch_step_q(lb_ctx *c, int src, bool macros, uint32_t dst
 // between sweeps instead (see lb_halo_buffers).
 int launch_step(lb_ctx *c, int src, bool macros, uint32_t dstep = 0) {
     return c->Q == 9 ? launch_step_q<9>(c, src, macros, dstep) : launch_step_q<19>(c, src, macros, dstep);
+}
+
+void launch_density(lb_ctx *c, int src) {
+    const bool q19 = c->Q == 19;
+    switch (c->cfg.storage) {
+    case LB_STORE_F32: q19 ? launch_density_t<19, StF32>(c, src) : launch_density_t<9, StF32>(c, src); break;
+    case LB_STORE_F32S: q19 ? launch_density_t<19, StF32S>(c, src) : launch_density_t<9, StF32S>(c, src); break;
+    case LB_STORE_F16: q19 ? launch_density_t<19, StF16>(c, src) : launch_density_t<9, StF16>(c, src); break;
+    case LB_STORE_BF16: q19 ? launch_density_t<19, StBF16>(c, src) : launch_density_t<9, StBF16>(c, src); break;
+    default: q19 ? launch_density_t<19, StF64>(c, src) : launch_density_t<9, StF64>(c, src); break;
+    }
 }
 
 // After n sweeps launched with dstep 0..n-1: advance the rounding stream.
@@ -678,6 +702,7 @@ int autotune(lb_ctx *c) {
     for (const Cand &k : cands) {
         c->bx_shift = ilog2(k.bx); c->by = k.by; c->xcd = k.xcd; c->mem = k.mem; c->layout = k.layout;
         c->packed = k.packed;
+        c->dbx_shift = c->bx_shift; c->dby = c->by; c->dxcd = c->xcd; c->dmem = c->mem == 2 ? 2 : 0;
         if (int rc = upload_args(c)) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -692,13 +717,49 @@ int autotune(lb_ctx *c) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (ms < best) { best = ms; bestc = k; }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     c->bx_shift = ilog2(bestc.bx); c->by = bestc.by; c->xcd = bestc.xcd; c->mem = bestc.mem;
     c->layout = bestc.layout;
     c->packed = bestc.packed;
+    c->dbx_shift = c->bx_shift; c->dby = c->by; c->dxcd = c->xcd; c->dmem = c->mem == 2 ? 2 : 0;
     int rc = (int)hipGetLastError();
     if (!rc) rc = upload_args(c);
+    // Shan-Chen: the density pass's own block shape, XCD remap and load
+    // hints (the population layout stays the sweep's)
+    if (!rc && c->cfg.sc_G != 0.0f) {
+        struct DCand { int bx, xcd, mem; };
+        std::vector<DCand> dc;
+        for (int bx : bxs) {
+            if (bx > 2 * c->cfg.nx && bx != 32) continue;
+            if (c->cfg.block_x > 0 && bx != c->cfg.block_x) continue;
+            for (int xcd = 0; xcd < 2; xcd++) {
+                if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
+                for (int mem : {0, 2}) {
+                    if (c->cfg.mem_mode >= 0 && mem != (c->cfg.mem_mode == 2 ? 2 : 0)) continue;
+                    dc.push_back({bx, xcd, mem});
+                }
+            }
+        }
+        float dbest = 1e30f;
+        DCand dbc{1 << c->dbx_shift, c->dxcd, c->dmem};
+        for (const DCand &k : dc) {
+            c->dbx_shift = ilog2(k.bx); c->dby = kThreads / k.bx; c->dxcd = k.xcd; c->dmem = k.mem;
+            rc = upload_args(c);
+            if (rc) break;
+            launch_density(c, 0);   // warm
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < reps; r++) launch_density(c, r & 1);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < dbest) { dbest = ms; dbc = k; }
+        }
+        c->dbx_shift = ilog2(dbc.bx); c->dby = kThreads / dbc.bx; c->dxcd = dbc.xcd; c->dmem = dbc.mem;
+        if (!rc) rc = (int)hipGetLastError();
+        if (!rc) rc = upload_args(c);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -785,7 +846,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipMalloc((void **)&c->partial, (size_t)kReduceBlocks * 4 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&c->dout, 8 * sizeof(double));
     if (!rc) rc = (int)hipMalloc((void **)&c->tmp_fields, (size_t)4 * c->N * sizeof(float));
-    if (!rc) rc = (int)hipMalloc(&c->dargs, 2 * lb_ctx::kArgBytes);
+    if (!rc) rc = (int)hipMalloc(&c->dargs, 4 * lb_ctx::kArgBytes);
     if (!rc && cfg->sc_G != 0.0f) rc = (int)hipMalloc(&c->psi, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4));
     if (!rc && cfg->halo_z && c->Q == 19) {
         for (int h = 0; h < 2 && !rc; h++) {
@@ -806,6 +867,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->mem = cfg->mem_mode;
             c->layout = cfg->layout;
             c->packed = cfg->node_words;
+            c->dbx_shift = c->bx_shift; c->dby = c->by; c->dxcd = c->xcd; c->dmem = c->mem == 2 ? 2 : 0;
             rc = upload_args(c);
         } else {
             rc = autotune(c);
