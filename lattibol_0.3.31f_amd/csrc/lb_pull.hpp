@@ -103,12 +103,8 @@ template <class S> LB_DEV int tile_of_block(const StepArgs<S> &a) {
     return b;
 }
 
-// Node of this thread inside tile b (tiles: BX x BY nodes of one z-plane).
-template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
-    int tx = b % a.ntx;
-    int r = b / a.ntx;
-    int ty = r % a.nty;
-    int z = r / a.nty;
+// Node of this thread in tile (tx, ty) of plane z (tiles: BX x BY nodes).
+template <class S> LB_DEV NodePos node_in_tile(const StepArgs<S> &a, int tx, int ty, int z) {
     int lx = threadIdx.x & ((1 << a.bx_shift) - 1);
     int ly = threadIdx.x >> a.bx_shift;
     NodePos p;
@@ -119,6 +115,30 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     p.id = (uint32_t)p.x + (uint32_t)a.nx * ((uint32_t)p.y + (uint32_t)a.ny * (uint32_t)z);
     p.e0 = ((uint32_t)tx << a.bx_shift) + a.sy * (uint32_t)(ty * a.by) + a.sz * (uint32_t)z;
     p.e = p.e0 + (uint32_t)lx + a.sy * (uint32_t)ly;
+    p.step = 0;
+    return p;
+}
+
+// Node of this thread inside tile b (tiles: BX x BY nodes of one z-plane).
+template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
+    int tx = b % a.ntx;
+    int r = b / a.ntx;
+    int ty = r % a.nty;
+    int z = r / a.nty;
+    return node_in_tile(a, tx, ty, z);
+}
+
+// A node given by its coordinates (inside the lattice), for work on nodes
+// outside the thread's own tile (per-link pull form only: e0 = e).
+template <class S> LB_DEV NodePos node_at(const StepArgs<S> &a, int x, int y, int z) {
+    NodePos p;
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.valid = true;
+    p.id = (uint32_t)x + (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z);
+    p.e = (uint32_t)x + a.sy * (uint32_t)y + a.sz * (uint32_t)z;
+    p.e0 = p.e;
     p.step = 0;
     return p;
 }
@@ -457,6 +477,21 @@ LB_DEV void collide_dev(T *s, T drho, T ux, T uy, T uz, T ti, bool rest_closure)
     });
 }
 
+// Node whose psi stands at (gx, gy, gz) of a psi tile: wrap across periodic
+// (and halo) faces, clamp otherwise.
+template <int Q, class S> LB_DEV void psi_neighbour(const StepArgs<S> &a, int &gx, int &gy, int &gz) {
+    if (gx < 0) gx = a.bc[0] == LB_BC_PERIODIC ? gx + a.nx : 0;
+    if (gx >= a.nx) gx = a.bc[1] == LB_BC_PERIODIC ? gx - a.nx : a.nx - 1;
+    if (gx >= a.nx) gx = a.nx - 1;   // tile columns past a ragged edge
+    if (gy < 0) gy = a.bc[2] == LB_BC_PERIODIC ? gy + a.ny : 0;
+    if (gy >= a.ny) gy = a.bc[3] == LB_BC_PERIODIC ? gy - a.ny : a.ny - 1;
+    if (gy >= a.ny) gy = a.ny - 1;
+    if (Q == 19) {
+        if (gz < 0) gz = (a.halo || a.bc[4] == LB_BC_PERIODIC) ? gz + a.nz : 0;
+        if (gz >= a.nz) gz = (a.halo || a.bc[5] == LB_BC_PERIODIC) ? gz - a.nz : a.nz - 1;
+    }
+}
+
 // Shan-Chen psi tile in LDS: (BX+2) x (BY+2) x 3 planes around the block.
 template <int Q, class S>
 LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int bx, int by) {
@@ -472,17 +507,7 @@ LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int b
         int dx = k % W, rest = k / W;
         int dy = rest % Hh, dz = rest / Hh;
         int gx = x0 + dx, gy = y0 + dy, gz = Q == 19 ? z - 1 + dz : 0;
-        // neighbour rule: wrap across periodic (and halo) faces, clamp otherwise
-        if (gx < 0) gx = a.bc[0] == LB_BC_PERIODIC ? gx + a.nx : 0;
-        if (gx >= a.nx) gx = a.bc[1] == LB_BC_PERIODIC ? gx - a.nx : a.nx - 1;
-        if (gx >= a.nx) gx = a.nx - 1;   // tile columns past a ragged edge
-        if (gy < 0) gy = a.bc[2] == LB_BC_PERIODIC ? gy + a.ny : 0;
-        if (gy >= a.ny) gy = a.bc[3] == LB_BC_PERIODIC ? gy - a.ny : a.ny - 1;
-        if (gy >= a.ny) gy = a.ny - 1;
-        if (Q == 19) {
-            if (gz < 0) gz = (a.halo || a.bc[4] == LB_BC_PERIODIC) ? gz + a.nz : 0;
-            if (gz >= a.nz) gz = (a.halo || a.bc[5] == LB_BC_PERIODIC) ? gz - a.nz : a.nz - 1;
-        }
+        psi_neighbour<Q>(a, gx, gy, gz);
         lds[k] = ldm(&a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)]);
     }
 }
@@ -586,16 +611,65 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
     store_node<Q, S, MEM>(a, p, f);
 }
 
-// Shan-Chen density pre-pass: psi(rho) of the post-streaming density, with
-// the same pull / boundary handling (and summation order) as k_step.
-template <int Q, class S, int MEM = 0>
-LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
+// The fluid-node part of step_body<Q, S, SC = true> after the pull and the
+// Zou-He correction (moments, body force, Shan-Chen force, collision, macros,
+// store) for populations already pulled into f, with the psi planes z - 1,
+// z, z + 1 of the tile at element offsets poff[0..2] of lds (the one-pass
+// Shan-Chen step keeps them in a ring of three slots).  The same operations
+// in the same order as step_body, so the results are identical.
+template <int Q, class S, bool MACROS, int MEM = 0>
+LB_DEV void sc_fluid_tail(const StepArgs<S> &a, const NodePos &p, typename S::T *f, const typename S::T *lds,
+                          const int poff[3]) {
     typedef typename S::T T;
-    if (!p.valid) return;
+    typedef Vel<Q> L;
+    const int bx = 1 << a.bx_shift;
+    T rho, ux, uy, uz;
+    T drho = T(0);
+    if constexpr (S::DEV) {
+        moments_dev<Q, T>(f, drho, ux, uy, uz);
+        rho = T(1) + drho;
+    } else {
+        moments<Q, T>(f, rho, ux, uy, uz);
+    }
+    const T ti = (T)a.ti;
+    T ex = ux, ey = uy, ez = uz;
+    if (a.force[0] != 0.0f) ex = ux + (T)a.force[0] / (rho * ti);
+    if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
+    if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
+    {
+        const int W = bx + 2;
+        const int lx = (threadIdx.x & (bx - 1)) + 1, ly = (threadIdx.x >> a.bx_shift) + 1;
+        T IF[3] = {T(0), T(0), T(0)};
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+            const int k = (lx + cx) + W * (ly + cy) + poff[Q == 19 ? 1 + cz : 1];
+            T wp = weight<Q, T>(L::wc(i)) * lds[k];
+            if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
+            if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
+            if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
+        });
+        const int kc = lx + W * ly + poff[1];
+        T gp = (T)(-a.scG) * lds[kc];
+        ex = ex + (gp * IF[0]) / (ti * rho);
+        ey = ey + (gp * IF[1]) / (ti * rho);
+        if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+    }
+    if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
+    else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
+    if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
+    store_node<Q, S, MEM>(a, p, f);
+}
+
+// Shan-Chen psi(rho) of one node's post-streaming density, with the same
+// pull / boundary handling (and summation order) as k_step.
+template <int Q, class S, int MEM = 0>
+LB_DEV typename S::T psi_of_node(const StepArgs<S> &a, const NodePos &p) {
+    typedef typename S::T T;
     uint8_t fl;
     uint32_t bm;
     node_word(a, p, fl, bm);
-    if (fl & F_SOLID) { stm(&a.psi[p.id], (T)a.scpsi_solid); return; }
+    if (fl & F_SOLID) return (T)a.scpsi_solid;
     T rho;
     if (fl & F_INLET) {
         rho = (T)a.bcrho[fl >> 4];
@@ -612,7 +686,14 @@ LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
             moments<Q, T>(f, rho, ux, uy, uz);
         }
     }
-    stm(&a.psi[p.id], sc_psi<T>(rho, a.scrho0));
+    return sc_psi<T>(rho, a.scrho0);
+}
+
+// Shan-Chen density pre-pass: psi of every node into a.psi.
+template <int Q, class S, int MEM = 0>
+LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
+    if (!p.valid) return;
+    stm(&a.psi[p.id], psi_of_node<Q, S, MEM>(a, p));
 }
 
 // ---------------------------------------------------------------------------
