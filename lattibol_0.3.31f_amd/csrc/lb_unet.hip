@@ -765,8 +765,9 @@ __global__ void k_avgpool(const bf16 *in, bf16 *out, int B, int H, int Wd, int C
     *(u16x8 *)(out + (size_t)p * C + c) = o;
 }
 
-// d(in) (+)= d(out)/4 at each of the four children, four channels per thread.
-__global__ void k_avgpool_bwd(const float *dout, float *din, int B, int H, int Wd, int C, int accumulate) {
+// din = skip + d(out)/4 at each of the four children (skip may be null: no
+// sum), four channels per thread.
+__global__ void k_avgpool_bwd(const float *dout, const float *skip, float *din, int B, int H, int Wd, int C) {
     const unsigned C4 = C >> 2;
     const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (unsigned)B * H * Wd * C4) return;
@@ -775,8 +776,8 @@ __global__ void k_avgpool_bwd(const float *dout, float *din, int B, int H, int W
     const float4 s = *(const float4 *)(dout + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * C + c);
     const float4 g = float4{0.25f * s.x, 0.25f * s.y, 0.25f * s.z, 0.25f * s.w};
     float4 *o = (float4 *)(din + (size_t)p * C + c);
-    if (accumulate) {
-        const float4 v = *o;
+    if (skip) {
+        const float4 v = *(const float4 *)(skip + (size_t)p * C + c);
         *o = float4{v.x + g.x, v.y + g.y, v.z + g.z, v.w + g.w};
     } else {
         *o = g;
@@ -818,33 +819,54 @@ __global__ void k_upcat_bwd_skip(const float *dcat, float *dskip, int B, int H, 
     *(float4 *)(dskip + (size_t)p * Cs + c) = *(const float4 *)(dcat + (size_t)p * (Cl + Cs) + Cl + c);
 }
 
-// 1x1 output head (FP32), one pixel per thread (C % 8 == 0, O <= 8):
-// y[m, o] = sum_c x[m, c] Wh[o, c] + bh[o], c ascending.
-__global__ void k_head_fwd(const bf16 *x, const bf16 *Wh, const float *bh, float *y, int npix, int C, int O) {
+// 1x1 output head (FP32), one pixel per thread (C % 8 == 0, C <= 256,
+// O <= 8): y[m, o] = sum_c x[m, c] Wh[o, c] + bh[o], c ascending.  The
+// weights are staged in LDS as floats once per workgroup.
+__global__ __launch_bounds__(256) void k_head_fwd(const bf16 *x, const bf16 *Wh, const float *bh, float *y, int npix,
+                                                  int C, int O) {
+    __shared__ float wsh[8 * 256];
+    for (int e = threadIdx.x; e < O * C; e += 256) wsh[e] = from_bf16(Wh[e]);
+    __syncthreads();
     const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= (unsigned)npix) return;
     float s[8];
     for (int o = 0; o < 8; o++) s[o] = 0.f;
     for (int c = 0; c < C; c += 8) {
         const u16x8 v = *(const u16x8 *)(x + (size_t)m * C + c);
+#pragma unroll
         for (int k = 0; k < 8; k++) {
             const float xv = lb::bf16_to_f32(v[k]);
-            for (int o = 0; o < O; o++) s[o] += xv * from_bf16(Wh[o * C + c + k]);
+#pragma unroll
+            for (int o = 0; o < 8; o++)
+                if (o < O) s[o] += xv * wsh[o * C + c + k];
         }
     }
-    for (int o = 0; o < O; o++) y[(size_t)m * O + o] = s[o] + bh[o];
+#pragma unroll
+    for (int o = 0; o < 8; o++)
+        if (o < O) y[(size_t)m * O + o] = s[o] + bh[o];
 }
-// dx[m, c] = sum_o dy[m, o] Wh[o, c], four channels per thread.
-__global__ void k_head_bwd_data(const float *dy, const bf16 *Wh, float *dx, int npix, int C, int O) {
+// dx[m, c] = sum_o dy[m, o] Wh[o, c], four channels per thread (weights
+// staged in LDS as floats, O <= 8, C <= 256).
+__global__ __launch_bounds__(256) void k_head_bwd_data(const float *dy, const bf16 *Wh, float *dx, int npix, int C,
+                                                       int O) {
+    __shared__ float wsh[8 * 256];
+    for (int e = threadIdx.x; e < O * C; e += 256) wsh[e] = from_bf16(Wh[e]);
+    __syncthreads();
     const unsigned C4 = C >> 2;
     const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (unsigned)npix * C4) return;
     const unsigned c = (e % C4) * 4, m = e / C4;
+    float g[8];
+#pragma unroll
+    for (int o = 0; o < 8; o++) g[o] = o < O ? dy[(size_t)m * O + o] : 0.f;
     float r[4];
+#pragma unroll
     for (int k = 0; k < 4; k++) {
-        float s = 0.f;
-        for (int o = 0; o < O; o++) s += dy[(size_t)m * O + o] * from_bf16(Wh[o * C + c + k]);
-        r[k] = s;
+        float sum = 0.f;
+#pragma unroll
+        for (int o = 0; o < 8; o++)
+            if (o < O) sum += g[o] * wsh[o * C + c + k];
+        r[k] = sum;
     }
     *(float4 *)(dx + (size_t)m * C + c) = float4{r[0], r[1], r[2], r[3]};
 }
@@ -1557,11 +1579,12 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const bf16 *in1 = l == 0 ? u->xin : u->pool[l - 1];
         conv_bwd(c1, in1, l, l > 0);
         if (l > 0) {
-            // gcur = d(pool[l-1]); add the pool backward into the skip gradient
+            // gcur = d(pool[l-1]); the gradient of the encoder output is its
+            // skip gradient plus the pool backward
             size_t n = (size_t)B * L[l - 1].H * L[l - 1].W * L[l - 1].C;
-            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb256(n / 4)), dim3(256), 0, u->stream, gcur, u->gskip[l - 1], B, L[l - 1].H,
-                               L[l - 1].W, L[l - 1].C, 1);
-            (void)hipMemcpyAsync(gcur, u->gskip[l - 1], n * 4, hipMemcpyDeviceToDevice, u->stream);
+            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb256(n / 4)), dim3(256), 0, u->stream, gcur, u->gskip[l - 1], gtmp, B,
+                               L[l - 1].H, L[l - 1].W, L[l - 1].C);
+            std::swap(gcur, gtmp);
         }
     }
     rc = (int)hipGetLastError();
