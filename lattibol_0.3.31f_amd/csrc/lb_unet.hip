@@ -1014,12 +1014,15 @@ __global__ void k_flip_transpose_all(const bf16 *wb, bf16 *wt, const long long *
 }
 
 // Input tensor: fp32 (B, H, W, cin) -> bf16 (B, H, W, cpad), zero padded.
+// One pixel per thread, written as 16-byte pieces (cpad % 8 == 0, cin <= cpad).
 __global__ void k_pack_input(const float *x, bf16 *out, size_t npix, int cin, int cpad) {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= npix * cpad) return;
-    int c = e % cpad;
-    size_t m = e / cpad;
-    out[e] = c < cin ? to_bf16(x[m * cin + c]) : to_bf16(0.f);
+    const size_t m = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= npix) return;
+    for (int c0 = 0; c0 < cpad; c0 += 8) {
+        u16x8 v;
+        for (int k = 0; k < 8; k++) v[k] = c0 + k < cin ? lb::f32_to_bf16(x[m * cin + c0 + k]) : (uint16_t)0;
+        *(u16x8 *)(out + m * cpad + c0) = v;
+    }
 }
 
 // Standardised prediction -> lattice fields of sample b, packed
@@ -1467,7 +1470,7 @@ extern "C" int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y) 
     std::vector<Level> L = levels(u);
     const int Lv = u->d.levels;
     size_t npix0 = (size_t)B * L[0].H * L[0].W;
-    hipLaunchKernelGGL(k_pack_input, dim3(nb256(npix0 * u->cin_pad)), dim3(256), 0, u->stream, d_x, u->xin, npix0,
+    hipLaunchKernelGGL(k_pack_input, dim3(nb256(npix0)), dim3(256), 0, u->stream, d_x, u->xin, npix0,
                        u->d.in_ch, u->cin_pad);
     const bf16 *cur = u->xin;
     int ci = 0;
