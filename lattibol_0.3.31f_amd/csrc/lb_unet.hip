@@ -354,8 +354,8 @@ __global__ void k_split_sum(const float *__restrict__ part, int ks, size_t n, in
 
 // Weight gradient partials: block (co tile, k tile, pixel chunk); each of the
 // 4 wavefronts reduces a quarter of the chunk, the block sums them in LDS.
-template <class TIn>
-__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const float *__restrict__ dout, const TIn *__restrict__ in,
+template <class TIn, class TD = float>
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const TD *__restrict__ dout, const TIn *__restrict__ in,
                                                        float *__restrict__ partial, int npix, int H, int Wd,
                                                        int Ci, int Co, int chunk) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -380,15 +380,15 @@ __global__ __launch_bounds__(256) void k_conv3x3_wgrad(const float *__restrict__
         int p = p0 + kh;
         int n = p / HW, h = (p - n * HW) / Wd, w = p - n * HW - h * Wd;
         for (int j = 0; j < 8; j++, p++) {
-            float g = 0.f;
+            uint16_t gb = 0;
             uint16_t xb = 0;
             if (p < pend) {
-                g = dout[(size_t)p * Co + co0 + row];
+                gb = bf16_bits(dout[(size_t)p * Co + co0 + row]);
                 const int hh = h + dy, ww = w + dx;
                 if (kvalid && hh >= 0 && hh < H && ww >= 0 && ww < Wd)
                     xb = bf16_bits(in[((size_t)(n * H + hh) * Wd + ww) * Ci + ci]);
             }
-            av[j] = lb::f32_to_bf16(g);
+            av[j] = gb;
             bv[j] = xb;
             if (++w == Wd) { w = 0; if (++h == H) { h = 0; n++; } }
         }
@@ -534,10 +534,9 @@ constexpr int kWgGS = 264;      // LDS stride of one output channel (256 pixels;
 __device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
     return uint2{(uint32_t)a | ((uint32_t)b << 16), (uint32_t)c | ((uint32_t)d << 16)};
 }
-__device__ __forceinline__ float lane_of(const float4 &v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
 
-template <int TW>
-__global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const float *__restrict__ dout, const bf16 *__restrict__ in,
+template <int TW, class TD = float>
+__global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const TD *__restrict__ dout, const bf16 *__restrict__ in,
                                                             float *__restrict__ partial, int H, int Wd, int Ci, int Co,
                                                             int ntiles, int tpw) {
     __shared__ __attribute__((aligned(16))) uint16_t sX[32 * kWgXS];
@@ -556,7 +555,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const float *__re
     constexpr int NXQ = (TR + 2) * NQX * 4;      // window: rows x 4-pixel groups x four 8-channel groups
     constexpr int RX = (NXQ + 255) / 256;
     u16x8 xq[RX][4];
-    float4 gq[4][2];
+    Piece8<TD> gq[4];
     auto fetch = [&](int tile) {
         const int seg = tile % segs, rest = tile / segs;
         const int y0 = (rest % rblocks) * TR, img = rest / rblocks, x0 = seg * TW;
@@ -581,12 +580,9 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const float *__re
         {   // gradient: 64 4-pixel groups x four 8-channel groups = one piece per thread
             const int q = tid & 3, r = tid >> 2;
             const int iy = r / QPR, qx = r % QPR;
-            const float *g = dout + (((size_t)img * H + y0 + iy) * Wd + x0 + 4 * qx) * Co + co0 + 8 * q;
+            const TD *g = dout + (((size_t)img * H + y0 + iy) * Wd + x0 + 4 * qx) * Co + co0 + 8 * q;
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-                gq[p][0] = *(const float4 *)(g + (size_t)p * Co);
-                gq[p][1] = *(const float4 *)(g + (size_t)p * Co + 4);
-            }
+            for (int p = 0; p < 4; p++) gq[p].load(g + (size_t)p * Co);
         }
     };
     auto store = [&]() {
@@ -605,13 +601,9 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const float *__re
         {
             const int q = tid & 3, r = tid >> 2;
             const int base = 4 * r;              // tile pixel (r / QPR) TW + 4 (r % QPR)
+            const u16x8 b0 = gq[0].bits(), b1 = gq[1].bits(), b2 = gq[2].bits(), b3 = gq[3].bits();
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int h = k >> 2, c = k & 3;
-                *(uint2 *)&sG[(8 * q + k) * kWgGS + base] =
-                    pack4(lb::f32_to_bf16(lane_of(gq[0][h], c)), lb::f32_to_bf16(lane_of(gq[1][h], c)),
-                          lb::f32_to_bf16(lane_of(gq[2][h], c)), lb::f32_to_bf16(lane_of(gq[3][h], c)));
-            }
+            for (int k = 0; k < 8; k++) *(uint2 *)&sG[(8 * q + k) * kWgGS + base] = pack4(b0[k], b1[k], b2[k], b3[k]);
         }
     };
 
@@ -707,12 +699,14 @@ inline BiasChunks bias_chunks(int npix) {
 
 // ReLU backward on the stored post-activation, dpre = d * (act > 0), and the
 // bias-gradient partials of the same pass, partial[chunk][co] =
-// sum_{m in chunk} dpre[m, co].  Block = one pixel chunk x all Co channels
+// sum_{m in chunk} dpre[m, co] (FP32).  dpre is stored as BF16 (nearest
+// even): its consumers, the weight gradient and the input gradient, round it
+// so before their MFMAs, so they see the same operands.  Block = one pixel chunk x all Co channels
 // (Co % 4 == 0, Co <= 1024): thread (4-channel group, pixel phase) walks the
 // chunk with stride 256 / (Co / 4); the phases are added in LDS in a fixed
 // order.
 __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__ d, const bf16 *__restrict__ act,
-                                                       float *__restrict__ dpre, int npix, int Co, int chunk,
+                                                       bf16 *__restrict__ dpre, int npix, int Co, int chunk,
                                                        float *__restrict__ partial) {
     const int tpp = Co >> 2, nph = 256 / tpp;
     const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
@@ -729,7 +723,8 @@ __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__
             r.y = lb::bf16_to_f32((uint16_t)(a.x >> 16)) > 0.f ? g.y : 0.f;
             r.z = lb::bf16_to_f32((uint16_t)(a.y & 0xffffu)) > 0.f ? g.z : 0.f;
             r.w = lb::bf16_to_f32((uint16_t)(a.y >> 16)) > 0.f ? g.w : 0.f;
-            *(float4 *)(dpre + o) = r;
+            *(uint2 *)(dpre + o) = uint2{(uint32_t)lb::f32_to_bf16(r.x) | ((uint32_t)lb::f32_to_bf16(r.y) << 16),
+                                          (uint32_t)lb::f32_to_bf16(r.z) | ((uint32_t)lb::f32_to_bf16(r.w) << 16)};
             s0 += r.x; s1 += r.y; s2 += r.z; s3 += r.w;
         }
     }
@@ -1234,7 +1229,8 @@ inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
 // Chunk count of the per-wavefront weight-gradient kernels (1,024 pixels each).
 inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
 
-// Weight gradient dW[co][tap][ci] from dout (FP32 NHWC) and the conv input
+// Weight gradient dW[co][tap][ci] from dout (NHWC, FP32 or BF16; every kernel
+// rounds it to BF16 to nearest even as an MFMA operand) and the conv input
 // (BF16 NHWC).  Shapes the LDS-tile kernel supports take it (method < 0 or
 // 2); otherwise, with W % 16 == 0 and channel-major scratch (tA >= npix Co,
 // tB >= npix Ci elements), both operands are transposed first and the
@@ -1242,18 +1238,18 @@ inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
 // (method 0).
 // extra: a second chunk sum (the bias gradient) done in the same launch as
 // the weight gradient's.
-template <class TIn>
-void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
+template <class TIn, class TD>
+void wgrad_launch(lb_unet *u, const TD *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
                   bf16 *tA, bf16 *tB, int method = -1, SumSeg extra = SumSeg{nullptr, 0, 0, nullptr}) {
     const int K = 9 * Ci;
     const WgradTiles wt = wgrad_tiles(npix, H, Wd, Ci, Co);
     if ((method < 0 || method == 2) && wt.ntiles) {
         const dim3 grd(Co / 32, (Ci + 31) / 32, wt.nchunk);
         if (wt.tw == 32)
-            hipLaunchKernelGGL(k_conv3x3_wgrad_tile<32>, grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<32, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
                                Wd, Ci, Co, wt.ntiles, wt.tpw);
         else
-            hipLaunchKernelGGL(k_conv3x3_wgrad_tile<16>, grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<16, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
                                Wd, Ci, Co, wt.ntiles, wt.tpw);
         sum_chunks2(u, SumSeg{u->partial, wt.nchunk, (size_t)Co * K, dW}, extra);
         return;
@@ -1262,14 +1258,14 @@ void wgrad_launch(lb_unet *u, const float *dout, const TIn *in, float *dW, int n
     const int nchunk = (npix + chunk - 1) / chunk;
     dim3 grd(Co / 32, (K + 31) / 32, nchunk);
     if (method != 0 && tA && tB && Wd % 16 == 0) {
-        hipLaunchKernelGGL(k_to_cmajor<float>, dim3((npix + 63) / 64, (Co + 63) / 64), dim3(256), 0, u->stream, dout, tA,
+        hipLaunchKernelGGL(k_to_cmajor<TD>, dim3((npix + 63) / 64, (Co + 63) / 64), dim3(256), 0, u->stream, dout, tA,
                            npix, Co);
         hipLaunchKernelGGL(k_to_cmajor<TIn>, dim3((npix + 63) / 64, (Ci + 63) / 64), dim3(256), 0, u->stream, in, tB,
                            npix, Ci);
         hipLaunchKernelGGL(k_conv3x3_wgrad_cm, grd, dim3(256), 0, u->stream, tA, tB, u->partial, npix, H, Wd, Ci, Co,
                            chunk);
     } else {
-        hipLaunchKernelGGL((k_conv3x3_wgrad<TIn>), grd, dim3(256), 0, u->stream, dout, in, u->partial, npix, H, Wd, Ci, Co,
+        hipLaunchKernelGGL((k_conv3x3_wgrad<TIn, TD>), grd, dim3(256), 0, u->stream, dout, in, u->partial, npix, H, Wd, Ci, Co,
                            chunk);
     }
     sum_chunks2(u, SumSeg{u->partial, nchunk, (size_t)Co * K, dW}, extra);
@@ -1543,7 +1539,8 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0 / 4)), dim3(256), 0, u->stream, u->dy, u->wb + u->head_woff,
                        u->gA, npix0, C0, O);
     // gA = d(out of current conv), post-activation gradient
-    float *gcur = u->gA, *gtmp = u->gB, *gpre = u->gC;
+    float *gcur = u->gA, *gtmp = u->gB;
+    bf16 *gpre = (bf16 *)u->gC;    // pre-activation gradient, BF16
     int ci = last;
     auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din) {
         const UConv &c = u->convs[cidx];
@@ -1553,10 +1550,10 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const BiasChunks bc = bias_chunks(npix);
         hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
                            c.co, bc.chunk, u->bpart);
-        wgrad_launch<bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
+        wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
                            SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff});
         if (need_din)
-            conv_launch<float, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
+            conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
         std::swap(gcur, gtmp);
     };
     for (int l = 0; l <= Lv - 2; l++) {
@@ -1669,7 +1666,7 @@ extern "C" int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void
     int rc = 0;
     if (!tmp.partial || (method && (!tA || !tB))) rc = LB_ERR_INVALID;
     if (!rc) {
-        wgrad_launch<bf16>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB, method);
+        wgrad_launch<bf16, float>(&tmp, d_dout, (const bf16 *)d_in, d_dw, npix, H, W, Ci, Co, tA, tB, method);
         rc = (int)hipGetLastError();
     }
     if (!rc) rc = (int)hipStreamSynchronize(tmp.stream);
