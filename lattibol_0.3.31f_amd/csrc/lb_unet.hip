@@ -204,9 +204,10 @@ template <> struct Piece8<float> {
 };
 
 template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU, int SW = 32, bool SPLIT = false>
-__global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in, const bf16 *__restrict__ W,
-                                                      const float *__restrict__ bias, TOut *__restrict__ out,
-                                                      float *__restrict__ part, int npix, int H, int Wd, int Ci, int Co) {
+__global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in, const TIn *__restrict__ low, int cl,
+                                                      const bf16 *__restrict__ W, const float *__restrict__ bias,
+                                                      TOut *__restrict__ out, float *__restrict__ part, int npix, int H,
+                                                      int Wd, int Ci, int Co) {
     constexpr int RPT = 32 / SW;                 // image rows per MFMA pixel tile
     constexpr int ROWS = 4 * PM * RPT, IR = ROWS + 2, IC = SW + 2;
     constexpr int NCO = 32 * NB;
@@ -241,8 +242,13 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
                 const int q = e & 3, p = e >> 2;
                 const int iy = p / IC, ix = p - iy * IC;
                 const int yy = y0 + iy - 1, xx = x0 + ix - 1;
-                if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
-                    pin[j].load(in + (((size_t)img * H + yy) * Wd + xx) * Ci + c0 + q * 8);
+                if (yy >= 0 && yy < H && xx >= 0 && xx < Wd) {
+                    const int c = c0 + q * 8;
+                    if (c < cl)
+                        pin[j].load(low + (((size_t)img * (H >> 1) + (yy >> 1)) * (Wd >> 1) + (xx >> 1)) * cl + c);
+                    else
+                        pin[j].load(in + (((size_t)img * H + yy) * Wd + xx) * (Ci - cl) + (c - cl));
+                }
             }
         }
 #pragma unroll
@@ -537,6 +543,7 @@ __device__ __forceinline__ uint2 pack4(uint16_t a, uint16_t b, uint16_t c, uint1
 
 template <int TW, class TD = float>
 __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const TD *__restrict__ dout, const bf16 *__restrict__ in,
+                                                            const bf16 *__restrict__ low, int cl,
                                                             float *__restrict__ partial, int H, int Wd, int Ci, int Co,
                                                             int ntiles, int tpw) {
     __shared__ __attribute__((aligned(16))) uint16_t sX[32 * kWgXS];
@@ -572,7 +579,10 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3_wgrad_tile(const TD *__restr
 #pragma unroll
                     for (int p = 0; p < 4; p++) {
                         const int xx = x0 + 4 * qx + p - 1;
-                        if (xx >= 0 && xx < Wd) xq[j][p] = *(const u16x8 *)(in + (((size_t)img * H + yy) * Wd + xx) * Ci + c);
+                        if (xx >= 0 && xx < Wd)
+                            xq[j][p] = c < cl ? *(const u16x8 *)(low + (((size_t)img * (H >> 1) + (yy >> 1)) * (Wd >> 1) +
+                                                                        (xx >> 1)) * cl + c)
+                                              : *(const u16x8 *)(in + (((size_t)img * H + yy) * Wd + xx) * (Ci - cl) + (c - cl));
                     }
                 }
             }
@@ -1062,6 +1072,10 @@ struct lb_unet {
     std::vector<bf16 *> act;       // per conv output
     std::vector<bf16 *> pool;      // per encoder level
     std::vector<bf16 *> cat;       // per decoder level
+    // per decoder level, set by the last forward: the first conv read its
+    // input [upsampled low | skip] in place (cat[l] not written), low's pointer
+    std::vector<char> cat_direct;
+    std::vector<const bf16 *> cat_low;
     bf16 *xin = nullptr;
     float *y = nullptr, *dy = nullptr;
     float *gA = nullptr, *gB = nullptr, *gC = nullptr;   // gradient scratch (largest tensor)
@@ -1140,10 +1154,12 @@ inline size_t conv_split_scratch(int npix, int H, int Wd, int Ci, int Co) {
 
 // LDS-tile launch for eligible shapes; returns false (nothing launched) when
 // the shape does not fit the tile kernel or a K-split finds no scratch
-// (u->partial, u->partial_n elements).
+// (u->partial, u->partial_n elements).  low != null: the input is the
+// concatenation [nearest-upsampled low (cl channels) | in (Ci - cl channels)]
+// read in place (cl % 32 == 0).
 template <class TIn, class TOut, bool BR>
 bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
-                      int Ci, int Co) {
+                      int Ci, int Co, const TIn *low = nullptr, int cl = 0) {
     const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
     if (!pl.sw) return false;
     float *part = nullptr;
@@ -1156,32 +1172,32 @@ bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bia
     const dim3 grd((unsigned)(B * (H / rows) * (Wd / pl.sw)), (unsigned)(Co / (32 * pl.nb)), (unsigned)pl.ks), blk(256);
     if (pl.sw == 32) {
         if (pl.nb == 2 && pl.pm == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 2, BR>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix, H,
                                Wd, Ci, Co);
         else if (pl.nb == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix, H,
                                Wd, Ci, Co);
         else if (pl.pm == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 2, 1, BR>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix, H,
                                Wd, Ci, Co);
         else
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR>), grd, blk, 0, u->stream, in, W, bias, out, part, npix, H,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix, H,
                                Wd, Ci, Co);
     } else if (pl.ks > 1) {
         if (pl.nb == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16, true>), grd, blk, 0, u->stream, in, W, bias, out, part,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16, true>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part,
                                npix, H, Wd, Ci, Co);
         else
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16, true>), grd, blk, 0, u->stream, in, W, bias, out, part,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16, true>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part,
                                npix, H, Wd, Ci, Co);
         const size_t n = (size_t)npix * Co;
         hipLaunchKernelGGL((k_split_sum<TOut, BR>), dim3(nb256(n / 4)), blk, 0, u->stream, part, pl.ks, n, Co, bias, out);
     } else {
         if (pl.nb == 2)
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16>), grd, blk, 0, u->stream, in, W, bias, out, part, npix,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 2, BR, 16>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix,
                                H, Wd, Ci, Co);
         else
-            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16>), grd, blk, 0, u->stream, in, W, bias, out, part, npix,
+            hipLaunchKernelGGL((k_conv3x3_tile<TIn, TOut, 1, 1, BR, 16>), grd, blk, 0, u->stream, in, low, cl, W, bias, out, part, npix,
                                H, Wd, Ci, Co);
     }
     return true;
@@ -1190,7 +1206,8 @@ bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bia
 template <class TIn, class TOut, bool BR>
 void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
                  int Ci, int Co, bool gather_only = false) {
-    if (!gather_only && conv_tile_launch<TIn, TOut, BR>(u, in, W, bias, out, npix, H, Wd, Ci, Co)) return;
+    if (!gather_only && conv_tile_launch<TIn, TOut, BR>(u, in, W, bias, out, npix, H, Wd, Ci, Co, (const TIn *)nullptr, 0))
+        return;
     dim3 blk(256);
     const int nb = Co % 64 == 0 ? 2 : 1;
     // fewer than ~4 workgroups per CU with one 32-pixel tile per wavefront:
@@ -1226,6 +1243,14 @@ inline WgradTiles wgrad_tiles(int npix, int H, int Wd, int Ci, int Co) {
     const int tpw = (int)std::max(1L, (ntiles * base + 511) / 512);
     return {ntiles, tpw, (ntiles + tpw - 1) / tpw, tw};
 }
+// The first conv of a decoder level can read [upsampled low | skip] in place
+// in both its forward and its weight gradient.
+inline bool cat_in_place(const lb_unet *u, int npix, int H, int Wd, int cl, int Ci, int Co) {
+    if (cl % 32) return false;
+    if (!conv_plan(npix, H, Wd, Ci, Co).sw || !wgrad_tiles(npix, H, Wd, Ci, Co).ntiles) return false;
+    return conv_split_scratch(npix, H, Wd, Ci, Co) <= u->partial_n;
+}
+
 // Chunk count of the per-wavefront weight-gradient kernels (1,024 pixels each).
 inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
 
@@ -1238,18 +1263,21 @@ inline int wgrad_chunks_legacy(int npix) { return (npix + 1023) / 1024; }
 // (method 0).
 // extra: a second chunk sum (the bias gradient) done in the same launch as
 // the weight gradient's.
+// low != null: the input is the concatenation [upsampled low | in] read in
+// place (LDS-tile kernel only; the caller checks wgrad_tiles and cl % 32).
 template <class TIn, class TD>
 void wgrad_launch(lb_unet *u, const TD *dout, const TIn *in, float *dW, int npix, int H, int Wd, int Ci, int Co,
-                  bf16 *tA, bf16 *tB, int method = -1, SumSeg extra = SumSeg{nullptr, 0, 0, nullptr}) {
+                  bf16 *tA, bf16 *tB, int method = -1, SumSeg extra = SumSeg{nullptr, 0, 0, nullptr},
+                  const bf16 *low = nullptr, int cl = 0) {
     const int K = 9 * Ci;
     const WgradTiles wt = wgrad_tiles(npix, H, Wd, Ci, Co);
     if ((method < 0 || method == 2) && wt.ntiles) {
         const dim3 grd(Co / 32, (Ci + 31) / 32, wt.nchunk);
         if (wt.tw == 32)
-            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<32, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<32, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, low, cl, u->partial, H,
                                Wd, Ci, Co, wt.ntiles, wt.tpw);
         else
-            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<16, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, u->partial, H,
+            hipLaunchKernelGGL((k_conv3x3_wgrad_tile<16, TD>), grd, dim3(256), 0, u->stream, dout, (const bf16 *)in, low, cl, u->partial, H,
                                Wd, Ci, Co, wt.ntiles, wt.tpw);
         sum_chunks2(u, SumSeg{u->partial, wt.nchunk, (size_t)Co * K, dW}, extra);
         return;
@@ -1483,17 +1511,30 @@ extern "C" int lb_unet_forward(lb_unet *u, const float *d_x, int B, float *d_y) 
             cur = u->pool[l];
         }
     }
+    u->cat_direct.assign(Lv, 0);
+    u->cat_low.assign(Lv, nullptr);
     for (int l = Lv - 2; l >= 0; l--) {
         const int npix = B * L[l].H * L[l].W;
-        size_t n = (size_t)npix * (L[l + 1].C + L[l].C) / 8;
-        hipLaunchKernelGGL(k_upcat, dim3(nb256(n)), dim3(256), 0, u->stream, cur, u->act[enc_conv(l) + 1], u->cat[l], B,
-                           L[l].H, L[l].W, L[l + 1].C, L[l].C);
-        cur = u->cat[l];
-        for (int k = 0; k < 2; k++, ci++) {
-            const UConv &c = u->convs[ci];
-            conv_launch<bf16, bf16, true>(u, cur, u->wb + c.woff, u->w + c.boff, u->act[ci], npix, L[l].H, L[l].W, c.ci, c.co);
-            cur = u->act[ci];
+        const UConv &c0 = u->convs[ci];
+        const bf16 *skip = u->act[enc_conv(l) + 1];
+        u->cat_low[l] = cur;
+        if (cat_in_place(u, npix, L[l].H, L[l].W, L[l + 1].C, c0.ci, c0.co) &&
+            conv_tile_launch<bf16, bf16, true>(u, skip, u->wb + c0.woff, u->w + c0.boff, u->act[ci], npix, L[l].H, L[l].W,
+                                               c0.ci, c0.co, cur, L[l + 1].C)) {
+            u->cat_direct[l] = 1;
+        } else {
+            size_t n = (size_t)npix * (L[l + 1].C + L[l].C) / 8;
+            hipLaunchKernelGGL(k_upcat, dim3(nb256(n)), dim3(256), 0, u->stream, cur, skip, u->cat[l], B, L[l].H, L[l].W,
+                               L[l + 1].C, L[l].C);
+            conv_launch<bf16, bf16, true>(u, u->cat[l], u->wb + c0.woff, u->w + c0.boff, u->act[ci], npix, L[l].H, L[l].W,
+                                          c0.ci, c0.co);
         }
+        cur = u->act[ci];
+        ci++;
+        const UConv &c1 = u->convs[ci];
+        conv_launch<bf16, bf16, true>(u, cur, u->wb + c1.woff, u->w + c1.boff, u->act[ci], npix, L[l].H, L[l].W, c1.ci, c1.co);
+        cur = u->act[ci];
+        ci++;
     }
     float *y = d_y ? d_y : u->y;
     hipLaunchKernelGGL(k_head_fwd, dim3(nb256(npix0)), dim3(256), 0, u->stream, cur, u->wb + u->head_woff,
@@ -1542,7 +1583,7 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     float *gcur = u->gA, *gtmp = u->gB;
     bf16 *gpre = (bf16 *)u->gC;    // pre-activation gradient, BF16
     int ci = last;
-    auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din) {
+    auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din, const bf16 *low = nullptr, int cl = 0) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
         // ReLU backward + bias-gradient partials, summed in the weight
@@ -1551,7 +1592,7 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
                            c.co, bc.chunk, u->bpart);
         wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
-                           SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff});
+                                 SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff}, low, cl);
         if (need_din)
             conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
         std::swap(gcur, gtmp);
@@ -1560,7 +1601,10 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const int lev = l;   // decoder levels in backward order: 0, 1, ..., Lv-2
         conv_bwd(ci, u->act[ci - 1], lev, true);
         ci--;
-        conv_bwd(ci, u->cat[lev], lev, true);
+        if (u->cat_direct.size() > (size_t)lev && u->cat_direct[lev])
+            conv_bwd(ci, u->act[enc_conv(lev) + 1], lev, true, u->cat_low[lev], L[lev + 1].C);
+        else
+            conv_bwd(ci, u->cat[lev], lev, true);
         ci--;
         // gcur = d(cat[lev]); split into d(low) and d(skip)
         const int npix = B * L[lev].H * L[lev].W;

@@ -179,11 +179,15 @@ def test_gpu_conv3x3_wgrad_matches_reference(gpu, ci, co, H, W):
         assert np.array_equal(out[0], out[1])   # same operands, same summation order
 
 
+# H = 32: every decoder level reads [upsampled low | skip] in place (LDS-tile
+# convs and weight gradients); H = 16: the 8x8 level takes the materialised
+# concatenation and the per-wavefront kernels.
 @pytest.mark.gpu
-def test_gpu_unet_forward_backward_match_oracle(gpu):
+@pytest.mark.parametrize("H", [32, 16])
+def test_gpu_unet_forward_backward_match_oracle(gpu, H):
     from lattibol_amd import _lib
     from lattibol_amd.unet import UNet
-    B, H = 2, 32
+    B = 2
     net = UNet(**SMALL, height=H, width=H, max_batch=B, seed=7)
     p, _ = _rand_params(**SMALL, seed=4)
     net.set_params(p)
