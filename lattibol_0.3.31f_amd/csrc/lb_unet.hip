@@ -715,9 +715,13 @@ inline BiasChunks bias_chunks(int npix) {
 // (Co % 4 == 0, Co <= 1024): thread (4-channel group, pixel phase) walks the
 // chunk with stride 256 / (Co / 4); the phases are added in LDS in a fixed
 // order.
+// dpool != null: the incoming gradient is d[m] + dpool[parent of m] / 4 (the
+// skip gradient plus the 2x2 average-pool backward of an H x W level), summed
+// here instead of in a pass of its own.
 __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__ d, const bf16 *__restrict__ act,
                                                        bf16 *__restrict__ dpre, int npix, int Co, int chunk,
-                                                       float *__restrict__ partial) {
+                                                       float *__restrict__ partial, const float *__restrict__ dpool,
+                                                       int H, int Wd) {
     const int tpp = Co >> 2, nph = 256 / tpp;
     const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
     const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
@@ -726,7 +730,12 @@ __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__
 #pragma unroll 4
         for (int m = pbeg + ph; m < pend; m += nph) {
             const size_t o = (size_t)m * Co + 4 * cg;
-            const float4 g = *(const float4 *)(d + o);
+            float4 g = *(const float4 *)(d + o);
+            if (dpool) {
+                const int x = m % Wd, r = m / Wd, y = r % H, b = r / H;
+                const float4 sp = *(const float4 *)(dpool + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * Co + 4 * cg);
+                g = float4{g.x + 0.25f * sp.x, g.y + 0.25f * sp.y, g.z + 0.25f * sp.z, g.w + 0.25f * sp.w};
+            }
             const uint2 a = *(const uint2 *)(act + o);
             float4 r;
             r.x = lb::bf16_to_f32((uint16_t)(a.x & 0xffffu)) > 0.f ? g.x : 0.f;
@@ -768,25 +777,6 @@ __global__ void k_avgpool(const bf16 *in, bf16 *out, int B, int H, int Wd, int C
         o[k] = lb::f32_to_bf16(0.25f * sum);
     }
     *(u16x8 *)(out + (size_t)p * C + c) = o;
-}
-
-// din = skip + d(out)/4 at each of the four children (skip may be null: no
-// sum), four channels per thread.
-__global__ void k_avgpool_bwd(const float *dout, const float *skip, float *din, int B, int H, int Wd, int C) {
-    const unsigned C4 = C >> 2;
-    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (unsigned)B * H * Wd * C4) return;
-    const unsigned c = (e % C4) * 4, p = e / C4;
-    const unsigned x = p % Wd, r = p / Wd, y = r % H, b = r / H;
-    const float4 s = *(const float4 *)(dout + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * C + c);
-    const float4 g = float4{0.25f * s.x, 0.25f * s.y, 0.25f * s.z, 0.25f * s.w};
-    float4 *o = (float4 *)(din + (size_t)p * C + c);
-    if (skip) {
-        const float4 v = *(const float4 *)(skip + (size_t)p * C + c);
-        *o = float4{v.x + g.x, v.y + g.y, v.z + g.z, v.w + g.w};
-    } else {
-        *o = g;
-    }
 }
 
 // out[b, y, x, :] = [low[b, y/2, x/2, 0:Cl] | skip[b, y, x, 0:Cs]], eight
@@ -1583,14 +1573,17 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     float *gcur = u->gA, *gtmp = u->gB;
     bf16 *gpre = (bf16 *)u->gC;    // pre-activation gradient, BF16
     int ci = last;
-    auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din, const bf16 *low = nullptr, int cl = 0) {
+    // dpool: the incoming gradient of this conv's output is gcur (its skip
+    // gradient) plus the average-pool backward of dpool (see k_relu_bwd_bias)
+    auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din, const bf16 *low = nullptr, int cl = 0,
+                        const float *dpool = nullptr) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
         // ReLU backward + bias-gradient partials, summed in the weight
         // gradient's chunk-sum launch
         const BiasChunks bc = bias_chunks(npix);
-        hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, gcur, u->act[cidx], gpre, npix,
-                           c.co, bc.chunk, u->bpart);
+        hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, dpool ? u->gskip[l] : gcur,
+                           u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W);
         wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
                                  SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff}, low, cl);
         if (need_din)
@@ -1617,19 +1610,15 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         std::swap(gcur, gtmp);
     }
     // bottom (level Lv-1) and encoder, deepest first
+    const float *dpool = nullptr;
     for (int l = Lv - 1; l >= 0; l--) {
         const int c2 = 2 * l + 1, c1 = 2 * l;
-        conv_bwd(c2, u->act[c1], l, true);
+        // the gradient of the encoder output act[c2] is its skip gradient
+        // plus the pool backward of d(pool[l]), summed in the ReLU backward
+        conv_bwd(c2, u->act[c1], l, true, nullptr, 0, dpool);
         const bf16 *in1 = l == 0 ? u->xin : u->pool[l - 1];
         conv_bwd(c1, in1, l, l > 0);
-        if (l > 0) {
-            // gcur = d(pool[l-1]); the gradient of the encoder output is its
-            // skip gradient plus the pool backward
-            size_t n = (size_t)B * L[l - 1].H * L[l - 1].W * L[l - 1].C;
-            hipLaunchKernelGGL(k_avgpool_bwd, dim3(nb256(n / 4)), dim3(256), 0, u->stream, gcur, u->gskip[l - 1], gtmp, B,
-                               L[l - 1].H, L[l - 1].W, L[l - 1].C);
-            std::swap(gcur, gtmp);
-        }
+        dpool = l > 0 ? gcur : nullptr;    // d(pool[l - 1])
     }
     rc = (int)hipGetLastError();
     if (!rc && loss) {
