@@ -224,11 +224,6 @@ typedef struct {
                                moving wall, no force, Shan-Chen or Zou-He, nx and ny
                                >= 16 + time_block; else 0 (a fixed value that cannot
                                apply is LB_ERR_UNSUPPORTED) */
-    int sc_fused;           /* Shan-Chen D3Q19 with FP32 / FP32-deviation / FP64
-                               populations: -1 autotune, 0 density pass + sweep,
-                               1 one pass (psi planes computed in the sweep's
-                               workgroups; LB_ERR_UNSUPPORTED elsewhere).  Results
-                               are identical either way. */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

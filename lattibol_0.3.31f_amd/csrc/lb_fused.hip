@@ -148,114 +148,6 @@ __global__ __launch_bounds__(kThreads) void k_density(const StepArgs<S> *__restr
     density_body<Q, S, MEM>(a, node_of(a, tile_of_block(a)));
 }
 
-// Shan-Chen step in one pass (D3Q19, 4- and 8-byte storage, whose pulls use
-// the per-link form): a workgroup owns one BX x BY tile and a run of zper
-// z-planes, and keeps psi of planes z - 1, z, z + 1 over the tile plus a
-// one-node halo in LDS, in a ring of three slots (the element layout of
-// load_psi_tile within a slot).  After the sweep of plane z it computes psi
-// of plane z + 2 itself, with the arithmetic of the density pass: every
-// thread pulls its own node of that plane (and keeps the populations in
-// registers for the plane's sweep two iterations later), and the ring of
-// halo nodes is pulled by the first threads (the node psi_neighbour names).
-// The density pass and its psi array leave the step: per update the
-// populations are read once from HBM (the halo ring's pulls mostly hit the
-// caches) and written once.  Solid and inlet nodes read no psi and run
-// step_body; fluid nodes run sc_fluid_tail on the kept populations.
-template <class S, bool MACROS, int MEM = 0>
-__global__ __launch_bounds__(kThreads) void k_step_scz(const StepArgs<S> *__restrict__ pa, uint32_t dstep, int zper) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    typedef typename S::T T;
-    constexpr int Q = 19;
-    const StepArgs<S> &a = *pa;
-    T *lds = (T *)lds_raw;
-    const int bx = 1 << a.bx_shift, W = bx + 2, Hh = a.by + 2, PL = W * Hh;
-    const int nring = 2 * W + 2 * (Hh - 2);
-    const int tiles = a.ntx * a.nty;
-    const int t = blockIdx.x % tiles, zr = blockIdx.x / tiles;
-    const int tx = t % a.ntx, ty = t / a.ntx;
-    const int z0 = zr * zper, z1 = min(a.nz, z0 + zper);
-    const int x0 = (tx << a.bx_shift) - 1, y0 = ty * a.by - 1;
-    const int lx = threadIdx.x & (bx - 1), ly = threadIdx.x >> a.bx_shift;
-    const int kin = (ly + 1) * W + lx + 1;           // this thread's element of a psi plane
-    const bool in_lat = x0 + 1 + lx < a.nx && y0 + 1 + ly < a.ny;
-    auto slot = [&](int zc) { return ((zc - z0 + 1) % 3) * PL; };
-    T fa[Q], fb[Q];                                  // pulls of this thread's node, planes z and z + 1
-    uint8_t fla = F_SOLID, flb = F_SOLID;
-    auto plane = [&](int zc, T *f, uint8_t &fl) {
-        T *dst = lds + slot(zc);
-        if (in_lat && zc >= z0 && zc < z1) {
-            const NodePos q = node_in_tile(a, tx, ty, zc);
-            uint32_t bm;
-            node_word(a, q, fl, bm);
-            T psi;
-            if (fl & F_SOLID) {
-                psi = (T)a.scpsi_solid;
-            } else {
-                T rho;
-                if (fl & F_INLET) {
-                    rho = (T)a.bcrho[fl >> 4];
-                } else {
-                    pull_node<Q, S, MEM>(a, q, bm, f);
-                    apply_zou_he<Q, S>(a, q, f);
-                    T ux, uy, uz;
-                    if constexpr (S::DEV) {
-                        T drho;
-                        moments_dev<Q, T>(f, drho, ux, uy, uz);
-                        rho = T(1) + drho;
-                    } else {
-                        moments<Q, T>(f, rho, ux, uy, uz);
-                    }
-                }
-                psi = sc_psi<T>(rho, a.scrho0);
-            }
-            dst[kin] = psi;
-        } else {
-            int gx = x0 + 1 + lx, gy = y0 + 1 + ly, gz = zc;
-            psi_neighbour<Q>(a, gx, gy, gz);
-            dst[kin] = psi_of_node<Q, S, MEM>(a, node_at(a, gx, gy, gz));
-        }
-        for (int r = threadIdx.x; r < nring; r += kThreads) {
-            int dx, dy;
-            if (r < W) {
-                dx = r; dy = 0;
-            } else if (r < 2 * W) {
-                dx = r - W; dy = Hh - 1;
-            } else {
-                const int r2 = r - 2 * W;
-                dx = r2 < Hh - 2 ? 0 : W - 1;
-                dy = 1 + (r2 < Hh - 2 ? r2 : r2 - (Hh - 2));
-            }
-            int gx = x0 + dx, gy = y0 + dy, gz = zc;
-            psi_neighbour<Q>(a, gx, gy, gz);
-            dst[dy * W + dx] = psi_of_node<Q, S, MEM>(a, node_at(a, gx, gy, gz));
-        }
-    };
-    plane(z0 - 1, fb, flb);                          // outside the run: nothing kept
-    plane(z0, fa, fla);
-    plane(z0 + 1, fb, flb);
-    __syncthreads();
-    for (int z = z0; z < z1; z++) {
-        NodePos p = node_in_tile(a, tx, ty, z);
-        if (a.round_mode) p.step = ldm(&a.ctr[0]) + dstep;
-        if (p.valid) {
-            if (fla & (F_SOLID | F_INLET)) {
-                step_body<Q, S, true, MACROS, MEM>(a, p, lds_raw);
-            } else {
-                const int poff[3] = {slot(z - 1), slot(z), slot(z + 1)};
-                sc_fluid_tail<Q, S, MACROS, MEM>(a, p, fa, lds, poff);
-            }
-        }
-        if (z + 1 < z1) {
-            __syncthreads();                         // every sweep of plane z has read slot(z - 1)
-#pragma unroll
-            for (int i = 0; i < Q; i++) fa[i] = fb[i];
-            fla = flb;
-            plane(z + 2, fb, flb);                   // into slot(z + 2) = slot(z - 1)
-            __syncthreads();
-        }
-    }
-}
-
 // Equilibrium initialisation from device fields.
 template <int Q, class S>
 __global__ __launch_bounds__(kThreads) void k_init_eq(typename S::raw *A, typename S::raw *B, uint32_t N,
@@ -471,7 +363,6 @@ struct lb_ctx {
     // launch geometry and load hints of the Shan-Chen density pass, tuned on
     // their own (its traffic is 19 loads and one store per node)
     int dbx_shift = 6, dby = 4, dxcd = 0, dmem = 0;
-    int scz = 0;                      // Shan-Chen: z-planes per workgroup of the one-pass step (0: two passes)
     void *dargs = nullptr;            // argument blocks: sweep from buffer 0 / 1, density pass from 0 / 1
     static constexpr size_t kArgBytes = 512;
 
@@ -571,19 +462,10 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
     size_t lds = 0;
-    if (sc) lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
-    if constexpr (Q == 19 && sizeof(typename S::raw) >= 4) {
-        if (sc && c->scz > 0) {
-            const int zper = c->scz;
-            const dim3 grd((unsigned)(a.ntx * a.nty * ((c->cfg.nz + zper - 1) / zper)));
-            if (macros) hipLaunchKernelGGL((k_step_scz<S, true>), grd, dim3(kThreads), lds, c->stream, pa, dstep, zper);
-            else if (c->mem == 1) hipLaunchKernelGGL((k_step_scz<S, false, 1>), grd, dim3(kThreads), lds, c->stream, pa, dstep, zper);
-            else if (c->mem == 2) hipLaunchKernelGGL((k_step_scz<S, false, 2>), grd, dim3(kThreads), lds, c->stream, pa, dstep, zper);
-            else hipLaunchKernelGGL((k_step_scz<S, false>), grd, dim3(kThreads), lds, c->stream, pa, dstep, zper);
-            return (int)hipGetLastError();
-        }
+    if (sc) {
+        lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
+        launch_density_t<Q, S>(c, src);
     }
-    if (sc) launch_density_t<Q, S>(c, src);
     if (sc) {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
         else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, true, false, 1>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
@@ -876,46 +758,6 @@ int autotune(lb_ctx *c) {
         if (!rc) rc = (int)hipGetLastError();
         if (!rc) rc = upload_args(c);
     }
-    // Shan-Chen D3Q19 with 4- / 8-byte populations: the one-pass step (tile
-    // shape x z-planes per workgroup x hints) against the two passes above
-    if (!rc && c->cfg.sc_G != 0.0f && c->Q == 19 && c->esz >= 4 && c->cfg.sc_fused != 0) {
-        auto time_steps = [&](float &ms) {
-            int lrc = launch_step(c, 0, false);   // warm
-            (void)hipEventRecord(e0, c->stream);
-            for (int r = 0; r < reps && !lrc; r++) lrc = launch_step(c, r & 1, false);
-            (void)hipEventRecord(e1, c->stream);
-            (void)hipEventSynchronize(e1);
-            ms = 1e30f;
-            if (!lrc) (void)hipEventElapsedTime(&ms, e0, e1);
-            return lrc;
-        };
-        float t_two = 1e30f;
-        if (c->cfg.sc_fused < 0) rc = time_steps(t_two);
-        const int bx2 = c->bx_shift, by2 = c->by, xcd2 = c->xcd, mem2 = c->mem;
-        float fbest = 1e30f;
-        int fbx = 64, fz = 16, fmem = 0;
-        for (int bx : {32, 64, 128}) {
-            if (rc) break;
-            if (c->cfg.block_x > 0 && bx != c->cfg.block_x) continue;
-            for (int zper : {8, 16, 32}) {
-                for (int mem = 0; mem < 3 && !rc; mem++) {
-                    if (c->cfg.mem_mode >= 0 && mem != c->cfg.mem_mode) continue;
-                    c->bx_shift = ilog2(bx); c->by = kThreads / bx; c->xcd = 0; c->mem = mem; c->scz = zper;
-                    rc = upload_args(c);
-                    float ms = 1e30f;
-                    if (!rc) rc = time_steps(ms);
-                    if (ms < fbest) { fbest = ms; fbx = bx; fz = zper; fmem = mem; }
-                }
-            }
-        }
-        if (!rc && fbest < t_two) {
-            c->bx_shift = ilog2(fbx); c->by = kThreads / fbx; c->xcd = 0; c->mem = fmem; c->scz = fz;
-        } else {
-            c->bx_shift = bx2; c->by = by2; c->xcd = xcd2; c->mem = mem2; c->scz = 0;
-        }
-        if (!rc) rc = (int)hipGetLastError();
-        if (!rc) rc = upload_args(c);
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -979,13 +821,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
     if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1 ||
         cfg->node_words > 1 ||
-        (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != 4 && cfg->time_block != 8) ||
-        cfg->sc_fused < -1 || cfg->sc_fused > 1)
+        (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != 4 && cfg->time_block != 8))
         return LB_ERR_INVALID;
-    // the one-pass Shan-Chen step: D3Q19, 4- or 8-byte populations
-    const bool scz_ok = cfg->lattice == 19 && cfg->sc_G != 0.0f && cfg->storage != LB_STORE_F16 &&
-                        cfg->storage != LB_STORE_BF16;
-    if (cfg->sc_fused == 1 && !scz_ok) return LB_ERR_UNSUPPORTED;
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
     c->cfg = *cfg;
@@ -1031,7 +868,6 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->layout = cfg->layout;
             c->packed = cfg->node_words;
             c->dbx_shift = c->bx_shift; c->dby = c->by; c->dxcd = c->xcd; c->dmem = c->mem == 2 ? 2 : 0;
-            c->scz = cfg->sc_fused == 1 ? 16 : 0;
             rc = upload_args(c);
         } else {
             rc = autotune(c);
@@ -1052,7 +888,6 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.layout = c->layout;
     c->cfg.node_words = c->packed;
     c->cfg.graph_chunk = c->graph_steps;
-    c->cfg.sc_fused = c->scz > 0 ? 1 : 0;
     *out = c;
     return 0;
 }

@@ -128,21 +128,6 @@ template <class S> LB_DEV NodePos node_of(const StepArgs<S> &a, int b) {
     return node_in_tile(a, tx, ty, z);
 }
 
-// A node given by its coordinates (inside the lattice), for work on nodes
-// outside the thread's own tile (per-link pull form only: e0 = e).
-template <class S> LB_DEV NodePos node_at(const StepArgs<S> &a, int x, int y, int z) {
-    NodePos p;
-    p.x = x;
-    p.y = y;
-    p.z = z;
-    p.valid = true;
-    p.id = (uint32_t)x + (uint32_t)a.nx * ((uint32_t)y + (uint32_t)a.ny * (uint32_t)z);
-    p.e = (uint32_t)x + a.sy * (uint32_t)y + a.sz * (uint32_t)z;
-    p.e0 = p.e;
-    p.step = 0;
-    return p;
-}
-
 // Global-memory accesses.  The pointers come from the argument block in
 // device memory, so the compiler cannot infer their address space and would
 // emit FLAT instructions, which also count against the scalar-load counter:
@@ -607,56 +592,6 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
         if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
         else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
     }
-    if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
-    store_node<Q, S, MEM>(a, p, f);
-}
-
-// The fluid-node part of step_body<Q, S, SC = true> after the pull and the
-// Zou-He correction (moments, body force, Shan-Chen force, collision, macros,
-// store) for populations already pulled into f, with the psi planes z - 1,
-// z, z + 1 of the tile at element offsets poff[0..2] of lds (the one-pass
-// Shan-Chen step keeps them in a ring of three slots).  The same operations
-// in the same order as step_body, so the results are identical.
-template <int Q, class S, bool MACROS, int MEM = 0>
-LB_DEV void sc_fluid_tail(const StepArgs<S> &a, const NodePos &p, typename S::T *f, const typename S::T *lds,
-                          const int poff[3]) {
-    typedef typename S::T T;
-    typedef Vel<Q> L;
-    const int bx = 1 << a.bx_shift;
-    T rho, ux, uy, uz;
-    T drho = T(0);
-    if constexpr (S::DEV) {
-        moments_dev<Q, T>(f, drho, ux, uy, uz);
-        rho = T(1) + drho;
-    } else {
-        moments<Q, T>(f, rho, ux, uy, uz);
-    }
-    const T ti = (T)a.ti;
-    T ex = ux, ey = uy, ez = uz;
-    if (a.force[0] != 0.0f) ex = ux + (T)a.force[0] / (rho * ti);
-    if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
-    if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
-    {
-        const int W = bx + 2;
-        const int lx = (threadIdx.x & (bx - 1)) + 1, ly = (threadIdx.x >> a.bx_shift) + 1;
-        T IF[3] = {T(0), T(0), T(0)};
-        static_for<1, Q>([&](auto I) {
-            constexpr int i = decltype(I)::value;
-            constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-            const int k = (lx + cx) + W * (ly + cy) + poff[Q == 19 ? 1 + cz : 1];
-            T wp = weight<Q, T>(L::wc(i)) * lds[k];
-            if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
-            if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
-            if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
-        });
-        const int kc = lx + W * ly + poff[1];
-        T gp = (T)(-a.scG) * lds[kc];
-        ex = ex + (gp * IF[0]) / (ti * rho);
-        ey = ey + (gp * IF[1]) / (ti * rho);
-        if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
-    }
-    if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
-    else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
     if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
     store_node<Q, S, MEM>(a, p, f);
 }

@@ -26,8 +26,7 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
                 zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
-                rest_closure=None, mem_mode=-1, layout=-1, round_mode=0, node_words=-1, time_block=-1,
-                sc_fused=-1):
+                rest_closure=None, mem_mode=-1, layout=-1, round_mode=0, node_words=-1, time_block=-1):
     """Plain-dict description of a lattice; the oracle binding reads the same dict.
 
     rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
@@ -59,16 +58,16 @@ def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, 
                 xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)),
                 mem_mode=int(mem_mode), layout=LAYOUT.get(layout, layout),
                 round_mode=ROUND.get(round_mode, round_mode), node_words=int(node_words),
-                time_block=int(time_block), sc_fused=int(sc_fused))
+                time_block=int(time_block))
 
 
 def to_lbconfig(d):
     c = LbConfig()
     defaults = dict(block_x=0, xcd_remap=-1, graph_chunk=0, mem_mode=-1, layout=-1, round_mode=0, node_words=-1,
-                    time_block=-1, sc_fused=-1)
+                    time_block=-1)
     for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid", "bb_full",
               "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure",
-              "mem_mode", "layout", "round_mode", "node_words", "time_block", "sc_fused"):
+              "mem_mode", "layout", "round_mode", "node_words", "time_block"):
         setattr(c, k, d[k] if k in d else defaults[k])
     for f in range(6):
         c.bc[f] = d["bc"][f]
@@ -125,7 +124,7 @@ class Solver:
         check(lib.lb_get_config(self._h, C.byref(c)), "lb_get_config")
         return dict(block_x=c.block_x, block_y=c.block_y, xcd_remap=c.xcd_remap, mem_mode=c.mem_mode, layout=c.layout,
                     node_words=c.node_words, time_block=c.time_block,
-                    graph_chunk=c.graph_chunk, sc_fused=c.sc_fused)
+                    graph_chunk=c.graph_chunk)
 
     def set_solid(self, solid):
         s = np.ascontiguousarray(np.asarray(solid, dtype=np.uint8).reshape(self.shape))

@@ -184,75 +184,6 @@ def test_fused_solver_matches_oracle(gpu, case, layout, words):
     sol.close()
 
 
-# One-pass Shan-Chen step (psi planes computed in the sweep's workgroups):
-# bit-identical to the oracle on periodic boxes (cubic and ragged against the
-# tile shapes, both layouts), and to the two-pass GPU step on a channel with
-# an inlet, an outflow face, a solid sphere and z halo planes (every psi
-# neighbour rule: wrap, clamp, halo, solid, inlet).
-def _scz_scene(kind):
-    from lattibol_amd import scenes
-    if kind == "spinodal32_f32":
-        return scenes.spinodal(n=32)
-    if kind == "ragged_f64":
-        cfg, _, init = scenes.spinodal(n=20, storage="f64")
-        cfg = gpu_make_config(cfg, (40, 28, 20))
-        init = dict(init, rho=scenes.uniform_noise(5, 40 * 28 * 20, 0.7, 0.01).reshape(20, 28, 40))
-        return cfg, np.zeros((20, 28, 40), np.uint8), init
-    if kind == "ragged_f32s":
-        cfg, _, init = scenes.spinodal(n=20, storage="f32s")
-        cfg = gpu_make_config(cfg, (36, 20, 24))
-        init = dict(init, rho=scenes.uniform_noise(6, 36 * 20 * 24, 0.7, 0.01).reshape(24, 20, 36))
-        return cfg, np.zeros((24, 20, 36), np.uint8), init
-    raise KeyError(kind)
-
-
-def gpu_make_config(cfg, shape):
-    c = dict(cfg)
-    c["nx"], c["ny"], c["nz"] = shape
-    return c
-
-
-@pytest.mark.parametrize("layout", [0, 1])
-@pytest.mark.parametrize("kind,steps", [("spinodal32_f32", 300), ("ragged_f64", 200), ("ragged_f32s", 200)])
-def test_one_pass_shan_chen_matches_oracle(gpu, kind, steps, layout):
-    cfg, solid, init = _scz_scene(kind)
-    cfg["layout"], cfg["sc_fused"] = layout, 1
-    print("one-pass Shan-Chen", kind, layout, flush=True)
-    sol, orc = _run_pair(gpu, cfg, solid, init, steps)
-    assert sol.launch_config["sc_fused"] == 1
-    _assert_pop_parity(sol.populations(), orc.A, cfg["storage"])
-    m, mo = sol.macros(), orc.macros()
-    for k in ("rho", "ux", "uy", "uz"):
-        np.testing.assert_array_equal(m[k], mo[k].astype(np.float32))
-    sol.close()
-
-
-def test_one_pass_shan_chen_equals_two_pass_on_channel(gpu):
-    from lattibol_amd import scenes
-    out = {}
-    for fused in (0, 1):
-        cfg, solid, init = scenes.sphere_channel(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), sc_G=-1.5,
-                                                 sc_psi_solid=0.3)
-        cfg["sc_fused"] = fused
-        print("channel with Shan-Chen, sc_fused", fused, flush=True)
-        sol = gpu.Solver(cfg, solid)
-        assert sol.launch_config["sc_fused"] == fused
-        sol.init_equilibrium(init["rho"], init["ux"], init["uy"], init.get("uz", 0.0))
-        sol.step(150)
-        out[fused] = (sol.populations(), sol.macros()["rho"])
-        sol.close()
-    np.testing.assert_array_equal(out[0][0], out[1][0])
-    np.testing.assert_array_equal(out[0][1], out[1][1])
-
-
-def test_one_pass_shan_chen_rejected_for_16_bit_storage(gpu):
-    from lattibol_amd import scenes
-    cfg, solid, init = scenes.spinodal(n=16, storage="f16")
-    cfg["sc_fused"] = 1
-    with pytest.raises(gpu.LatticeError):
-        gpu.Solver(cfg, solid)
-
-
 def _tb_scene(kind):
     from lattibol_amd import scenes
     from lattibol_amd.solver import make_config
