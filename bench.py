@@ -51,6 +51,17 @@ def parse():
     return ap.parse_args()
 
 
+def spinodal_traffic():
+    """HBM bytes per Shan-Chen step (density pass + sweep) measured with
+    rocprofv3 --pmc on the 192^3 FP32 workload (committed summary)."""
+    p = os.path.join(ROOT, "profiles", "d3q19_spinodal_192_f32_traffic.json")
+    try:
+        with open(p) as fh:
+            return json.load(fh).get("hbm_bytes_per_step")
+    except Exception:
+        return None
+
+
 def traffic_from_profiles(storage):
     """HBM bytes per sweep measured with rocprofv3 --pmc (committed summary)."""
     p = os.path.join(ROOT, "profiles", "d3q19_sphere_256x128x128_%s_traffic.json" % storage)
@@ -195,7 +206,7 @@ def bench_spinodal(args):
                   {"workload": "D3Q19 Shan-Chen spinodal decomposition 192^3 periodic, psi=1-exp(-rho), G=-5, "
                                "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out,
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": spinodal_traffic(),
                             "kernel": "k_density<19,f32> + k_step<19,f32,SC> per step", "bytes_per_update": 236},
                   cpu_baseline=cpu)
 
