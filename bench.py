@@ -162,7 +162,8 @@ def bench_spinodal(args):
         raise SystemExit("the spinodal workload reports the f32 run; include f32 in --spinodal-storages")
     for st, rm in runs:
         key = st + ("_sr" if rm else "")
-        cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm)
+        cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm, block_x=args.block_x,
+                                           xcd_remap=args.xcd_remap, mem_mode=args.mem_mode, layout=args.layout)
         sol = lb.Solver(cfg, solid)
         sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
         m0 = sol.reduce()[0]
@@ -178,9 +179,9 @@ def bench_spinodal(args):
         p5, p95 = np.percentile(rho, [5, 95])
         out[key] = {"mlups": round(n * steps / dt / 1e6, 1), "seconds": round(dt, 3),
                     "mass_drift_rel": float((m1 - m0) / m0), "rho_min": float(rho.min()), "rho_max": float(rho.max()),
-                    "rho_p5": float(p5), "rho_p95": float(p95)}
-        log("%s: %.1f MLUPS, drift %.2e, rho p5/p95 %.4f %.4f" % (key, out[key]["mlups"], out[key]["mass_drift_rel"],
-                                                                   p5, p95))
+                    "rho_p5": float(p5), "rho_p95": float(p95), "launch": sol.launch_config}
+        log("%s: %.1f MLUPS, drift %.2e, rho p5/p95 %.4f %.4f, launch %s" % (
+            key, out[key]["mlups"], out[key]["mass_drift_rel"], p5, p95, out[key]["launch"]))
         sol.close()
     ref = fields.get("f64")
     for key in out:
