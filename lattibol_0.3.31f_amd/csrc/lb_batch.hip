@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32S> 
                                                          const int2 *__restrict__ table, int src) {
     const int2 e = table[blockIdx.x];          // (lattice, local tile)
     const StepArgs<StF32S> &a = args[2 * e.x + src];
-    step_body<9, StF32S, false, MACROS, MEM>(a, node_of(a, e.y), nullptr);
+    step_body<9, StF32S, false, MACROS, MEM>(a, node_of(a, e.y));
 }
 
 // Per-tile residual partials of the velocity change since the snapshot, plus

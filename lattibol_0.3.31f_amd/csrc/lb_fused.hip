@@ -30,16 +30,10 @@ namespace {
 // of the stochastic-rounding stream was last advanced (k_bump).
 template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
 __global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa, uint32_t dstep) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
     const StepArgs<S> &a = *pa;
-    const int b = tile_of_block(a);
-    NodePos p = node_of(a, b);
+    NodePos p = node_of(a, tile_of_block(a));
     if (a.round_mode) p.step = ldm(&a.ctr[0]) + dstep;
-    if constexpr (SC) {
-        load_psi_tile<Q, S>(a, b, (typename S::T *)lds_raw, 1 << a.bx_shift, a.by);
-        __syncthreads();
-    }
-    step_body<Q, S, SC, MACROS, MEM>(a, p, lds_raw);
+    step_body<Q, S, SC, MACROS, MEM>(a, p);
 }
 
 // Advances the step counter of the stochastic-rounding stream after a launch
@@ -461,16 +455,12 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     StepArgs<S> a = make_args<S>(c, src);   // host copy: launch geometry only
     const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
-    size_t lds = 0;
     if (sc) {
-        lds = (size_t)((1 << c->bx_shift) + 2) * (c->by + 2) * (Q == 19 ? 3 : 1) * sizeof(typename S::T);
         launch_density_t<Q, S>(c, src);
-    }
-    if (sc) {
-        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
-        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, true, false, 1>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
-        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, true, false, 2>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
-        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), lds, c->stream, pa, dstep);
+        if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, true, false, 1>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else if (c->mem == 2) hipLaunchKernelGGL((k_step<Q, S, true, false, 2>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
+        else hipLaunchKernelGGL((k_step<Q, S, true, false>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
     } else {
         if (macros) hipLaunchKernelGGL((k_step<Q, S, false, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
         else if (c->mem == 1) hipLaunchKernelGGL((k_step<Q, S, false, false, 1>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);

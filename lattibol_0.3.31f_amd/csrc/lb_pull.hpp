@@ -462,7 +462,7 @@ LB_DEV void collide_dev(T *s, T drho, T ux, T uy, T uz, T ti, bool rest_closure)
     });
 }
 
-// Node whose psi stands at (gx, gy, gz) of a psi tile: wrap across periodic
+// Node whose psi a node at (gx, gy, gz) reads: wrap across periodic
 // (and halo) faces, clamp otherwise.
 template <int Q, class S> LB_DEV void psi_neighbour(const StepArgs<S> &a, int &gx, int &gy, int &gz) {
     if (gx < 0) gx = a.bc[0] == LB_BC_PERIODIC ? gx + a.nx : 0;
@@ -474,26 +474,6 @@ template <int Q, class S> LB_DEV void psi_neighbour(const StepArgs<S> &a, int &g
     if (Q == 19) {
         if (gz < 0) gz = (a.halo || a.bc[4] == LB_BC_PERIODIC) ? gz + a.nz : 0;
         if (gz >= a.nz) gz = (a.halo || a.bc[5] == LB_BC_PERIODIC) ? gz - a.nz : a.nz - 1;
-    }
-}
-
-// Shan-Chen psi tile in LDS: (BX+2) x (BY+2) x 3 planes around the block.
-template <int Q, class S>
-LB_DEV void load_psi_tile(const StepArgs<S> &a, int b, typename S::T *lds, int bx, int by) {
-    int tx = b % a.ntx;
-    int r = b / a.ntx;
-    int ty = r % a.nty;
-    int z = r / a.nty;
-    const int W = bx + 2, Hh = by + 2;
-    const int planes = Q == 19 ? 3 : 1;
-    const int total = W * Hh * planes;
-    const int x0 = (tx << a.bx_shift) - 1, y0 = ty * a.by - 1;
-    for (int k = threadIdx.x; k < total; k += kThreads) {
-        int dx = k % W, rest = k / W;
-        int dy = rest % Hh, dz = rest / Hh;
-        int gx = x0 + dx, gy = y0 + dy, gz = Q == 19 ? z - 1 + dz : 0;
-        psi_neighbour<Q>(a, gx, gy, gz);
-        lds[k] = ldm(&a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)]);
     }
 }
 
@@ -524,13 +504,14 @@ template <class S> LB_DEV void store_macros(const StepArgs<S> &a, uint32_t id, f
 }
 
 // ---------------------------------------------------------------------------
-// The body of the fused step for one node.  lds_raw holds the psi tile when
-// SC is set (loaded by the caller, all threads, before a barrier).
+// The body of the fused step for one node.  With SC set, the Shan-Chen force
+// reads the neighbours' psi (written by the density pass) straight from global
+// memory: the reads hit L1 / L2, and the sweep needs no LDS tile and no
+// workgroup barrier.
 template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
-LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned char *lds_raw) {
+LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p) {
     typedef typename S::T T;
     typedef Vel<Q> L;
-    const int bx = 1 << a.bx_shift;
     if (!p.valid) return;
     uint8_t fl;
     uint32_t bm;
@@ -570,21 +551,19 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p, const unsigned cha
         if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
         if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
         if constexpr (SC) {
-            const T *lds = (const T *)lds_raw;
-            const int W = bx + 2, Hh = a.by + 2;
-            const int lx = (threadIdx.x & (bx - 1)) + 1, ly = (threadIdx.x >> a.bx_shift) + 1;
             T IF[3] = {T(0), T(0), T(0)};
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-                int k = (lx + cx) + W * ((ly + cy) + Hh * (Q == 19 ? 1 + cz : 0));
-                T wp = weight<Q, T>(L::wc(i)) * lds[k];
+                int gx = p.x + cx, gy = p.y + cy, gz = p.z + cz;
+                psi_neighbour<Q>(a, gx, gy, gz);
+                T wp = weight<Q, T>(L::wc(i)) *
+                       (T)ldm(&a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)]);
                 if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
                 if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
                 if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
             });
-            const int kc = lx + W * (ly + Hh * (Q == 19 ? 1 : 0));
-            T gp = (T)(-a.scG) * lds[kc];
+            T gp = (T)(-a.scG) * (T)ldm(&a.psi[p.id]);
             ex = ex + (gp * IF[0]) / (ti * rho);
             ey = ey + (gp * IF[1]) / (ti * rho);
             if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
