@@ -32,6 +32,19 @@
  *   orc_ref_collision_*     equilibrium is a fixed point; per-node mass and
  *                           momentum conserved by BGK; Zou-He reproduces the
  *                           prescribed inlet velocity / outlet density.
+ *   orc_ref_collision_d2q9  Shan-Chen / adhesion pass (G < 0), read back as
+ *                           the post-collision momentum of rest populations
+ *                           (tests/test_oracle_shanchen_d2q9.py): psi store =
+ *                           4 exp(-200/rho); per-node force = the textbook
+ *                           -G psi(x) sum w psi(x+e) e by array shifts; total
+ *                           momentum conserved on a periodic box; uniform psi
+ *                           beside solids gives (G psi - Gads) psi sum_solid w e
+ *                           in closed form; a flat interface settles at the
+ *                           reference's own coexistence pair 85.7 / 524.39
+ *                           (simulation_d2q9.cu:150-153) and near the
+ *                           mechanical-stability integral.  Planted slips (a
+ *                           stencil sign, a transposed neighbour, an adhesion
+ *                           sign, the psi constant) each fail these checks.
  *   orc_ref_*_border        get->copy->apply with k subdomains equals the
  *                           single-domain run (single phase, no Zou-He).
  *   orc_ref_init_*          Sigma feq = rho, Sigma feq c = rho u (FP64 check).
