@@ -46,8 +46,10 @@ def parse():
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--heldout", type=int, default=256)
     ap.add_argument("--max-steady-steps", type=int, default=300000)
-    ap.add_argument("--spinodal-storages", default="f64,f32,bf16,f16,bf16_sr,f16_sr",
-                    help="storages of the spinodal workload (comma list; f64 is the reference of the error columns)")
+    ap.add_argument("--spinodal-storages", default="f64,f32,bf16,f16,bf16_nearest,f16_nearest",
+                    help="storages of the spinodal workload (comma list; f64 is the reference of the error "
+                         "columns; bf16 / f16 round stochastically, the Shan-Chen default; *_nearest round to "
+                         "nearest even, for comparison)")
     return ap.parse_args()
 
 
@@ -74,11 +76,39 @@ def traffic_from_profiles(storage):
         return None
 
 
+def cpu_info():
+    """Cores of the host and its CPU model (from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return {"nproc": os.cpu_count(), "cpus_allowed": affinity, "cpu_model": model}
+
+
+def cpu_threads():
+    """Oracle threads: the CPUs this process may use, at most 16 (the GPU
+    box's CPU share per GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(cfg, solid, init, steps_hint):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import numpy as np
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     O.set_threads(threads)
     orc = O.PullOracle(cfg, solid)
     orc.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
@@ -94,6 +124,7 @@ def cpu_baseline(cfg, solid, init, steps_hint):
     dt = time.perf_counter() - t0
     n = cfg["nx"] * cfg["ny"] * cfg["nz"]
     return {"value": round(n * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            **cpu_info(),
             "sample": "%d steps of the full %dx%dx%d lattice, FP32 oracle (C++ host loop, OpenMP over z-planes, "
                       "same arithmetic as the GPU path), %.1f s" % (steps, cfg["nx"], cfg["ny"], cfg["nz"], dt)}
 
@@ -118,8 +149,9 @@ def bench_cavity(args):
     sol = lb.Solver(cfg, solid)
     sol.init_equilibrium(init["rho"], init["ux"], init["uy"])
     sol.step(args.warmup)
-    sol.sync()
     steps = args.steps
+    sol.prepare(steps)
+    sol.sync()
     t0 = time.perf_counter()
     sol.step(steps)
     sol.sync()
@@ -156,17 +188,19 @@ def bench_spinodal(args):
     out = {}
     fields = {}
     want = set(args.spinodal_storages.split(","))
-    runs = [(st, rm) for st, rm in (("f64", 0), ("f32", 0), ("bf16", 0), ("f16", 0), ("bf16", 1), ("f16", 1))
-            if st + ("_sr" if rm else "") in want]
+    runs = [(st, rm) for st, rm in (("f64", "default"), ("f32", "default"), ("bf16", "default"),
+                                    ("f16", "default"), ("bf16", "nearest"), ("f16", "nearest"))
+            if st + ("_nearest" if rm == "nearest" else "") in want]
     if "f32" not in want:
         raise SystemExit("the spinodal workload reports the f32 run; include f32 in --spinodal-storages")
     for st, rm in runs:
-        key = st + ("_sr" if rm else "")
+        key = st + ("_nearest" if rm == "nearest" else "")
         cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm, block_x=args.block_x,
                                            xcd_remap=args.xcd_remap, mem_mode=args.mem_mode, layout=args.layout)
         sol = lb.Solver(cfg, solid)
         sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
         m0 = sol.reduce()[0]
+        sol.prepare(steps)
         sol.sync()
         t0 = time.perf_counter()
         sol.step(steps)
@@ -177,7 +211,10 @@ def bench_spinodal(args):
         fields[key] = rho.astype(np.float64)
         n = rho.size
         p5, p95 = np.percentile(rho, [5, 95])
+        esz = {"f64": 8, "f32": 4}.get(st, 2)
         out[key] = {"mlups": round(n * steps / dt / 1e6, 1), "seconds": round(dt, 3),
+                    "pct_hbm_at_2Q_bytes": round(100 * n * steps / dt * 38 * esz / (PEAK_HBM_GBS * 1e9), 2),
+                    "round_mode": sol.launch_config["round_mode"] if esz == 2 else None,
                     "mass_drift_rel": float((m1 - m0) / m0), "rho_min": float(rho.min()), "rho_max": float(rho.max()),
                     "rho_p5": float(p5), "rho_p95": float(p95), "launch": sol.launch_config}
         log("%s: %.1f MLUPS, drift %.2e, rho p5/p95 %.4f %.4f, launch %s" % (
@@ -192,11 +229,14 @@ def bench_spinodal(args):
         out[key]["rho_err_max_vs_fp64"] = float(np.abs(d).max())
         out[key]["phase_density_err_vs_fp64"] = [out[key]["rho_p5"] - out["f64"]["rho_p5"],
                                                  out[key]["rho_p95"] - out["f64"]["rho_p95"]]
-    # one Shan-Chen step = density pass (19 loads, psi store) + sweep (19 loads,
-    # 19 stores, psi tile): 2*19*4 + 19*4 + 2*4 = 236 algorithmic bytes per node
+    # Algorithmic bytes per update are 2*Q*sizeof(population) = 152 B (FP32),
+    # psi and masks excluded.  The two-pass Shan-Chen step moves more: the
+    # density pass reads the 19 populations again and stores psi, and the
+    # sweep reads psi: 152 + 76 + 4 + 4 = 236 B; the traffic counters measure
+    # hbm_bytes_per_step (profiles/d3q19_spinodal_192_f32_traffic.json).
     n = 192 ** 3
     ms = out["f32"]["seconds"] / steps * 1e3
-    achieved = 236.0 * n / (ms * 1e-3) / 1e9
+    achieved = 152.0 * n / (ms * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu_baseline:
         cfg, solid, init = scenes.spinodal(storage="f32")
@@ -208,7 +248,9 @@ def bench_spinodal(args):
                                "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out,
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": spinodal_traffic(),
-                            "kernel": "k_density<19,f32> + k_step<19,f32,SC> per step", "bytes_per_update": 236},
+                            "kernel": "k_density<19,f32> + k_step<19,f32,SC> per step", "bytes_per_update": 152,
+                            "two_pass_bytes_per_update": 236,
+                            "frac_at_two_pass_bytes": round(236.0 * n / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
                   cpu_baseline=cpu)
 
 
@@ -236,7 +278,7 @@ def bench_ensemble(args):
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         O.set_threads(threads)
         nodes = 0
         t0 = time.perf_counter()
@@ -246,7 +288,7 @@ def bench_ensemble(args):
             orc.run(200)
             nodes += sp["nx"] * sp["ny"] * 200
         tc = time.perf_counter() - t0
-        cpu = {"value": round(nodes / tc / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+        cpu = {"value": round(nodes / tc / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port", **cpu_info(),
                "sample": "the first 16 lattices of the ensemble, 200 steps each, FP32-deviation oracle, %.1f s" % tc}
     return common("D2Q9 batched-ensemble MLUPS", round(mlups, 1), "MLUPS", args, args.steps,
                   round(dt / args.steps * 1e3, 5), "fp32",
@@ -439,6 +481,7 @@ def main():
     launch = sol.launch_config
     print("bench: launch %s" % json.dumps(launch), file=sys.stderr, flush=True)
     sol.step(args.warmup)
+    sol.prepare(args.steps)          # graph capture stays out of the timed region
     sol.sync()
     t0 = time.perf_counter()
     sol.step(args.steps)

@@ -213,9 +213,12 @@ typedef struct {
     int mem_mode;           /* population loads/stores: -1 autotune, 0 plain,
                                1 non-temporal stores, 2 non-temporal loads + stores */
     int layout;             /* -1 autotune, LB_LAYOUT_SOA, LB_LAYOUT_ROWS */
-    int round_mode;         /* FP16 / BF16 stores: 0 round to nearest even, 1 stochastic
-                               rounding (unbiased; counter-based random bits per step,
-                               node and population, reproducible) */
+    int round_mode;         /* FP16 / BF16 stores: 0 default = stochastic rounding for
+                               Shan-Chen lattices (nearest-even there loses mass or
+                               stalls the phase separation) and nearest-even otherwise;
+                               1 stochastic rounding (unbiased; counter-based random
+                               bits per step, node and population, reproducible);
+                               2 nearest-even always */
     int node_words;         /* -1 autotune, 0 flag byte (+ link mask for nodes next to
                                walls / solids), 1 one packed 32-bit word per node */
     int time_block;         /* steps per launch through an LDS tile (temporal blocking,
@@ -244,8 +247,14 @@ int lb_init_uniform(lb_ctx *ctx, float rho, float ux, float uy, float uz);
  * whatever the device layout), host side. */
 int lb_set_populations(lb_ctx *ctx, const void *host);
 int lb_get_populations(lb_ctx *ctx, void *host);
-/* nsteps time steps; the last one also writes rho,u (the macros). */
+/* nsteps time steps; the last one also writes rho,u (the macros).  The
+ * steps run as captured hipGraphs: whole chunks of graph_chunk steps, then
+ * one graph for the rest of the call (cached per length; the first call of a
+ * new length captures and instantiates it). */
 int lb_step(lb_ctx *ctx, int nsteps);
+/* Captures and instantiates the graphs a following lb_step(ctx, nsteps) will
+ * launch, without running anything (keeps capture out of a timed loop). */
+int lb_step_prepare(lb_ctx *ctx, int nsteps);
 int lb_sync(lb_ctx *ctx);
 /* Macros of the last step to host (N floats each, any may be NULL). */
 int lb_get_macros(lb_ctx *ctx, float *rho, float *ux, float *uy, float *uz);
@@ -413,6 +422,68 @@ int lb_write_vti(const char *dir, int vtk_id, int device_id, int width, int heig
                  const float *ux, const float *uy, const float *uz, const float *rho);
 
 #ifdef __cplusplus
+}
+
+/* C++ callers: the reference's call sites (simulation_d2q9.cu:462-554,
+ * simulation_d3q19.cu:352-430) compile unchanged.  The launchers also accept
+ * any dim3-like grid / block type (HIP's dim3: unsigned x, y, z), converted
+ * field by field, and copy_border_devices_* take the stream by reference as
+ * copy_border.h:30-47 declares it.  These are inline overloads of the
+ * extern "C" entry points above; the library exports only those. */
+template <class D3> inline lb_dim3 lb_dim3_of(const D3 &d) {
+    lb_dim3 r = {(unsigned int)d.x, (unsigned int)d.y, (unsigned int)d.z};
+    return r;
+}
+template <class D3> inline void streaming_d2q9(DataD2Q9 *data, D3 grid, D3 block) {
+    streaming_d2q9(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3> inline void streaming_d3q19(DataD3Q19 *data, D3 grid, D3 block) {
+    streaming_d3q19(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3>
+inline void collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float w1, float w5, float tau_inv,
+                           float ux_in, float uy_in, float rho_out, float u_min, float u_max, float rho_min,
+                           float rho_max, D3 grid, D3 block, float Fx, float Fy, float G, float Gads) {
+    collision_d2q9(plot_data, data, w0, w1, w5, tau_inv, ux_in, uy_in, rho_out, u_min, u_max, rho_min, rho_max,
+                   lb_dim3_of(grid), lb_dim3_of(block), Fx, Fy, G, Gads);
+}
+template <class D3>
+inline void collision_d3q19(float *plot_data, DataD3Q19 *data, float w0, float w1, float w7, float tau_inv,
+                            float ux_in, float uy_in, float uz_in, float rho_out, float u_min, float u_max,
+                            float rho_min, float rho_max, D3 grid, D3 block) {
+    collision_d3q19(plot_data, data, w0, w1, w7, tau_inv, ux_in, uy_in, uz_in, rho_out, u_min, u_max, rho_min,
+                    rho_max, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3> inline void get_border_d2q9(DataD2Q9 *data, D3 grid, D3 block) {
+    get_border_d2q9(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3> inline void get_border_d3q19(DataD3Q19 *data, D3 grid, D3 block) {
+    get_border_d3q19(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3> inline void apply_border_d2q9(DataD2Q9 *data, D3 grid, D3 block) {
+    apply_border_d2q9(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+template <class D3> inline void apply_border_d3q19(DataD3Q19 *data, D3 grid, D3 block) {
+    apply_border_d3q19(data, lb_dim3_of(grid), lb_dim3_of(block));
+}
+inline void copy_border_devices_d2q9(float *nb_border_f1, float *nb_border_f5, float *nb_border_f8,
+                                     float *nb_border_f3, float *nb_border_f6, float *nb_border_f7,
+                                     float *border_f1, float *border_f5, float *border_f8, float *border_f3,
+                                     float *border_f6, float *border_f7, int height, lb_stream_t &s) {
+    copy_border_devices_d2q9(nb_border_f1, nb_border_f5, nb_border_f8, nb_border_f3, nb_border_f6, nb_border_f7,
+                             border_f1, border_f5, border_f8, border_f3, border_f6, border_f7, height, &s);
+}
+inline void copy_border_devices_d3q19(float *nb_border_f1, float *nb_border_f7, float *nb_border_f8,
+                                      float *nb_border_f9, float *nb_border_f10, float *nb_border_f2,
+                                      float *nb_border_f11, float *nb_border_f12, float *nb_border_f13,
+                                      float *nb_border_f14, float *border_f1, float *border_f7,
+                                      float *border_f8, float *border_f9, float *border_f10, float *border_f2,
+                                      float *border_f11, float *border_f12, float *border_f13,
+                                      float *border_f14, int height, int depth, lb_stream_t &s) {
+    copy_border_devices_d3q19(nb_border_f1, nb_border_f7, nb_border_f8, nb_border_f9, nb_border_f10, nb_border_f2,
+                              nb_border_f11, nb_border_f12, nb_border_f13, nb_border_f14, border_f1, border_f7,
+                              border_f8, border_f9, border_f10, border_f2, border_f11, border_f12, border_f13,
+                              border_f14, height, depth, &s);
 }
 #endif
 #endif /* LATTIBOL_AMD_H */

@@ -130,6 +130,9 @@ extern "C" int lb_deck_read_bmp(const char *bmp_path, int *width, int *height, u
     const uint32_t comp = rd32(&b[30]);
     uint32_t ncol = rd32(&b[46]);
     if (w <= 0 || h_signed == 0 || comp != 0) return LB_ERR_UNSUPPORTED;   // BI_RGB only
+    // a lattice edge beyond 65,536 nodes is no deck; also keeps -h_signed
+    // defined and stride * h far from overflow
+    if (w > (1 << 16) || h_signed > (1 << 16) || h_signed < -(1 << 16)) return LB_ERR_UNSUPPORTED;
     if (bpp != 1 && bpp != 4 && bpp != 8 && bpp != 24 && bpp != 32) return LB_ERR_UNSUPPORTED;
     const bool top_down = h_signed < 0;
     const int h = top_down ? -h_signed : h_signed;
@@ -137,7 +140,7 @@ extern "C" int lb_deck_read_bmp(const char *bmp_path, int *width, int *height, u
     const size_t pal = 14 + (size_t)hdr;
     const size_t stride = (((size_t)w * bpp + 31) / 32) * 4;
     if (bpp <= 8 && pal + 4 * (size_t)ncol > b.size()) return LB_ERR_INVALID;
-    if ((size_t)off + stride * (size_t)h > b.size()) return LB_ERR_INVALID;
+    if ((size_t)off > b.size() || stride * (size_t)h > b.size() - (size_t)off) return LB_ERR_INVALID;
     *width = w;
     *height = h;
     if (!solids) return 0;

@@ -348,6 +348,12 @@ struct lb_ctx {
     std::vector<uint8_t> solid;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     int graph_end[2] = {0, 1};        // buffer holding the result of graph[s]
+    // The end of an lb_step call (r plain steps, the macro-writing step and
+    // the rounding-stream bump) as a graph per (start buffer, r); the few
+    // most recent kept.
+    struct Tail { int cur, r, end; hipGraphExec_t g; };
+    std::vector<Tail> tails;
+    static constexpr size_t kMaxTails = 8;
     int graph_steps = 0;
     int bx_shift = 6, by = 4;
     int xcd = 0;
@@ -362,6 +368,7 @@ struct lb_ctx {
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
+        for (auto &t : tails) (void)hipGraphExecDestroy(t.g);
         void *ps[] = {buf[0], buf[1], flags, lmask, ctr, nodew, macros, snap, psi, halo[0], halo[1],
                       partial, dout, tmp_fields, dargs};
         for (void *p : ps) if (p) (void)hipFree(p);
@@ -370,12 +377,22 @@ struct lb_ctx {
     PopLayout pl() const { return pop_layout(layout, Q, cfg.nx, cfg.ny, cfg.nz); }
     void drop_graphs() {
         for (auto &g : graph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        for (auto &t : tails) (void)hipGraphExecDestroy(t.g);
+        tails.clear();
     }
 };
 
 namespace {
 
 int ilog2(int v) { int s = 0; while ((1 << s) < v) s++; return s; }
+
+// 16-bit stores round stochastically: asked for, or by default on Shan-Chen
+// lattices, whose per-step deviation increments fall below half a 16-bit
+// spacing and are rounded away with nearest-even.
+bool stochastic_rounding(const lb_config &g) {
+    if (g.storage != LB_STORE_F16 && g.storage != LB_STORE_BF16) return false;
+    return g.round_mode == 1 || (g.round_mode == 0 && g.sc_G != 0.0f);
+}
 
 // Argument block of the sweep (dens = false) or of the density pass from
 // buffer src; the two differ only in launch geometry.
@@ -407,7 +424,7 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src, bool dens = f
     for (int d = 0; d < 3; d++) a.force[d] = c->cfg.force[d];
     a.scG = c->cfg.sc_G; a.scrho0 = c->cfg.sc_rho0; a.scpsi_solid = c->cfg.sc_psi_solid;
     a.bb_full = c->cfg.bb_full; a.zou_he = c->cfg.zou_he; a.rest_closure = c->cfg.rest_closure;
-    a.round_mode = (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16) ? c->cfg.round_mode : 0;
+    a.round_mode = stochastic_rounding(c->cfg) ? 1 : 0;
     a.ctr = c->ctr;
     a.halo = c->cfg.halo_z && c->Q == 19;
     a.zh_ux = c->cfg.zh_ux_in; a.zh_rho = c->cfg.zh_rho_out;
@@ -504,8 +521,7 @@ void launch_density(lb_ctx *c, int src) {
 
 // After n sweeps launched with dstep 0..n-1: advance the rounding stream.
 int launch_bump(lb_ctx *c, uint32_t n) {
-    const bool sr = c->cfg.round_mode && (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16);
-    if (!sr || n == 0) return 0;
+    if (!stochastic_rounding(c->cfg) || n == 0) return 0;
     hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, c->stream, c->ctr, n);
     return (int)hipGetLastError();
 }
@@ -620,26 +636,69 @@ int upload_flags(lb_ctx *c) {
     return rc;
 }
 
+// The graph of one chunk of graph_steps plain steps from buffer src, built
+// on first use.
+int chunk_graph(lb_ctx *c, int src, hipGraphExec_t *out) {
+    hipGraphExec_t &ge = c->graph[src];
+    if (!ge) {
+        hipGraph_t g = nullptr;
+        int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (rc) return rc;
+        int lrc = 0;
+        c->graph_end[src] = launch_plain_steps(c, src, c->graph_steps, &lrc);
+        if (!lrc) lrc = launch_bump(c, (uint32_t)c->graph_steps);
+        rc = (int)hipStreamEndCapture(c->stream, &g);
+        if (!rc) rc = lrc;
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (rc) { ge = nullptr; return rc; }
+    }
+    *out = ge;
+    return 0;
+}
+
+// r plain steps from buffer src, then the macro-writing step, then the bump
+// of the rounding stream by r + 1; returns the buffer holding the result.
+int launch_tail(lb_ctx *c, int src, int r, int *rc) {
+    int end = launch_plain_steps(c, src, r, rc);
+    if (!*rc) *rc = launch_step(c, end, true, (uint32_t)r);
+    if (!*rc) *rc = launch_bump(c, (uint32_t)r + 1);
+    return 1 - end;
+}
+
+// The graph of an lb_step call's end from buffer src: r plain steps, the
+// macro-writing step and the rounding-stream bump (cached per (src, r)).
+int tail_graph(lb_ctx *c, int src, int r, lb_ctx::Tail *out) {
+    for (const lb_ctx::Tail &t : c->tails)
+        if (t.cur == src && t.r == r) { *out = t; return 0; }
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (rc) return rc;
+    int lrc = 0;
+    const int end = launch_tail(c, src, r, &lrc);
+    rc = (int)hipStreamEndCapture(c->stream, &g);
+    if (!rc) rc = lrc;
+    if (!rc) rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    if (c->tails.size() >= lb_ctx::kMaxTails) {
+        (void)hipGraphExecDestroy(c->tails.front().g);
+        c->tails.erase(c->tails.begin());
+    }
+    c->tails.push_back({src, r, end, ge});
+    *out = c->tails.back();
+    return 0;
+}
+
 int run_graph_or_eager(lb_ctx *c, int nsteps) {
     // nsteps plain steps (no macros), chunks of graph_steps through a graph.
     const int chunk = c->graph_steps;
     while (nsteps >= chunk && chunk > 0) {
-        hipGraphExec_t &ge = c->graph[c->cur];
-        if (!ge) {
-            hipGraph_t g = nullptr;
-            int rc = (int)hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-            if (rc) return rc;
-            int lrc = 0;
-            c->graph_end[c->cur] = launch_plain_steps(c, c->cur, chunk, &lrc);
-            launch_bump(c, (uint32_t)chunk);
-            rc = (int)hipStreamEndCapture(c->stream, &g);
-            if (!rc) rc = lrc;
-            if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-            rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (rc) { ge = nullptr; return rc; }
-        }
-        int rc = (int)hipGraphLaunch(ge, c->stream);
+        hipGraphExec_t ge = nullptr;
+        int rc = chunk_graph(c, c->cur, &ge);
+        if (!rc) rc = (int)hipGraphLaunch(ge, c->stream);
         if (rc) return rc;
         c->cur = c->graph_end[c->cur];
         nsteps -= chunk;
@@ -808,8 +867,11 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (N64 * (uint64_t)cfg->lattice >= (1ull << 31)) return LB_ERR_UNSUPPORTED;   // 32-bit node indices
     if (cfg->halo_z && cfg->lattice == 19 && (cfg->bc[4] != LB_BC_PERIODIC || cfg->bc[5] != LB_BC_PERIODIC))
         return LB_ERR_UNSUPPORTED;   // the single-slab ring realises periodic z
+    // Shan-Chen reads the psi of z-neighbours, which the halo planes do not
+    // carry: a ring of several slabs would need a psi-plane exchange too
+    if (cfg->halo_z && cfg->lattice == 19 && cfg->sc_G != 0.0f) return LB_ERR_UNSUPPORTED;
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
-    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 1 ||
+    if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 2 ||
         cfg->node_words > 1 ||
         (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != 4 && cfg->time_block != 8))
         return LB_ERR_INVALID;
@@ -878,6 +940,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.layout = c->layout;
     c->cfg.node_words = c->packed;
     c->cfg.graph_chunk = c->graph_steps;
+    if (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16)
+        c->cfg.round_mode = stochastic_rounding(c->cfg) ? 1 : 2;
     *out = c;
     return 0;
 }
@@ -991,15 +1055,33 @@ extern "C" int lb_get_populations(lb_ctx *c, void *host) {
 extern "C" int lb_step(lb_ctx *c, int nsteps) {
     if (!c || nsteps < 0) return LB_ERR_INVALID;
     if (nsteps == 0) return 0;
-    int rc = run_graph_or_eager(c, nsteps - 1);
+    const int chunk = c->graph_steps;
+    const int full = chunk > 0 ? (nsteps - 1) / chunk * chunk : 0;
+    int rc = run_graph_or_eager(c, full);
     if (rc) return rc;
-    rc = launch_step(c, c->cur, true);
-    if (!rc) rc = launch_bump(c, 1);
+    const int r = nsteps - 1 - full;
+    lb_ctx::Tail t{};
+    rc = tail_graph(c, c->cur, r, &t);
+    if (!rc) rc = (int)hipGraphLaunch(t.g, c->stream);
     if (rc) return rc;
-    c->cur = 1 - c->cur;
-    c->steps++;
+    c->cur = t.end;
+    c->steps += r + 1;
     c->macros_valid = true;
     return 0;
+}
+
+extern "C" int lb_step_prepare(lb_ctx *c, int nsteps) {
+    if (!c || nsteps < 0) return LB_ERR_INVALID;
+    if (nsteps == 0) return 0;
+    const int chunk = c->graph_steps;
+    int cur = c->cur, n = chunk > 0 ? (nsteps - 1) / chunk : 0;
+    for (int k = 0; k < n; k++) {
+        hipGraphExec_t ge = nullptr;
+        if (int rc = chunk_graph(c, cur, &ge)) return rc;
+        cur = c->graph_end[cur];
+    }
+    lb_ctx::Tail t{};
+    return tail_graph(c, cur, nsteps - 1 - n * chunk, &t);
 }
 
 extern "C" int lb_sync(lb_ctx *c) {

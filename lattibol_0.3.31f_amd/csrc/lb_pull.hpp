@@ -60,7 +60,7 @@ template <class S> struct StepArgs {
     float scG, scrho0, scpsi_solid;
     int bb_full, zou_he, halo;
     int rest_closure;             // feq_0 = rho - sum_{i>0} feq_i (exact mass balance)
-    int round_mode;               // 16-bit stores: 0 nearest-even, 1 stochastic
+    int round_mode;               // 16-bit stores: 0 nearest-even, 1 stochastic (resolved on the host)
     uint32_t *ctr;                // step counter of the stochastic-rounding stream
     float zh_ux, zh_rho;
     int bx_shift, by, ntx, nty, nblocks, xcd;

@@ -51,12 +51,18 @@ extern "C" int lb_device_alloc(size_t bytes, void **out) {
 
 extern "C" int lb_device_free(void *p) { return p ? (int)hipFree(p) : 0; }
 
+// The solver, batch and U-Net handles run on non-blocking streams, which do
+// not order against the null stream: wait for the whole device first, so an
+// upload never overwrites a buffer that queued kernels still read and a
+// download never reads one they still write.
 extern "C" int lb_memcpy_h2d(void *dst, const void *src, size_t bytes) {
-    return (int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    int rc = (int)hipDeviceSynchronize();
+    return rc ? rc : (int)hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
 }
 
 extern "C" int lb_memcpy_d2h(void *dst, const void *src, size_t bytes) {
-    return (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    int rc = (int)hipDeviceSynchronize();
+    return rc ? rc : (int)hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
 extern "C" int lb_device_synchronize(void) { return (int)hipDeviceSynchronize(); }

@@ -149,6 +149,7 @@ _PROTOS = {
     "lb_set_populations": (_i, [_vp, _vp]),
     "lb_get_populations": (_i, [_vp, _vp]),
     "lb_step": (_i, [_vp, _i]),
+    "lb_step_prepare": (_i, [_vp, _i]),
     "lb_sync": (_i, [_vp]),
     "lb_get_macros": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "lb_macros_device": (_i, [_vp, C.POINTER(_vp)]),

@@ -16,7 +16,9 @@ STORAGE = {"f32": 0, "f16": 1, "bf16": 2, "f64": 3, "f32s": 4}
 RAW_DTYPE = {0: np.float32, 1: np.uint16, 2: np.uint16, 3: np.float64, 4: np.float32}
 FACES = ("-x", "+x", "-y", "+y", "-z", "+z")
 LAYOUT = {"auto": -1, "soa": 0, "rows": 1}
-ROUND = {"nearest": 0, "stochastic": 1}
+# 16-bit rounding: "default" = stochastic on Shan-Chen lattices, nearest-even
+# elsewhere; "nearest" forces nearest-even (for comparisons)
+ROUND = {"default": 0, "stochastic": 1, "nearest": 2}
 
 # Lattice weights as float32 (simulation_d2q9.cu:56-58, simulation_d3q19.cu:54-56).
 W9 = np.array([16 / 36] + [4 / 36] * 4 + [1 / 36] * 4, dtype=np.float64)
@@ -26,7 +28,7 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
                 zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
-                rest_closure=None, mem_mode=-1, layout=-1, round_mode=0, node_words=-1, time_block=-1):
+                rest_closure=None, mem_mode=-1, layout=-1, round_mode="default", node_words=-1, time_block=-1):
     """Plain-dict description of a lattice; the oracle binding reads the same dict.
 
     rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
@@ -124,7 +126,8 @@ class Solver:
         check(lib.lb_get_config(self._h, C.byref(c)), "lb_get_config")
         return dict(block_x=c.block_x, block_y=c.block_y, xcd_remap=c.xcd_remap, mem_mode=c.mem_mode, layout=c.layout,
                     node_words=c.node_words, time_block=c.time_block,
-                    graph_chunk=c.graph_chunk)
+                    graph_chunk=c.graph_chunk,
+                    round_mode={1: "stochastic", 2: "nearest"}.get(c.round_mode, "default"))
 
     def set_solid(self, solid):
         s = np.ascontiguousarray(np.asarray(solid, dtype=np.uint8).reshape(self.shape))
@@ -150,6 +153,10 @@ class Solver:
     # -- stepping --------------------------------------------------------
     def step(self, n=1):
         check(lib.lb_step(self._h, int(n)), "lb_step")
+
+    def prepare(self, n):
+        """Instantiate the graphs step(n) will launch, without stepping."""
+        check(lib.lb_step_prepare(self._h, int(n)), "lb_step_prepare")
 
     def sync(self):
         check(lib.lb_sync(self._h), "lb_sync")

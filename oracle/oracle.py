@@ -38,7 +38,11 @@ class OrcCfg(C.Structure):
         for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid",
                   "bb_full", "zou_he", "zh_ux_in", "zh_rho_out", "rest_closure"):
             setattr(c, k, d[k])
-        c.round_mode = int(d.get("round_mode", 0))
+        # the library's round_mode: 0 default (stochastic for Shan-Chen with
+        # 16-bit storage), 1 stochastic, 2 nearest-even; the oracle's: 0 / 1
+        rm = int(d.get("round_mode", 0))
+        sixteen = d["storage"] in (1, 2)
+        c.round_mode = int(sixteen and (rm == 1 or (rm == 0 and d["sc_G"] != 0.0)))
         c.step = 0
         for f in range(6):
             c.bc[f] = d["bc"][f]

@@ -57,9 +57,12 @@ def pull_cases():
     out["pull_sphere16x8x6_halo_f32"] = (cfg, solid, init, 60)
     cfg, solid, init = scenes.sphere_channel(16, 8, 6, radius=2, center=(5, 4, 3), storage="bf16")
     out["pull_sphere16x8x6_halo_bf16"] = (cfg, solid, init, 60)
-    for st in ("f32", "f16"):
-        cfg, solid, init = scenes.spinodal(n=10, storage=st)
-        out["pull_spinodal10_%s" % st] = (cfg, solid, init, 40)
+    cfg, solid, init = scenes.spinodal(n=10, storage="f32")
+    out["pull_spinodal10_f32"] = (cfg, solid, init, 40)
+    cfg, solid, init = scenes.spinodal(n=10, storage="f16", round_mode="nearest")
+    out["pull_spinodal10_f16"] = (cfg, solid, init, 40)
+    cfg, solid, init = scenes.spinodal(n=10, storage="f16")          # default: stochastic rounding
+    out["pull_spinodal10_f16_sr"] = (cfg, solid, init, 40)
     spec = scenes.channel_spec(3, 1, (48, 24))
     cfg = scenes.channel_config(spec)
     out["pull_channel48x24_zouhe_f32s"] = (cfg, spec["solid"][None], dict(rho=1.0, ux=0.0, uy=0.0, uz=0.0), 200)

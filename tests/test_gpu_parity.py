@@ -137,6 +137,7 @@ FUSED_CASES = {
     "spinodal32_f32_1000": ("spinodal", dict(n=32), 1000),
     "spinodal24_f16": ("spinodal", dict(n=24, storage="f16"), 300),
     "spinodal24_bf16": ("spinodal", dict(n=24, storage="bf16"), 300),
+    "spinodal24_f16_nearest": ("spinodal", dict(n=24, storage="f16", round_mode="nearest"), 300),
     "spinodal20_f64": ("spinodal", dict(n=20, storage="f64"), 300),
     "cavity48_f32s": ("cavity", dict(n=48, storage="f32s"), 500),
     "sphere_halo_f32s": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f32s"), 300),
@@ -146,7 +147,35 @@ FUSED_CASES = {
     "sphere_halo_f16_sr": ("sphere", dict(nx=48, ny=24, nz=16, radius=4, center=(12, 12, 8), storage="f16",
                                           round_mode=1), 200),
     "spinodal24_bf16_sr": ("spinodal", dict(n=24, storage="bf16", round_mode=1), 300),
+    # Shan-Chen beside walls, a moving wall, an outflow face and solids
+    # (psi clamped at non-periodic faces, sc_psi_solid on solid nodes), ragged
+    # tile edges; D2Q9 and D3Q19
+    "sc9_walls_obstacle_f32": ("sc", dict(name="sc9"), 400),
+    "sc19_walls_box_f32": ("sc", dict(name="sc19"), 300),
+    "sc19_walls_box_bf16_sr": ("sc", dict(name="sc19", storage="bf16", round_mode=1), 200),
 }
+
+
+def _sc_scene(name, storage="f32", round_mode="default"):
+    from lattibol_amd import scenes
+    from lattibol_amd.solver import make_config
+    if name == "sc9":
+        nx, ny = 50, 37
+        cfg = make_config(9, (nx, ny), tau=1.0, sc_G=-5.0, sc_rho0=1.0, sc_psi_solid=0.4, storage=storage,
+                          round_mode=round_mode, bc={"-y": "wall", "+y": "wall"})
+        y, x = np.mgrid[0:ny, 0:nx]
+        solid = (((x - 20) ** 2 + (y - 15) ** 2) <= 36).astype(np.uint8)[None]
+        rho = scenes.uniform_noise(5, nx * ny, 0.7, 0.05).reshape(1, ny, nx)
+    else:
+        nx, ny, nz = 45, 20, 16
+        cfg = make_config(19, (nx, ny, nz), tau=1.0, sc_G=-5.0, sc_rho0=1.0, sc_psi_solid=0.2, storage=storage,
+                          round_mode=round_mode,
+                          bc={"-y": "wall", "+y": "outflow", "-z": "wall", "+z": "moving_wall"},
+                          bc_u={"+z": (0.02, 0.0, 0.0)})
+        solid = np.zeros((nz, ny, nx), np.uint8)
+        solid[4:9, 5:10, 10:16] = 1
+        rho = scenes.uniform_noise(6, nx * ny * nz, 0.7, 0.05).reshape(nz, ny, nx)
+    return cfg, solid, dict(rho=rho, ux=0.0, uy=0.0, uz=0.0)
 
 
 def _channel_scene(seed, index, size):
@@ -158,7 +187,7 @@ def _channel_scene(seed, index, size):
 def _scene(kind, kw):
     from lattibol_amd import scenes
     return {"cavity": scenes.cavity, "sphere": scenes.sphere_channel, "spinodal": scenes.spinodal,
-            "channel": _channel_scene}[kind](**kw)
+            "channel": _channel_scene, "sc": _sc_scene}[kind](**kw)
 
 
 @pytest.mark.parametrize("words", [0, 1])
