@@ -13,6 +13,8 @@ HEADER = os.path.join(ROOT, "include", "lattibol_amd.h")
 
 def declared_functions():
     text = open(HEADER).read()
+    # the C declarations; the C++ inline overloads after them export nothing
+    text = text.split("/* C++ callers")[0]
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = set()
     for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(", text, re.M):
