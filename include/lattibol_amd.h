@@ -86,7 +86,9 @@ enum {
     LB_ERR_SHAPE = -2,        /* grid*block does not match the lattice     */
     LB_ERR_UNSUPPORTED = -3,  /* configuration not implemented             */
     LB_ERR_NO_DEVICE = -4,    /* no usable GPU                             */
-    LB_ERR_NOT_READY = -5     /* state missing (e.g. macros never written)  */
+    LB_ERR_NOT_READY = -5,    /* state missing (e.g. macros never written)  */
+    LB_ERR_SCRATCH = -6       /* work exceeds the fixed scratch of a
+                                 non-allocating entry point                */
 };
 int lb_last_error(void);                 /* returns and clears the sticky error */
 const char *lb_error_string(int code);
@@ -369,6 +371,17 @@ int lb_unet_to_fields(lb_unet *u, const float *d_y, int B, const float *d_uin, c
  * W is BF16 [Co][9][Ci]; Ci % 16 == 0, Co % 32 == 0. */
 int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B, int H,
                     int W, int Ci, int Co, lb_stream_t stream);
+/* lb_conv3x3_bf16 without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under hipGraph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results.  A K-split launch keeps its shares in a fixed per-GPU device
+ * array of 16 MiB (4 Mi floats: e.g. batch 16 of 16x16 images with 256
+ * output channels split 4 ways); a call needing more returns LB_ERR_SCRATCH
+ * and launches nothing.  Asynchronous on `stream` (it never waits).  Every
+ * call on one GPU uses that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race; calls ordered on one stream do not. */
+int lb_conv3x3_bf16_noalloc(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out, int B,
+                            int H, int W, int Ci, int Co, lb_stream_t stream);
 /* Weight gradient of one 3x3 convolution (pad 1): d_dw[Co][9][Ci] (FP32) =
  * sum over pixels of d_dout (FP32 NHWC, rounded to BF16 as an MFMA operand)
  * times the shifted input d_in (BF16 NHWC).  method 0: per-element gathers;

@@ -30,6 +30,7 @@ extern "C" const char *lb_error_string(int code) {
     case LB_ERR_UNSUPPORTED: return "unsupported configuration";
     case LB_ERR_NO_DEVICE: return "no usable GPU device";
     case LB_ERR_NOT_READY: return "requested state has not been computed";
+    case LB_ERR_SCRATCH: return "work exceeds the fixed device scratch of the non-allocating entry point";
     default: break;
     }
     if (code > 0) return hipGetErrorString((hipError_t)code);

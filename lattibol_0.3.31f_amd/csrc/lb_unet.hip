@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -1677,6 +1678,69 @@ extern "C" int lb_conv3x3_bf16(int mode, const void *d_in, const void *d_w, cons
     if (!rc && scratch) rc = (int)hipStreamSynchronize(tmp.stream);
     tmp.stream = nullptr;
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// lb_conv3x3_bf16_noalloc: lb_conv3x3_bf16 for callers that cannot allocate
+// (code under hipGraph capture, serving loops).  The K-split shares of the
+// 16-pixel tile plan go to a fixed per-device array instead of a hipMalloc'd
+// buffer; every share writes its whole region before k_split_sum reads it,
+// so nothing needs clearing between calls.  Calls that overlap on one GPU
+// (two streams, or two host threads) share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchFloats = size_t(4) << 20;   // 16 MiB per GPU
+static __device__ __align__(256) float scratch[kNoAllocScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array).
+int noalloc_scratch(float **out) {
+    int dev = 0;
+    int rc = (int)hipGetDevice(&dev);
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return LB_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        rc = (int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc) return rc;
+        if (!p) return LB_ERR_INVALID;
+        g_scratch_ptr[dev] = (float *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int lb_conv3x3_bf16_noalloc(int mode, const void *d_in, const void *d_w, const float *d_bias, void *d_out,
+                                       int B, int H, int W, int Ci, int Co, lb_stream_t stream) {
+    if (Ci % 16 || Co % 32 || B < 1 || H < 1 || W < 1 || mode < 0 || mode > 3) return LB_ERR_INVALID;
+    if (!d_in || !d_w || !d_out || (mode % 2 == 0 && !d_bias)) return LB_ERR_INVALID;
+    if ((int64_t)B * H * W > INT32_MAX) return LB_ERR_INVALID;
+    const int npix = B * H * W;
+    const bool gather = mode >= 2;
+    const size_t need = gather ? 0 : conv_split_scratch(npix, H, W, Ci, Co);
+    if (need > kNoAllocScratchFloats) return LB_ERR_SCRATCH;
+    lb_unet tmp;
+    tmp.stream = (hipStream_t)stream;
+    if (need) {
+        int rc = noalloc_scratch(&tmp.partial);
+        if (rc) { tmp.stream = nullptr; return rc; }
+        tmp.partial_n = kNoAllocScratchFloats;
+    }
+    if (mode % 2 == 0)
+        conv_launch<bf16, bf16, true>(&tmp, (const bf16 *)d_in, (const bf16 *)d_w, d_bias, (bf16 *)d_out, npix, H, W, Ci,
+                                      Co, gather);
+    else
+        conv_launch<float, float, false>(&tmp, (const float *)d_in, (const bf16 *)d_w, nullptr, (float *)d_out, npix, H, W,
+                                         Ci, Co, gather);
+    tmp.partial = nullptr;   // not the handle's to free
+    tmp.stream = nullptr;
+    return (int)hipGetLastError();
 }
 
 // Weight gradient of one 3x3 convolution (test and diagnostic entry point;

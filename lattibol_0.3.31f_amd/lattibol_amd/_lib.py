@@ -182,6 +182,7 @@ _PROTOS = {
     "lb_unet_sync": (_i, [_vp]),
     "lb_unet_to_fields": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
     "lb_conv3x3_bf16": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
+    "lb_conv3x3_bf16_noalloc": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_conv3x3_wgrad_bf16": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_check_bf16_rounding": (_i, [C.POINTER(C.c_uint64)]),
     "lb_deck_read_params": (_i, [C.c_char_p, C.POINTER(LbDeckParams)]),
@@ -213,6 +214,7 @@ def bind_optional(name, res, args):
 
 # error codes (include/lattibol_amd.h)
 LB_OK, LB_ERR_INVALID, LB_ERR_SHAPE, LB_ERR_UNSUPPORTED, LB_ERR_NO_DEVICE, LB_ERR_NOT_READY = 0, -1, -2, -3, -4, -5
+LB_ERR_SCRATCH = -6
 
 
 def check(rc, what=""):
