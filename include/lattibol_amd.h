@@ -314,7 +314,10 @@ int lb_batch_init_fields_device_ex(lb_batch *b, const float *d_fields, int mode)
 int lb_batch_step(lb_batch *b, int nsteps);
 /* Run to residual < tol (checked every k steps) or max_steps; per lattice
  * steps taken and final residual.  Converged lattices retire; a lattice whose
- * |u| exceeds 0.5 or turns non-finite retires with residual +inf. */
+ * |u| exceeds 0.5 or turns non-finite retires with residual +inf.  The loop
+ * runs on the device: a captured graph of 20 chunks (k steps, residual,
+ * retirement, compaction of the block table, all kernels) is built once and
+ * replayed; the host reads the active count after each replay. */
 int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *steps_out,
                            double *res_out);
 /* Macros (rho, ux, uy) of lattice i to host. */

@@ -1,8 +1,11 @@
 // lb_batch.hip -- many independent D2Q9 channels of uneven size stepped
 // together: one launch per time step covers every tile of every active
-// lattice (a block table maps workgroup -> (lattice, tile)), the step loop
-// between convergence checks is captured in a hipGraph, and lattices whose
-// residual falls below the tolerance retire from the block table.
+// lattice (a block table maps workgroup -> (lattice, tile)).  Run to steady
+// state, the whole loop lives on the device: one captured hipGraph holds
+// many chunks of k steps, each followed by the residual, the retirement of
+// converged (or diverged) lattices and the compaction of the block table,
+// all by kernels; the graph is built once and replayed, and the host reads
+// two counters per replay.
 //
 // Each lattice runs the exact per-node code of the single-lattice solver
 // (step_body in lb_pull.hpp) with the configuration of lattice_config():
@@ -26,6 +29,11 @@ using namespace lb;
 namespace {
 
 constexpr int kBX = 64, kBY = 4;   // 256-thread tiles of 64 x 4 nodes
+
+// Per-lattice state of a run to steady state, kept on the device: still
+// stepping, steps taken, snapshot primed, buffer holding the current state,
+// last residual (1 until a snapshot exists, +inf when diverged).
+struct BatchState { int active, steps, snap_valid, cur; double res; };
 
 struct BatchLat {
     int nx, ny;
@@ -81,9 +89,13 @@ StepArgs<StF32S> lattice_args(const BatchLat &L, float *A, float *B, int src, co
     return a;
 }
 
+// counts (device, may be null): {active lattices, active tiles} of the
+// compacted table; workgroups past the active tiles exit at once.
 template <bool MACROS, int MEM>
 __global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32S> *__restrict__ args,
-                                                         const int2 *__restrict__ table, int src) {
+                                                         const int2 *__restrict__ table, int src,
+                                                         const int *__restrict__ counts) {
+    if (counts && (int)blockIdx.x >= counts[1]) return;
     const int2 e = table[blockIdx.x];          // (lattice, local tile)
     const StepArgs<StF32S> &a = args[2 * e.x + src];
     step_body<9, StF32S, false, MACROS, MEM>(a, node_of(a, e.y));
@@ -94,7 +106,9 @@ __global__ __launch_bounds__(kThreads) void k_batch_step(const StepArgs<StF32S> 
 // that the relative residual alone would call converged).
 __global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lats, const int2 *table,
                                                              const float *macros, float *snap,
-                                                             const uint8_t *flags, double *partial) {
+                                                             const uint8_t *flags, double *partial,
+                                                             const int *counts) {
+    if ((int)blockIdx.x >= counts[1]) return;
     const int2 e = table[blockIdx.x];
     const BatchLat L = lats[e.x];
     int tx = e.y % L.ntx, ty = e.y / L.ntx;
@@ -131,13 +145,17 @@ __global__ __launch_bounds__(kThreads) void k_batch_residual(const BatchLat *lat
     }
 }
 
-// One thread per active lattice: fixed-order sum over its tiles.  A lattice
-// whose |u| exceeds kUDiverged (Mach ~0.9) or turns non-finite reports +inf.
+// One thread per active lattice: fixed-order sum over its tiles, then the
+// lattice's state: steps += k, the buffer now holding it, the residual, and
+// retirement when converged or diverged.  A lattice whose |u| exceeds
+// kUDiverged (Mach ~0.9) or turns non-finite reports +inf.  The first check
+// after an initialisation only primes the snapshot (residual 1).
 constexpr double kUDiverged = 0.5;
-__global__ void k_batch_final(const int2 *ranges, int nact, const double *partial, double *res) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nact) return;
-    int2 r = ranges[k];   // (first block, count)
+__global__ void k_batch_final(const int *act, const int2 *ranges, const int *counts, const double *partial,
+                              BatchState *st, double tol, int k, int parity_after) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= counts[0]) return;
+    int2 r = ranges[a];   // (first block, count)
     double num = 0, den = 0, umax = 0;
     for (int b = r.x; b < r.x + r.y; b++) {
         num += partial[3 * b];
@@ -147,7 +165,53 @@ __global__ void k_batch_final(const int2 *ranges, int nact, const double *partia
     // divergence is judged on the velocities alone (umax is +inf when any
     // velocity is NaN or inf), never on the residual value itself
     double v = den > 0 ? sqrt(num / den) : 0.0;
-    res[k] = umax > kUDiverged ? INFINITY : v;
+    if (umax > kUDiverged) v = INFINITY;
+    BatchState &s = st[act[a]];
+    s.steps += k;
+    s.cur = parity_after;
+    const bool diverged = isinf(v);
+    s.res = (s.snap_valid || diverged) ? v : 1.0;
+    if (diverged || (s.snap_valid && v < tol)) s.active = 0;
+    s.snap_valid = 1;
+}
+
+// Compacted work list of the active lattices, in lattice order (one
+// workgroup): act[a] = lattice, ranges[a] = (first tile, tile count),
+// table[first + t] = (lattice, t), counts = {lattices, tiles}.  The same
+// table the host builds, so residual sums run in the same order.
+constexpr int kCompactThreads = 1024;
+__global__ __launch_bounds__(kCompactThreads) void k_batch_compact(const BatchLat *lats, const BatchState *st, int n,
+                                                                   int *act, int2 *ranges, int2 *table, int *counts) {
+    __shared__ int s_l[kCompactThreads], s_t[kCompactThreads];
+    __shared__ int base_l, base_t;
+    const int t = threadIdx.x;
+    if (t == 0) { base_l = 0; base_t = 0; }
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += kCompactThreads) {
+        const int i = c0 + t;
+        const int on = (i < n && st[i].active) ? 1 : 0;
+        const int nt = on ? lats[i].ntx * lats[i].nty : 0;
+        s_l[t] = on;
+        s_t[t] = nt;
+        __syncthreads();
+        for (int off = 1; off < kCompactThreads; off <<= 1) {   // inclusive scans
+            const int al = t >= off ? s_l[t - off] : 0, at = t >= off ? s_t[t - off] : 0;
+            __syncthreads();
+            s_l[t] += al;
+            s_t[t] += at;
+            __syncthreads();
+        }
+        if (on) {
+            const int a = base_l + s_l[t] - 1, first = base_t + s_t[t] - nt;
+            act[a] = i;
+            ranges[a] = int2{first, nt};
+            for (int q = 0; q < nt; q++) table[first + q] = int2{i, q};
+        }
+        __syncthreads();
+        if (t == 0) { base_l += s_l[kCompactThreads - 1]; base_t += s_t[kCompactThreads - 1]; }
+        __syncthreads();
+    }
+    if (t == 0) { counts[0] = base_l; counts[1] = base_t; }
 }
 
 // Initial populations from (rho, ux, uy) fields (NULL: rest).  neq = 0:
@@ -223,19 +287,23 @@ struct lb_batch {
     float *A = nullptr, *B = nullptr, *macros = nullptr, *snap = nullptr;
     uint8_t *flags = nullptr;
     uint32_t *lmask = nullptr;
-    double *partial = nullptr, *d_res = nullptr;
+    double *partial = nullptr;
+    BatchState *d_state = nullptr;
+    int *d_act = nullptr, *d_counts = nullptr;   // compacted active lattices; {lattices, tiles}
     std::vector<int2> table, ranges;
     std::vector<int> act_list;
-    int nblocks = 0;
-    hipGraphExec_t graph = nullptr;
-    int graph_k = 0, graph_parity = -1;
+    int nblocks = 0, total_tiles = 0;
+    // graphs of the device-side loop: `chunks` chunks of k steps from buffer parity
+    struct LoopGraph { int chunks, k, parity; double tol; hipGraphExec_t g; };
+    std::vector<LoopGraph> loops;
     std::vector<char> macros_valid;
     int layout = LB_LAYOUT_SOA;    // population layout of every lattice (autotuned)
     int mem = 0;                   // load / store hints (autotuned, see ldm / stm)
 
     ~lb_batch() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        void *ps[] = {d_lats, d_args, d_table, d_ranges, A, B, macros, snap, flags, lmask, partial, d_res};
+        for (auto &l : loops) (void)hipGraphExecDestroy(l.g);
+        void *ps[] = {d_lats, d_args, d_table, d_ranges, A, B, macros, snap, flags, lmask, partial, d_state, d_act,
+                      d_counts};
         for (void *p : ps) if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -257,7 +325,6 @@ int rebuild_table(lb_batch *b, bool all_lattices = false) {
         for (int t = 0; t < nt; t++) b->table.push_back(int2{i, t});
     }
     b->nblocks = (int)b->table.size();
-    if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
     if (b->nblocks == 0) return 0;
     int rc = (int)hipMemcpyAsync(b->d_table, b->table.data(), b->table.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->d_ranges, b->ranges.data(), b->ranges.size() * sizeof(int2), hipMemcpyHostToDevice, b->stream);
@@ -265,12 +332,16 @@ int rebuild_table(lb_batch *b, bool all_lattices = false) {
     return rc;
 }
 
-void launch_batch_step(lb_batch *b, int parity, bool macros) {
-    const dim3 g(b->nblocks), t(kThreads);
-    if (macros) hipLaunchKernelGGL((k_batch_step<true, 0>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
-    else if (b->mem == 1) hipLaunchKernelGGL((k_batch_step<false, 1>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
-    else if (b->mem == 2) hipLaunchKernelGGL((k_batch_step<false, 2>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
-    else hipLaunchKernelGGL((k_batch_step<false, 0>), g, t, 0, b->stream, b->d_args, b->d_table, parity);
+// One step of the lattices in the block table: the host-built table
+// (counts null, nblocks workgroups) or the device-compacted one (a grid over
+// every tile of every lattice, workgroups past counts[1] idle).
+void launch_batch_step(lb_batch *b, int parity, bool macros, const int *counts = nullptr) {
+    const dim3 g(counts ? b->total_tiles : b->nblocks), t(kThreads);
+    const StepArgs<StF32S> *a = b->d_args;
+    if (macros) hipLaunchKernelGGL((k_batch_step<true, 0>), g, t, 0, b->stream, a, b->d_table, parity, counts);
+    else if (b->mem == 1) hipLaunchKernelGGL((k_batch_step<false, 1>), g, t, 0, b->stream, a, b->d_table, parity, counts);
+    else if (b->mem == 2) hipLaunchKernelGGL((k_batch_step<false, 2>), g, t, 0, b->stream, a, b->d_table, parity, counts);
+    else hipLaunchKernelGGL((k_batch_step<false, 0>), g, t, 0, b->stream, a, b->d_table, parity, counts);
 }
 
 // Both argument blocks of every lattice for the current layout.
@@ -320,11 +391,42 @@ int autotune_batch(lb_batch *b) {
     return rc;
 }
 
-void launch_batch_residual(lb_batch *b) {
-    hipLaunchKernelGGL(k_batch_residual, dim3(b->nblocks), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table, b->macros,
-                       b->snap, b->flags, b->partial);
-    int na = (int)b->act_list.size();
-    hipLaunchKernelGGL(k_batch_final, dim3((na + 63) / 64), dim3(64), 0, b->stream, b->d_ranges, na, b->partial, b->d_res);
+// One chunk of the device-side loop: k steps from buffer parity (the last
+// writes the macros), residual partials, per-lattice state, compaction.
+void launch_loop_chunk(lb_batch *b, int parity, int k, double tol) {
+    int par = parity;
+    for (int s = 0; s < k; s++) { launch_batch_step(b, par, s == k - 1, b->d_counts); par = 1 - par; }
+    hipLaunchKernelGGL(k_batch_residual, dim3(b->total_tiles), dim3(kThreads), 0, b->stream, b->d_lats, b->d_table,
+                       b->macros, b->snap, b->flags, b->partial, b->d_counts);
+    hipLaunchKernelGGL(k_batch_final, dim3((b->n + 63) / 64), dim3(64), 0, b->stream, b->d_act, b->d_ranges, b->d_counts,
+                       b->partial, b->d_state, tol, k, par);
+    hipLaunchKernelGGL(k_batch_compact, dim3(1), dim3(kCompactThreads), 0, b->stream, b->d_lats, b->d_state, b->n, b->d_act,
+                       b->d_ranges, b->d_table, b->d_counts);
+}
+
+// The captured loop of `chunks` chunks from buffer parity (built once).
+int loop_graph(lb_batch *b, int chunks, int k, int parity, double tol, hipGraphExec_t *out) {
+    for (const auto &l : b->loops)
+        if (l.chunks == chunks && l.k == k && l.parity == parity && l.tol == tol) { *out = l.g; return 0; }
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    int rc = (int)hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (rc) return rc;
+    int par = parity;
+    for (int c = 0; c < chunks; c++) {
+        launch_loop_chunk(b, par, k, tol);
+        if (k & 1) par = 1 - par;
+    }
+    const int lrc = (int)hipGetLastError();
+    rc = (int)hipStreamEndCapture(b->stream, &g);
+    if (!rc) rc = lrc;
+    if (!rc) rc = (int)hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    if (b->loops.size() >= 4) { (void)hipGraphExecDestroy(b->loops.front().g); b->loops.erase(b->loops.begin()); }
+    b->loops.push_back({chunks, k, parity, tol, ge});
+    *out = ge;
+    return 0;
 }
 
 }  // namespace
@@ -361,6 +463,8 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
         pop += 9ull * L.N;
         node += L.N;
     }
+    if (total_tiles >= (size_t)INT32_MAX) { delete b; return LB_ERR_UNSUPPORTED; }
+    b->total_tiles = (int)total_tiles;
     b->pop_total = pop;
     b->node_total = node;
     b->cur.assign(n, 0);
@@ -379,7 +483,9 @@ extern "C" int lb_batch_create(int n, const lb_batch_lattice *lat, const uint8_t
     if (!rc) rc = (int)hipMalloc((void **)&b->flags, node);
     if (!rc) rc = (int)hipMalloc((void **)&b->lmask, node * 4);
     if (!rc) rc = (int)hipMalloc((void **)&b->partial, total_tiles * 3 * sizeof(double));
-    if (!rc) rc = (int)hipMalloc((void **)&b->d_res, n * sizeof(double));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_state, n * sizeof(BatchState));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_act, n * sizeof(int));
+    if (!rc) rc = (int)hipMalloc((void **)&b->d_counts, 2 * sizeof(int));
     if (!rc) rc = (int)hipMemcpyAsync(b->d_lats, b->lats.data(), n * sizeof(BatchLat), hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->flags, flags_all.data(), node, hipMemcpyHostToDevice, b->stream);
     if (!rc) rc = (int)hipMemcpyAsync(b->lmask, lmask_all.data(), node * 4, hipMemcpyHostToDevice, b->stream);
@@ -448,59 +554,47 @@ extern "C" int lb_batch_step(lb_batch *b, int nsteps) {
     return rc;
 }
 
+// Chunks of k steps per replay of the loop graph (about 0.2 s of work for
+// the 512-lattice ensemble at k = 100): the host checks once per replay.
+constexpr int kLoopChunks = 20;
+
 extern "C" int lb_batch_run_to_steady(lb_batch *b, double tol, int k, int max_steps, int *steps_out, double *res_out) {
     if (!b || k < 2 || max_steps < 0) return LB_ERR_INVALID;
-    std::vector<int> steps(b->n, 0);
-    std::vector<double> res(b->n, 1.0);
-    std::vector<double> hres(b->n);
-    int done = 0;
-    int rc = 0;
-    while (b->nblocks > 0 && done + k <= max_steps) {
-        // graph: k-1 plain steps, one step writing macros, residual kernels
-        if (!b->graph || b->graph_k != k || b->graph_parity != b->parity) {
-            if (b->graph) { (void)hipGraphExecDestroy(b->graph); b->graph = nullptr; }
-            hipGraph_t g = nullptr;
-            rc = (int)hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
-            if (rc) return rc;
-            int par = b->parity;
-            for (int s = 0; s < k; s++) { launch_batch_step(b, par, s == k - 1); par = 1 - par; }
-            launch_batch_residual(b);
-            rc = (int)hipStreamEndCapture(b->stream, &g);
-            if (rc) return rc;
-            rc = (int)hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (rc) { b->graph = nullptr; return rc; }
-            b->graph_k = k;
-            b->graph_parity = b->parity;
-        }
-        rc = (int)hipGraphLaunch(b->graph, b->stream);
-        if (!rc) rc = (int)hipMemcpyAsync(hres.data(), b->d_res, b->act_list.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream);
-        if (!rc) rc = (int)hipStreamSynchronize(b->stream);
-        if (rc) return rc;
-        if (k & 1) b->parity = 1 - b->parity;
-        done += k;
-        bool changed = false;
-        for (size_t a = 0; a < b->act_list.size(); a++) {
-            int i = b->act_list[a];
-            steps[i] += k;
-            b->cur[i] = b->parity;
-            b->macros_valid[i] = 1;
-            const bool diverged = std::isinf(hres[a]);
-            res[i] = (b->snap_valid[i] || diverged) ? hres[a] : 1.0;
-            if (diverged || (b->snap_valid[i] && hres[a] < tol)) {   // converged, or diverged (+inf)
-                b->active[i] = 0;
-                changed = true;
-            }
-            b->snap_valid[i] = 1;
-        }
-        if (changed) {
-            rc = rebuild_table(b);
-            if (rc) return rc;
-        }
+    // device state from the host's view (lattices retired earlier stay so)
+    std::vector<BatchState> st(b->n);
+    for (int i = 0; i < b->n; i++) st[i] = BatchState{b->active[i], 0, b->snap_valid[i], b->cur[i], 1.0};
+    int rc = (int)hipMemcpyAsync(b->d_state, st.data(), st.size() * sizeof(BatchState), hipMemcpyHostToDevice, b->stream);
+    if (!rc) {
+        hipLaunchKernelGGL(k_batch_compact, dim3(1), dim3(kCompactThreads), 0, b->stream, b->d_lats, b->d_state, b->n,
+                           b->d_act, b->d_ranges, b->d_table, b->d_counts);
+        rc = (int)hipGetLastError();
     }
-    if (steps_out) for (int i = 0; i < b->n; i++) steps_out[i] = steps[i];
-    if (res_out) for (int i = 0; i < b->n; i++) res_out[i] = res[i];
-    return 0;
+    int counts[2] = {0, 0};
+    if (!rc) rc = (int)hipMemcpyAsync(counts, b->d_counts, sizeof(counts), hipMemcpyDeviceToHost, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    int done = 0;
+    while (!rc && counts[0] > 0 && done + k <= max_steps) {
+        const int chunks = std::min(kLoopChunks, (max_steps - done) / k);
+        hipGraphExec_t g = nullptr;
+        rc = loop_graph(b, chunks, k, b->parity, tol, &g);
+        if (!rc) rc = (int)hipGraphLaunch(g, b->stream);
+        if (!rc) rc = (int)hipMemcpyAsync(counts, b->d_counts, sizeof(counts), hipMemcpyDeviceToHost, b->stream);
+        if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+        if (((long long)chunks * k) & 1) b->parity = 1 - b->parity;
+        done += chunks * k;
+    }
+    if (!rc) rc = (int)hipMemcpyAsync(st.data(), b->d_state, st.size() * sizeof(BatchState), hipMemcpyDeviceToHost, b->stream);
+    if (!rc) rc = (int)hipStreamSynchronize(b->stream);
+    if (rc) return rc;
+    for (int i = 0; i < b->n; i++) {
+        b->active[i] = (char)st[i].active;
+        b->snap_valid[i] = (char)st[i].snap_valid;
+        b->cur[i] = st[i].cur;
+        if (st[i].steps > 0) b->macros_valid[i] = 1;
+        if (steps_out) steps_out[i] = st[i].steps;
+        if (res_out) res_out[i] = st[i].res;
+    }
+    return rebuild_table(b);   // the host's table for lb_batch_step
 }
 
 extern "C" int lb_batch_get_macros(lb_batch *b, int i, float *rho, float *ux, float *uy) {

@@ -9,9 +9,11 @@ from ._lib import LbBatchLattice, check, lib
 class Batch:
     """Many independent D2Q9 channels stepped together on one GPU.
 
-    Each lattice: equilibrium inlet (rho 1, u_in) at x=0, zero-gradient
-    outflow at x=nx-1, half-way bounce-back walls at rows 0 and ny-1, solid
-    obstacles with half-way bounce-back.
+    Each lattice: the reference's Zou-He velocity inlet (u_in, 0) at x=0 and
+    pressure outlet rho=1 at x=nx-1, half-way bounce-back walls at rows 0 and
+    ny-1, solid obstacles with half-way bounce-back, FP32 deviation storage.
+    run_to_steady keeps the whole loop on the device (one captured graph of
+    many k-step chunks with on-device retirement and table compaction).
     """
 
     def __init__(self, specs):
