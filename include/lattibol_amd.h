@@ -156,7 +156,8 @@ typedef struct {
     float viscosity;              /* tau_inv = 1/(3 nu + 0.5)                */
     float Fx, Fy, Fz;             /* body force (D2Q9)                       */
     float G, Gads;                /* Shan-Chen coupling / adhesion (D2Q9)    */
-    int zou_he;                   /* 1 = reference inlet/outlet (always on there) */
+    int zou_he;                   /* 1 = reference inlet/outlet (its launchers always
+                                     apply them); 0 = x periodic over the ring */
     int cursor_x, cursor_y, cursor_z;
 } lb_ref_params;
 typedef struct lb_refsim lb_refsim;

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_stream19(DataD3Q19 d) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_ref_coll9(float *plot, DataD2Q9 d, float w0, float w1,
                                                       float w5, float ti, float ux_in,
-                                                      float rho_out, float Fx, float Fy, float G) {
+                                                      float rho_out, float Fx, float Fy, float G, int zou_he) {
     const int W = d.width, H = d.height;
     const int64_t N = (int64_t)W * H;
     int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -98,8 +98,10 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll9(float *plot, DataD2Q9 d, f
     if (!d.solid[o]) {
         float v[9] = {ldn(&d.f0[o]), ldn(&d.f3[o]), ldn(&d.f4[o]), ldn(&d.f1[o]), ldn(&d.f2[o]),
                       ldn(&d.f7[o]), ldn(&d.f8[o]), ldn(&d.f5[o]), ldn(&d.f6[o])};
-        if (x == 0 && d.dev == 0) zouhe9_in(v, ux_in);
-        else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe9_out(v, rho_out);
+        if (zou_he) {
+            if (x == 0 && d.dev == 0) zouhe9_in(v, ux_in);
+            else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe9_out(v, rho_out);
+        }
         float rho, ux, uy;
         mom9(v, rho, ux, uy);
         float ex = ux, ey = uy;
@@ -183,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_ref_sc9(float *plot, DataD2Q9 d, flo
 // Collision D3Q19 (collision.cu:357-552).
 __global__ __launch_bounds__(kBlock) void k_ref_coll19(float *plot, DataD3Q19 d, float w0,
                                                        float w1, float w7, float ti,
-                                                       float ux_in, float rho_out) {
+                                                       float ux_in, float rho_out, int zou_he) {
     const int W = d.width;
     const int64_t N = (int64_t)W * d.height * d.depth;
     int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -195,8 +197,10 @@ __global__ __launch_bounds__(kBlock) void k_ref_coll19(float *plot, DataD3Q19 d,
                        ldn(&d.f6[o]),  ldn(&d.f5[o]),  ldn(&d.f12[o]), ldn(&d.f11[o]), ldn(&d.f14[o]),
                        ldn(&d.f13[o]), ldn(&d.f8[o]),  ldn(&d.f7[o]),  ldn(&d.f10[o]), ldn(&d.f9[o]),
                        ldn(&d.f18[o]), ldn(&d.f17[o]), ldn(&d.f16[o]), ldn(&d.f15[o])};
-        if (x == 0 && d.dev == 0) zouhe19_in(v, ux_in);
-        else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe19_out(v, rho_out);
+        if (zou_he) {
+            if (x == 0 && d.dev == 0) zouhe19_in(v, ux_in);
+            else if (x == W - 1 && d.dev == d.deviceCount - 1) zouhe19_out(v, rho_out);
+        }
         float rho, ux, uy, uz;
         mom19(v, rho, ux, uy, uz);
         d.ux[o] = ux; d.uy[o] = uy; d.uz[o] = uz; d.rho[o] = rho;
@@ -286,16 +290,18 @@ extern "C" void streaming_d3q19(DataD3Q19 *data, lb_dim3 grid, lb_dim3 block) {
     check_launch();
 }
 
-extern "C" void collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float w1, float w5,
-                               float tau_inv, float ux_in, float uy_in, float rho_out, float u_min,
-                               float u_max, float rho_min, float rho_max, lb_dim3 grid,
-                               lb_dim3 block, float Fx, float Fy, float G, float Gads) {
-    (void)uy_in; (void)u_min; (void)u_max; (void)rho_min; (void)rho_max;
+// The collision launches with the Zou-He inlet / outlet on (the reference's
+// launchers, collision.cu:211-262, always apply them) or off (x periodic over
+// the ring of subdomains: the host driver's switch).
+namespace {
+void launch_collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float w1, float w5, float tau_inv,
+                           float ux_in, float rho_out, lb_dim3 grid, lb_dim3 block, float Fx, float Fy, float G,
+                           float Gads, int zou_he) {
     if (!data || !plot_data) { lb_set_error(LB_ERR_INVALID); return; }
     if (!shape_ok(grid, block, data->width, data->height, 1)) { lb_set_error(LB_ERR_SHAPE); return; }
     unsigned nb = nblocks((int64_t)data->width * data->height);
     hipLaunchKernelGGL(k_ref_coll9, dim3(nb), dim3(kBlock), 0, 0, plot_data, *data, w0, w1, w5, tau_inv,
-                       ux_in, rho_out, Fx, Fy, G);
+                       ux_in, rho_out, Fx, Fy, G, zou_he);
     check_launch();
     if (G < 0) {
         hipLaunchKernelGGL(k_ref_sc9, dim3(nb), dim3(kBlock), 0, 0, plot_data, *data, w0, w1, w5, tau_inv, G, Gads);
@@ -303,16 +309,30 @@ extern "C" void collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float
     }
 }
 
+void launch_collision_d3q19(float *plot_data, DataD3Q19 *data, float w0, float w1, float w7, float tau_inv,
+                            float ux_in, float rho_out, lb_dim3 grid, lb_dim3 block, int zou_he) {
+    if (!data || !plot_data) { lb_set_error(LB_ERR_INVALID); return; }
+    if (!shape_ok(grid, block, data->width, data->height, data->depth)) { lb_set_error(LB_ERR_SHAPE); return; }
+    hipLaunchKernelGGL(k_ref_coll19, dim3(nblocks((int64_t)data->width * data->height * data->depth)),
+                       dim3(kBlock), 0, 0, plot_data, *data, w0, w1, w7, tau_inv, ux_in, rho_out, zou_he);
+    check_launch();
+}
+}  // namespace
+
+extern "C" void collision_d2q9(float *plot_data, DataD2Q9 *data, float w0, float w1, float w5,
+                               float tau_inv, float ux_in, float uy_in, float rho_out, float u_min,
+                               float u_max, float rho_min, float rho_max, lb_dim3 grid,
+                               lb_dim3 block, float Fx, float Fy, float G, float Gads) {
+    (void)uy_in; (void)u_min; (void)u_max; (void)rho_min; (void)rho_max;
+    launch_collision_d2q9(plot_data, data, w0, w1, w5, tau_inv, ux_in, rho_out, grid, block, Fx, Fy, G, Gads, 1);
+}
+
 extern "C" void collision_d3q19(float *plot_data, DataD3Q19 *data, float w0, float w1, float w7,
                                 float tau_inv, float ux_in, float uy_in, float uz_in, float rho_out,
                                 float u_min, float u_max, float rho_min, float rho_max,
                                 lb_dim3 grid, lb_dim3 block) {
     (void)uy_in; (void)uz_in; (void)u_min; (void)u_max; (void)rho_min; (void)rho_max;
-    if (!data || !plot_data) { lb_set_error(LB_ERR_INVALID); return; }
-    if (!shape_ok(grid, block, data->width, data->height, data->depth)) { lb_set_error(LB_ERR_SHAPE); return; }
-    hipLaunchKernelGGL(k_ref_coll19, dim3(nblocks((int64_t)data->width * data->height * data->depth)),
-                       dim3(kBlock), 0, 0, plot_data, *data, w0, w1, w7, tau_inv, ux_in, rho_out);
-    check_launch();
+    launch_collision_d3q19(plot_data, data, w0, w1, w7, tau_inv, ux_in, rho_out, grid, block, 1);
 }
 
 extern "C" void get_border_d2q9(DataD2Q9 *data, lb_dim3 grid, lb_dim3 block) {
@@ -668,7 +688,6 @@ extern "C" int lb_refsim_create(const lb_ref_params *p, const uint8_t *solids, l
     if (lb_device_count() <= 0) return LB_ERR_NO_DEVICE;
     if (p->deviceCount < 1 || p->width < 2 || p->height < 1 || p->depth < 1) return LB_ERR_INVALID;
     if (p->width % p->deviceCount) return LB_ERR_SHAPE;
-    if (!p->zou_he) return LB_ERR_UNSUPPORTED;  // the reference collision always applies it
     lb_refsim *s = new (std::nothrow) lb_refsim();
     if (!s) return LB_ERR_INVALID;
     s->prm = *p;
@@ -780,8 +799,8 @@ extern "C" int lb_refsim_run(lb_refsim *s, int steps) {
                 for (int d = 0; d < nd; d++) apply_border_d2q9(&s->d2[d], grid, block);
             }
             for (int d = 0; d < nd; d++)
-                collision_d2q9(s->plot[d], &s->d2[d], kW9[0], kW9[1], kW9[2], s->tau_inv, p.ux, p.uy, p.rho,
-                               0.f, 0.f, 0.f, 0.f, grid, block, p.Fx, p.Fy, p.G, p.Gads);
+                launch_collision_d2q9(s->plot[d], &s->d2[d], kW9[0], kW9[1], kW9[2], s->tau_inv, p.ux, p.rho, grid,
+                                      block, p.Fx, p.Fy, p.G, p.Gads, p.zou_he);
         } else {
             for (int d = 0; d < nd; d++) streaming_d3q19(&s->d3[d], grid, block);
             if (nd > 1) {
@@ -799,8 +818,8 @@ extern "C" int lb_refsim_run(lb_refsim *s, int steps) {
                 for (int d = 0; d < nd; d++) apply_border_d3q19(&s->d3[d], grid, block);
             }
             for (int d = 0; d < nd; d++)
-                collision_d3q19(s->plot[d], &s->d3[d], kW19[0], kW19[1], kW19[2], s->tau_inv, p.ux, p.uy, p.uz,
-                                p.rho, 0.f, 0.f, 0.f, 0.f, grid, block);
+                launch_collision_d3q19(s->plot[d], &s->d3[d], kW19[0], kW19[1], kW19[2], s->tau_inv, p.ux, p.rho,
+                                       grid, block, p.zou_he);
         }
         hipLaunchKernelGGL(k_ref_read_value, dim3(1), dim3(1), 0, 0, s->plot[0], s->value_dev, s->W, s->H,
                            p.cursor_x, p.cursor_y, p.cursor_z);

@@ -56,14 +56,15 @@ def load(dat_path):
     return dict(params=params, solids=solids, path=dat_path)
 
 
-def simulation(deck, cursor=(0, 0, 0)):
-    """The reference driver (refsim.Simulation) configured from a deck."""
+def simulation(deck, cursor=(0, 0, 0), **overrides):
+    """The reference driver (refsim.Simulation) configured from a deck;
+    keyword overrides (e.g. zou_he=False, Gads=...) replace deck values."""
     from .refsim import Simulation
     p = deck["params"]
-    return Simulation(deck["solids"], deviceCount=max(1, p["device_count"]), ux=p["ux"], uy=p["uy"], uz=p["uz"],
-                      rho=p["rho"], viscosity=p["viscosity"], Fx=p["fx"], Fy=p["fy"], Fz=p["fz"], G=p["G"],
-                      Gads=p["Gads"],
-                      cursor=cursor)
+    kw = dict(deviceCount=max(1, p["device_count"]), ux=p["ux"], uy=p["uy"], uz=p["uz"], rho=p["rho"],
+              viscosity=p["viscosity"], Fx=p["fx"], Fy=p["fy"], Fz=p["fz"], G=p["G"], Gads=p["Gads"], cursor=cursor)
+    kw.update(overrides)
+    return Simulation(deck["solids"], **kw)
 
 
 def write_vtk(directory, vtk_id, fields, device_count=1):

@@ -15,7 +15,9 @@ from ._lib import LbRefParams, check, lib, DataD2Q9, DataD3Q19
 
 class Simulation:
     def __init__(self, solids, deviceCount=1, ux=0.0, uy=0.0, uz=0.0, rho=1.0, viscosity=1.0 / 6.0,
-                 Fx=0.0, Fy=0.0, Fz=0.0, G=0.0, Gads=0.0, cursor=(0, 0, 0)):
+                 Fx=0.0, Fy=0.0, Fz=0.0, G=0.0, Gads=0.0, cursor=(0, 0, 0), zou_he=True):
+        """zou_he: the reference's always-on Zou-He inlet (x = 0) / outlet
+        (x = W-1) (collision.cu:211-262); False makes x periodic."""
         s = np.ascontiguousarray(np.asarray(solids, dtype=np.uint8))
         if s.ndim == 2:
             s = s[None]
@@ -28,7 +30,7 @@ class Simulation:
         p.width, p.height, p.depth = self.width, self.height, self.depth
         p.ux, p.uy, p.uz, p.rho, p.viscosity = ux, uy, uz, rho, viscosity
         p.Fx, p.Fy, p.Fz, p.G, p.Gads = Fx, Fy, Fz, G, Gads
-        p.zou_he = 1
+        p.zou_he = 1 if zou_he else 0
         p.cursor_x, p.cursor_y, p.cursor_z = cursor
         self.params = p
         self.steps = 0
@@ -68,6 +70,16 @@ class Simulation:
         args = [a.ctypes.data_as(C.c_void_p) for a in (f, ux, uy, uz, rho, plot)]
         check(lib.lb_refsim_download(self._h, dev, *args), "lb_refsim_download")
         return dict(f=f, ux=ux, uy=uy, uz=uz, rho=rho, plot=plot)
+
+    def upload_populations(self, f, dev=0):
+        """Overwrite subdomain dev's populations (Q x N float32, f0 plane
+        first) through the device pointers of its Data struct, e.g. to start
+        from a droplet instead of the reference's uniform density."""
+        f = np.ascontiguousarray(np.asarray(f, np.float32).reshape(self.Q, -1))
+        d = self.data(dev)
+        for i in range(self.Q):
+            ptr = C.cast(getattr(d, "f%d" % i), C.c_void_p)
+            check(lib.lb_memcpy_h2d(ptr, f[i].ctypes.data_as(C.c_void_p), f[i].nbytes), "lb_memcpy_h2d")
 
     def data(self, dev=0):
         if self.is3d:
