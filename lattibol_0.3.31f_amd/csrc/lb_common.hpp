@@ -133,13 +133,19 @@ LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16,
 // bits on any device, launch shape or thread count).  The value rounds up in
 // magnitude with probability (distance to the lower neighbour) / spacing, so
 // the stored value is unbiased and increments below half a spacing survive.
-LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) {
+// The node's word (64-bit arithmetic, once per node) and each slot's bits
+// from it; sr_bits is the two together.
+LB_DEV uint32_t sr_seed(uint32_t step, uint32_t id) {
     uint64_t z = 0x5EEDull + (uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)id;
     z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
-    uint32_t x = (uint32_t)z + (uint32_t)slot * 0x9E3779B9u;
+    return (uint32_t)z;
+}
+LB_DEV uint32_t sr_mix(uint32_t seed, int slot) {
+    uint32_t x = seed + (uint32_t)slot * 0x9E3779B9u;
     x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
     return x;
 }
+LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) { return sr_mix(sr_seed(step, id), slot); }
 // BF16: add 16 random bits below the kept ones and truncate.
 LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
     const uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -148,14 +154,24 @@ LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
 }
 // FP16: the same with 13 dropped bits in the normal range (the truncated
 // value is exact in FP16); below 2^-14 the spacing is 2^-24:
-// floor(|v| 2^24 + u), u in [0, 1) from 23 random bits, exact in double.
+// q = floor(|v| 2^24 + u), u = r23 / 2^23 from 23 random bits.  With
+// |v| = m 2^(e-150) (m the 24-bit significand, e >= 1 the biased exponent)
+// |v| 2^24 = m 2^-k, k = 126 - e >= 14, and q = (m 2^(23-k) + r23) >> 23
+// exactly: for k > 23 the fraction dropped from m 2^(23-k) cannot carry,
+// since m 2^(23-k) + r23 crosses a multiple of 2^23 only through its
+// integer part.  Integer arithmetic only.
 LB_DEV uint16_t f32_to_f16_sr(float f, uint32_t r) {
     const uint32_t x = __builtin_bit_cast(uint32_t, f);
     const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
     if (ax >= 0x477ff000u) return f32_to_f16(f);
     if (ax < 0x38800000u) {
-        const double q = (double)__builtin_bit_cast(float, ax) * 16777216.0 + (double)(r >> 9) * (1.0 / 8388608.0);
-        return (uint16_t)(sign | (uint32_t)q);
+        const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | (e ? 0x800000u : 0u);
+        const int k = 126 - (int)(e ? e : 1u);
+        const uint32_t r23 = r >> 9;
+        uint32_t q;
+        if (k <= 23) q = (uint32_t)((((uint64_t)m << (23 - k)) + r23) >> 23);
+        else q = ((k - 23 >= 32 ? 0u : (m >> (k - 23))) + r23) >> 23;
+        return (uint16_t)(sign | q);
     }
     const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
     return (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
@@ -232,6 +248,9 @@ LB_DEV double exp_cw(double x) {
 
 // psi = rho0 (1 - exp(-rho/rho0)) in double, rounded to T.
 template <class T> LB_DEV T sc_psi(T rho, float rho0) {
+    // rho0 = 1 (a uniform branch) skips the FP64 division and product, which
+    // are exact there: the same bits either way
+    if (rho0 == 1.0f) return (T)(1.0 - exp_cw(-(double)rho));
     double r = (double)rho / (double)rho0;
     return (T)((double)rho0 * (1.0 - exp_cw(-r)));
 }

@@ -32,7 +32,10 @@ template <int Q, class S, bool SC, bool MACROS, int MEM = 0>
 __global__ __launch_bounds__(kThreads) void k_step(const StepArgs<S> *__restrict__ pa, uint32_t dstep) {
     const StepArgs<S> &a = *pa;
     NodePos p = node_of(a, tile_of_block(a));
-    if (a.round_mode) p.step = ldm(&a.ctr[0]) + dstep;
+    if (a.round_mode) {
+        p.step = ldm(&a.ctr[0]) + dstep;
+        p.seed = sr_seed(p.step, p.id);
+    }
     step_body<Q, S, SC, MACROS, MEM>(a, p);
 }
 

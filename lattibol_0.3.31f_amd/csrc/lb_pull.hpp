@@ -77,7 +77,8 @@ LB_DEV int xcd_remap(int b, int nb) {
 // id: node index x + nx (y + ny z) (flags, macros, psi); e: element offset of
 // the node's population 0 in the population buffers; e0: that of the tile's
 // first node (the same for the whole workgroup).
-struct NodePos { int x, y, z; uint32_t id, e, e0; bool valid; uint32_t step; };
+// step / seed: the stochastic-rounding stream's step and node word (sr_seed).
+struct NodePos { int x, y, z; uint32_t id, e, e0; bool valid; uint32_t step, seed; };
 
 // Population layouts.  LB_LAYOUT_SOA: [i][z][y][x], element i*N + id.
 // LB_LAYOUT_ROWS: [z][y][i][x], the Q populations of one x-row stored back to
@@ -116,6 +117,7 @@ template <class S> LB_DEV NodePos node_in_tile(const StepArgs<S> &a, int tx, int
     p.e0 = ((uint32_t)tx << a.bx_shift) + a.sy * (uint32_t)(ty * a.by) + a.sz * (uint32_t)z;
     p.e = p.e0 + (uint32_t)lx + a.sy * (uint32_t)ly;
     p.step = 0;
+    p.seed = 0;
     return p;
 }
 
@@ -318,8 +320,8 @@ template <class S> constexpr bool kRoundable = std::is_same<S, StF16>::value || 
 
 template <class S, bool SR>
 LB_DEV typename S::raw store_val_m(const NodePos &p, typename S::T v, float w, int slot) {
-    if constexpr (SR && std::is_same<S, StF16>::value) return f32_to_f16_sr(v, sr_bits(p.step, p.id, slot));
-    else if constexpr (SR && std::is_same<S, StBF16>::value) return f32_to_bf16_sr(v, sr_bits(p.step, p.id, slot));
+    if constexpr (SR && std::is_same<S, StF16>::value) return f32_to_f16_sr(v, sr_mix(p.seed, slot));
+    else if constexpr (SR && std::is_same<S, StBF16>::value) return f32_to_bf16_sr(v, sr_mix(p.seed, slot));
     else return S::st(v, w);
 }
 template <class S>
@@ -462,21 +464,6 @@ LB_DEV void collide_dev(T *s, T drho, T ux, T uy, T uz, T ti, bool rest_closure)
     });
 }
 
-// Node whose psi a node at (gx, gy, gz) reads: wrap across periodic
-// (and halo) faces, clamp otherwise.
-template <int Q, class S> LB_DEV void psi_neighbour(const StepArgs<S> &a, int &gx, int &gy, int &gz) {
-    if (gx < 0) gx = a.bc[0] == LB_BC_PERIODIC ? gx + a.nx : 0;
-    if (gx >= a.nx) gx = a.bc[1] == LB_BC_PERIODIC ? gx - a.nx : a.nx - 1;
-    if (gx >= a.nx) gx = a.nx - 1;   // tile columns past a ragged edge
-    if (gy < 0) gy = a.bc[2] == LB_BC_PERIODIC ? gy + a.ny : 0;
-    if (gy >= a.ny) gy = a.bc[3] == LB_BC_PERIODIC ? gy - a.ny : a.ny - 1;
-    if (gy >= a.ny) gy = a.ny - 1;
-    if (Q == 19) {
-        if (gz < 0) gz = (a.halo || a.bc[4] == LB_BC_PERIODIC) ? gz + a.nz : 0;
-        if (gz >= a.nz) gz = (a.halo || a.bc[5] == LB_BC_PERIODIC) ? gz - a.nz : a.nz - 1;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // The fused step.
 // Zou-He inlet (x = 0) / outlet (x = nx-1) in the form matching the storage.
@@ -551,14 +538,32 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p) {
         if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
         if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
         if constexpr (SC) {
+            // psi index of neighbour x + c_i: the node's index plus c_i's
+            // offset, plus a face correction on the lanes at a face the link
+            // crosses (uniform per face): across a periodic (or halo) face the
+            // neighbour wraps to the opposite plane, across any other face it
+            // is clamped to the node's own plane.  Six corrections per lane
+            // instead of a wrap test per link.
+            const int sx = 1, sy = a.nx, sz = a.nx * a.ny;
+            const bool zw_lo = Q == 19 && (a.halo || a.bc[4] == LB_BC_PERIODIC);
+            const bool zw_hi = Q == 19 && (a.halo || a.bc[5] == LB_BC_PERIODIC);
+            const int xm = p.x == 0 ? (a.bc[0] == LB_BC_PERIODIC ? a.nx * sx : sx) : 0;
+            const int xp = p.x == a.nx - 1 ? (a.bc[1] == LB_BC_PERIODIC ? -a.nx * sx : -sx) : 0;
+            const int ym = p.y == 0 ? (a.bc[2] == LB_BC_PERIODIC ? a.ny * sy : sy) : 0;
+            const int yp = p.y == a.ny - 1 ? (a.bc[3] == LB_BC_PERIODIC ? -a.ny * sy : -sy) : 0;
+            int zm = 0, zp = 0;
+            if constexpr (Q == 19) {
+                zm = p.z == 0 ? (zw_lo ? a.nz * sz : sz) : 0;
+                zp = p.z == a.nz - 1 ? (zw_hi ? -a.nz * sz : -sz) : 0;
+            }
             T IF[3] = {T(0), T(0), T(0)};
             static_for<1, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-                int gx = p.x + cx, gy = p.y + cy, gz = p.z + cz;
-                psi_neighbour<Q>(a, gx, gy, gz);
-                T wp = weight<Q, T>(L::wc(i)) *
-                       (T)ldm(&a.psi[(uint32_t)gx + (uint32_t)a.nx * ((uint32_t)gy + (uint32_t)a.ny * (uint32_t)gz)]);
+                const int corr = (cx > 0 ? xp : (cx < 0 ? xm : 0)) + (cy > 0 ? yp : (cy < 0 ? ym : 0)) +
+                                 (cz > 0 ? zp : (cz < 0 ? zm : 0));
+                const uint32_t q = p.id + (uint32_t)(cx * sx + cy * sy + cz * sz + corr);
+                T wp = weight<Q, T>(L::wc(i)) * (T)ldm(&a.psi[q]);
                 if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
                 if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
                 if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
