@@ -43,7 +43,7 @@ def parse():
                     choices=["sphere", "cavity", "spinodal", "ensemble", "warmstart", "unet", "reference", "copy"],
                     help="sphere = D3Q19 channel past a sphere (the headline); the others print their own line")
     ap.add_argument("--train-samples", type=int, default=4096)
-    ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--heldout", type=int, default=256)
     ap.add_argument("--max-steady-steps", type=int, default=300000)
     ap.add_argument("--spinodal-storages", default="f64,f32,bf16,f16,bf16_nearest,f16_nearest",
@@ -312,15 +312,22 @@ def bench_warmstart(args):
     rest; evaluation.wall_ratio is the warm start alone."""
     from lattibol_amd import warmstart
     from lattibol_amd.unet import UNet
+    import numpy as np
     t0 = time.perf_counter()
-    data = warmstart.dataset(args.train_samples, seed=4242, max_steps=args.max_steady_steps)
+    data = warmstart.dataset(args.train_samples, seed=4242, max_steps=args.max_steady_steps, log=log)
     gen = time.perf_counter() - t0
     log("training set: %d/%d converged in %.1f s" % (data["converged"].sum(), args.train_samples, gen))
     net = UNet(levels=4, base=32, in_ch=3, out_ch=3, height=128, width=128, max_batch=16, seed=1)
     hist = warmstart.train(net, data, epochs=args.epochs, batch=16, lr=1e-3, log=log)
     ev = warmstart.evaluate(net, n=args.heldout, seed=777, max_steps=args.max_steady_steps, init_mode=0,
-                            extra_modes=(1,))
-    ev.pop("per_lattice")
+                            extra_modes=(1,), curve_eps=(0.0, 0.01, 0.03, 0.1))
+    pl = ev.pop("per_lattice")
+    sp, sw = np.array(pl["steps_plain"], float), np.array(pl["steps_warm"], float)
+    ok = (sp > 0) & (sw > 0)
+    ev["per_channel_step_ratio_median"] = float(np.median(sw[ok] / sp[ok])) if ok.any() else None
+    for c in ev["curve"]:
+        log("curve: eps %.2f init mode %d: step ratio %.3f, L2 median %.2e max %.2e" % (
+            c["eps"], c["init_mode"], c["step_ratio"], c["l2_median"] or 0, c["l2_max"] or 0))
     return common("time-to-steady-state, warm-start / plain LBM (wall-time ratio)",
                   round(ev["with_fallback"]["wall_ratio"], 4),
                   "ratio", args, 0, None, "fp32 LBM, bf16 U-Net",
@@ -328,8 +335,10 @@ def bench_warmstart(args):
                                "4-level 32-channel U-Net trained on the batched solver's steady fields; "
                                "%d held-out geometries" % args.heldout},
                   higher=False, train={"samples": int(data["converged"].sum()), "generation_s": round(gen, 1),
-                                       "epochs": args.epochs, "train_s": round(hist["seconds"], 1),
-                                       "final_loss": hist["loss"][-1]}, evaluation=ev)
+                                       "epochs": args.epochs, "best_epoch": hist["best_epoch"],
+                                       "train_s": round(hist["seconds"], 1), "final_loss": hist["loss"][-1],
+                                       "best_val_loss": min(hist["val_loss"]) if hist["val_loss"] else None,
+                                       "mirror_augmentation": True}, evaluation=ev)
 
 
 def unet_flops(levels=4, base=32, H=128, W=128, B=16, out_ch=3):
