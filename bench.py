@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--layout", type=int, default=-1,
                     help="fix the population layout (0 [i][z][y][x], 1 [z][y][i][x]; -1 = autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=5000,
+                    help="steps of the FP64/FP32/FP16/BF16 error runs of the sphere workload (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=0, help="oracle steps for the CPU baseline (0 = auto)")
     ap.add_argument("--kernel-launches", type=int, default=200)
     ap.add_argument("--workload", default="sphere",
@@ -127,6 +129,39 @@ def cpu_baseline(cfg, solid, init, steps_hint):
             **cpu_info(),
             "sample": "%d steps of the full %dx%dx%d lattice, FP32 oracle (C++ host loop, OpenMP over z-planes, "
                       "same arithmetic as the GPU path), %.1f s" % (steps, cfg["nx"], cfg["ny"], cfg["nz"], dt)}
+
+
+def sphere_errors(steps):
+    """C2's storage error: the sphere channel run `steps` steps in FP64, FP32,
+    FP16 and BF16 (shifted deviations, nearest-even) from the same start;
+    velocity and density of each against the FP64 run (relative L2 over the
+    lattice, maximum absolute difference) and the total mass (the channel is
+    open, so its mass changes physically: compared with the FP64 run's)."""
+    import numpy as np
+    import lattibol_amd as lb
+    from lattibol_amd import scenes
+    ref = None
+    out = {}
+    for st in ("f64", "f32", "f16", "bf16"):
+        cfg, solid, init = scenes.sphere_channel(storage=st)
+        sol = lb.Solver(cfg, solid)
+        sol.init_equilibrium(init["rho"], init["ux"], init["uy"], init["uz"])
+        sol.step(steps)
+        mass = sol.reduce()[0]
+        m = sol.macros()
+        sol.close()
+        u = np.stack([m["ux"], m["uy"], m["uz"]]).astype(np.float64)
+        rho = m["rho"].astype(np.float64)
+        if ref is None:
+            ref = (u, rho, mass)
+        du, dr = u - ref[0], rho - ref[1]
+        out[st] = {"u_l2_rel": float(np.linalg.norm(du) / np.linalg.norm(ref[0])),
+                   "u_max_abs": float(np.abs(du).max()),
+                   "rho_l2_rel": float(np.linalg.norm(dr) / np.linalg.norm(ref[1])),
+                   "rho_max_abs": float(np.abs(dr).max()),
+                   "mass_rel_vs_fp64": float((mass - ref[2]) / ref[2])}
+        log("C2 error %s after %d steps: %s" % (st, steps, json.dumps(out[st])))
+    return {"steps": steps, "u_in": 0.05, "storage": out}
 
 
 def log(msg):
@@ -518,6 +553,7 @@ def main():
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(scenes.sphere_channel(storage="f32")[0], solid, init, args.cpu_steps)
+    errors = sphere_errors(args.error_steps) if args.error_steps > 0 else None
 
     line = {
         "metric": "D3Q19 MLUPS + % HBM peak; time-to-steady-state, warm-start vs plain LBM",
@@ -539,6 +575,7 @@ def main():
                    "pct_hbm_peak_whole_step": round(100 * mlups * 1e6 * bytes_per_update / (PEAK_HBM_GBS * 1e9), 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "storage_error_vs_fp64": errors,
     }
     print(json.dumps(line), flush=True)
 

@@ -19,6 +19,20 @@ def bf16(t, emulate=True):
     return t + (r - t).detach()
 
 
+class _GradBF16(torch.autograd.Function):
+    """Identity forward; the backward rounds the incoming gradient to BF16
+    (nearest-even), as the GPU stores the pre-activation gradient it feeds
+    to the weight- and input-gradient convolutions."""
+
+    @staticmethod
+    def forward(ctx, t):
+        return t.view_as(t)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.to(torch.bfloat16).to(g.dtype)
+
+
 def conv_layout(levels, base, in_ch, out_ch):
     C_ = [base << l for l in range(levels)]
     convs = []
@@ -52,8 +66,12 @@ def unpack(params, levels, base, in_ch, out_ch):
     return out, (wh, bh)
 
 
-def forward(params, x, levels=4, base=32, in_ch=3, out_ch=3, emulate=True):
-    """params: float64 tensor (requires_grad ok); x: (B, H, W, in_ch) -> (B, H, W, out_ch)."""
+def forward(params, x, levels=4, base=32, in_ch=3, out_ch=3, emulate=True, emulate_grad=False):
+    """params: float64 tensor (requires_grad ok); x: (B, H, W, in_ch) -> (B, H, W, out_ch).
+
+    emulate_grad: also round each convolution's pre-activation gradient to
+    BF16 on its way into the weight and input gradients (the bias gradient
+    sums the unrounded values), as the GPU backward does."""
     convs, (wh, bh) = unpack(params, levels, base, in_ch, out_ch)
     B, H, W, _ = x.shape
     xin = torch.zeros(B, 16, H, W, dtype=params.dtype)
@@ -66,7 +84,10 @@ def forward(params, x, levels=4, base=32, in_ch=3, out_ch=3, emulate=True):
         nonlocal k
         w, b = convs[k]
         k += 1
-        y = F.conv2d(t, bf16(w, emulate), padding=1) + b.view(1, -1, 1, 1)
+        y = F.conv2d(t, bf16(w, emulate), padding=1)
+        if emulate_grad:
+            y = _GradBF16.apply(y)
+        y = y + b.view(1, -1, 1, 1)
         return bf16(torch.relu(y), emulate)
 
     for l in range(levels):
