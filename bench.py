@@ -36,6 +36,8 @@ def parse():
                     help="fix the load/store mode (0 plain, 1 non-temporal stores, 2 non-temporal loads+stores)")
     ap.add_argument("--layout", type=int, default=-1,
                     help="fix the population layout (0 [i][z][y][x], 1 [z][y][i][x]; -1 = autotune)")
+    ap.add_argument("--sc-sweep", type=int, default=-1,
+                    help="Shan-Chen step form (0 density pass + sweep, 1 one z-marching sweep; -1 = autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--error-steps", type=int, default=5000,
                     help="steps of the FP64/FP32/FP16/BF16 error runs of the sphere workload (0 = skip)")
@@ -230,7 +232,7 @@ def bench_spinodal(args):
         raise SystemExit("the spinodal workload reports the f32 run; include f32 in --spinodal-storages")
     for st, rm in runs:
         key = st + ("_nearest" if rm == "nearest" else "")
-        cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm, block_x=args.block_x,
+        cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm, block_x=args.block_x, sc_sweep=args.sc_sweep,
                                            xcd_remap=args.xcd_remap, mem_mode=args.mem_mode, layout=args.layout)
         sol = lb.Solver(cfg, solid)
         sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)

@@ -230,6 +230,15 @@ typedef struct {
                                moving wall, no force, Shan-Chen or Zou-He, nx and ny
                                >= 16 + time_block; else 0 (a fixed value that cannot
                                apply is LB_ERR_UNSUPPORTED) */
+    int sc_sweep;           /* Shan-Chen D3Q19 step: -1 autotune, 0 density pass + sweep
+                               (two launches, psi through HBM), 1 or 2 one z-marching
+                               sweep with the psi planes in LDS (one launch), keeping
+                               the pulled populations in registers (1) or pulling them
+                               again from L2 for the sweep (2); all bit-identical.
+                               1 / 2 need Shan-Chen D3Q19, FP32 / FP16 / BF16 storage and
+                               plain accesses (mem_mode -1 or 0), else
+                               LB_ERR_UNSUPPORTED */
+    int sc_zchunk;          /* z-planes per workgroup of the one-pass sweep, 0 = autotune */
 } lb_config;
 
 typedef struct lb_ctx lb_ctx;

@@ -366,8 +366,13 @@ struct lb_ctx {
     // launch geometry and load hints of the Shan-Chen density pass, tuned on
     // their own (its traffic is 19 loads and one store per node)
     int dbx_shift = 6, dby = 4, dxcd = 0, dmem = 0;
+    // Shan-Chen D3Q19 step form: 0 density pass + sweep; 1 / 2 the one-pass
+    // z-marching sweep (k_step_scz; the sweep's launch fields then hold its
+    // tile shape), keeping the pulled populations in registers (1) or pulling
+    // them again for the sweep (2), with zc z-planes per workgroup
+    int scz = 0, zc = 24;
     void *dargs = nullptr;            // argument blocks: sweep from buffer 0 / 1, density pass from 0 / 1
-    static constexpr size_t kArgBytes = 512;
+    static constexpr size_t kArgBytes = 640;
 
     ~lb_ctx() {
         for (auto &g : graph) if (g) (void)hipGraphExecDestroy(g);
@@ -388,6 +393,14 @@ struct lb_ctx {
 namespace {
 
 int ilog2(int v) { int s = 0; while ((1 << s) < v) s++; return s; }
+
+// Tile shapes of the one-pass Shan-Chen sweep: 256, 512 or 1,024 threads, the
+// frame (2 (bx + 2) + 2 by positions) within the workgroup and the psi plane
+// within its LDS slot.
+bool scz_shape_ok(int bx, int by) {
+    return (bx == 32 || bx == 64 || bx == 128) && (bx * by == 256 || bx * by == 512 || bx * by == 1024) &&
+           2 * (bx + 2) + 2 * by <= bx * by && (bx + 2) * (by + 2) <= kZSlot;
+}
 
 // 16-bit stores round stochastically: asked for, or by default on Shan-Chen
 // lattices, whose per-step deviation increments fall below half a 16-bit
@@ -437,6 +450,8 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src, bool dens = f
     a.nty = (c->cfg.ny + a.by - 1) / a.by;
     a.nblocks = a.ntx * a.nty * c->cfg.nz;
     a.xcd = dens ? c->dxcd : c->xcd;
+    a.zc = c->zc;
+    if (!dens && c->scz) a.nblocks = a.ntx * a.nty * ((c->cfg.nz + c->zc - 1) / c->zc);
     if (c->Q == 9) fast_path_args<9>(a, 1 << a.bx_shift, a.by);
     else fast_path_args<19>(a, 1 << a.bx_shift, a.by);
     return a;
@@ -475,6 +490,11 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     StepArgs<S> a = make_args<S>(c, src);   // host copy: launch geometry only
     const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
+    if constexpr (Q == 19) {
+        if (sc && c->scz)
+            return launch_step_scz(c->cfg.storage, macros, c->scz == 1, (1 << c->bx_shift) * c->by,
+                                   (unsigned)a.nblocks, c->stream, pa, dstep);
+    }
     if (sc) {
         launch_density_t<Q, S>(c, src);
         if (macros) hipLaunchKernelGGL((k_step<Q, S, true, true>), dim3(a.nblocks), dim3(kThreads), 0, c->stream, pa, dstep);
@@ -810,6 +830,66 @@ int autotune(lb_ctx *c) {
         if (!rc) rc = (int)hipGetLastError();
         if (!rc) rc = upload_args(c);
     }
+    // Shan-Chen D3Q19: the one-pass z-marching sweep (tile shape x z-chunk x
+    // XCD remap x hints x layout) against the tuned two passes
+    if (!rc && c->cfg.sc_G != 0.0f && c->Q == 19 && c->cfg.sc_sweep != 0 && scz_storage(c->cfg.storage)) {
+        auto time_steps = [&]() -> float {
+            launch_step(c, 0, false);   // warm
+            (void)hipEventRecord(e0, c->stream);
+            for (int r = 0; r < reps; r++) launch_step(c, r & 1, false);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            return ms;
+        };
+        struct ZCand { int bx, by, zc, xcd, form, layout; };
+        const int two_bx = c->bx_shift, two_by = c->by, two_xcd = c->xcd, two_mem = c->mem, two_lay = c->layout;
+        const float two = c->cfg.sc_sweep > 0 ? 1e30f : time_steps();
+        float zbest = 1e30f;
+        ZCand zb{64, 8, 24, 0, 1, c->layout};
+        const int zcs[5] = {8, 16, 24, 32, 48};
+        // tiles of 256 / 512 / 1,024 threads whose frame (2 (bx + 2) + 2 by) fits the workgroup
+        const int shapes[8][2] = {{32, 8}, {64, 4}, {32, 16}, {64, 8}, {128, 4}, {32, 32}, {64, 16}, {128, 8}};
+        for (const auto &sh : shapes) {
+            const int bx = sh[0], by = sh[1];
+            if (c->cfg.block_x > 0 ? bx != c->cfg.block_x : (bx > 2 * c->cfg.nx && bx != 32)) continue;
+            if (c->cfg.block_x > 0 && c->cfg.block_y > 0 && by != c->cfg.block_y) continue;
+            for (int zc : zcs) {
+                if (c->cfg.sc_zchunk > 0 ? zc != zcs[0] : (zc > c->cfg.nz && zc != zcs[0])) continue;
+                const int zcv = c->cfg.sc_zchunk > 0 ? c->cfg.sc_zchunk : zc;
+                for (int xcd = 0; xcd < 2; xcd++) {
+                    if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
+                    for (int form : {1, 2}) {
+                        if (c->cfg.sc_sweep > 0 && form != c->cfg.sc_sweep) continue;
+                        for (int lay = 0; lay < 2; lay++) {
+                            if (c->cfg.layout >= 0 && lay != c->cfg.layout) continue;
+                            c->scz = form; c->zc = zcv;
+                            c->bx_shift = ilog2(bx); c->by = by; c->xcd = xcd; c->mem = 0; c->layout = lay;
+                            rc = upload_args(c);
+                            if (rc) break;
+                            const float ms = time_steps();
+                            if (ms < zbest) { zbest = ms; zb = {bx, by, zcv, xcd, form, lay}; }
+                        }
+                        if (rc) break;
+                    }
+                    if (rc) break;
+                }
+                if (rc) break;
+            }
+            if (rc) break;
+        }
+        if (!rc && zbest < two) {
+            c->scz = zb.form; c->zc = zb.zc;
+            c->bx_shift = ilog2(zb.bx); c->by = zb.by; c->xcd = zb.xcd; c->mem = 0; c->layout = zb.layout;
+        } else {
+            c->scz = 0;
+            c->bx_shift = two_bx; c->by = two_by; c->xcd = two_xcd; c->mem = two_mem; c->layout = two_lay;
+        }
+        if (!rc) rc = (int)hipGetLastError();
+        if (!rc) rc = upload_args(c);
+        if (!rc && c->cfg.sc_sweep > 0 && !c->scz) rc = LB_ERR_UNSUPPORTED;   // no tile shape fits
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
@@ -874,6 +954,12 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     // carry: a ring of several slabs would need a psi-plane exchange too
     if (cfg->halo_z && cfg->lattice == 19 && cfg->sc_G != 0.0f) return LB_ERR_UNSUPPORTED;
     if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
+    if (cfg->sc_sweep < -1 || cfg->sc_sweep > 2 || cfg->sc_zchunk < 0) return LB_ERR_INVALID;
+    if (cfg->sc_sweep > 0 && (cfg->lattice != 19 || cfg->sc_G == 0.0f || !scz_storage(cfg->storage) ||
+                               cfg->mem_mode > 0 ||
+                               (cfg->block_x > 0 && !scz_shape_ok(cfg->block_x, cfg->block_y > 0 ? cfg->block_y
+                                                                                       : kZThreads / cfg->block_x))))
+        return LB_ERR_UNSUPPORTED;
     if (cfg->mem_mode > 2 || cfg->layout > LB_LAYOUT_ROWS || cfg->round_mode < 0 || cfg->round_mode > 2 ||
         cfg->node_words > 1 ||
         (cfg->time_block != -1 && cfg->time_block != 0 && cfg->time_block != 4 && cfg->time_block != 8))
@@ -915,9 +1001,14 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc && c->psi) rc = (int)hipMemsetAsync(c->psi, 0, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4), c->stream);
     if (!rc) rc = upload_flags(c);
     if (!rc) {
-        if (cfg->block_x > 0 && cfg->xcd_remap >= 0 && cfg->mem_mode >= 0 && cfg->layout >= 0 && cfg->node_words >= 0) {
+        const bool scq = cfg->sc_G != 0.0f && c->Q == 19;
+        if (cfg->block_x > 0 && cfg->xcd_remap >= 0 && cfg->mem_mode >= 0 && cfg->layout >= 0 && cfg->node_words >= 0 &&
+            (!scq || cfg->sc_sweep == 0 || (cfg->sc_sweep > 0 && cfg->sc_zchunk > 0))) {
+            c->scz = scq ? std::max(0, cfg->sc_sweep) : 0;
+            if (c->scz) c->zc = cfg->sc_zchunk;
             c->bx_shift = ilog2(cfg->block_x);
-            c->by = std::max(1, kThreads / cfg->block_x);
+            c->by = c->scz ? (cfg->block_y > 0 ? cfg->block_y : kZThreads / cfg->block_x)
+                           : std::max(1, kThreads / cfg->block_x);
             c->xcd = cfg->xcd_remap;
             c->mem = cfg->mem_mode;
             c->layout = cfg->layout;
@@ -943,6 +1034,8 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     c->cfg.layout = c->layout;
     c->cfg.node_words = c->packed;
     c->cfg.graph_chunk = c->graph_steps;
+    c->cfg.sc_sweep = c->scz;
+    c->cfg.sc_zchunk = c->scz ? c->zc : 0;
     if (c->cfg.storage == LB_STORE_F16 || c->cfg.storage == LB_STORE_BF16)
         c->cfg.round_mode = stochastic_rounding(c->cfg) ? 1 : 2;
     *out = c;

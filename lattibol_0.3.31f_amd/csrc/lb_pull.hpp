@@ -64,6 +64,7 @@ template <class S> struct StepArgs {
     uint32_t *ctr;                // step counter of the stochastic-rounding stream
     float zh_ux, zh_rho;
     int bx_shift, by, ntx, nty, nblocks, xcd;
+    int zc;                       // one-pass Shan-Chen sweep: z-planes per workgroup
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; this bijection
@@ -220,7 +221,9 @@ template <class S> LB_DEV void node_word(const StepArgs<S> &a, const NodePos &p,
     }
 }
 
-template <int Q, class S, int MEM = 0>
+// GENERIC: always the per-link path (a node outside the tile of its lanes,
+// whose element offset need not lie in the tile's descriptor range).
+template <int Q, class S, int MEM = 0, bool GENERIC = false>
 LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint32_t bm, typename S::T *f) {
     typedef typename S::T T;
     typedef typename S::raw R;
@@ -242,7 +245,7 @@ LB_DEV void pull_node(const StepArgs<S> &a, const NodePos &p, uint32_t bm, typen
     // instruction issue, and the per-link address arithmetic below costs
     // them about 10 %.  The 4- and 8-byte sweeps are bound by HBM and run
     // 3-5 % faster with the per-link form (measured), so they keep it.
-    if constexpr (sizeof(R) == 2) {
+    if constexpr (sizeof(R) == 2 && !GENERIC) {
         const bool zedge = Q == 19 && ((p.z == 0 && (a.halo || a.corr_lo[2])) ||
                                        (p.z == a.nz - 1 && (a.halo || a.corr_hi[2])));
         if (a.fast && !zedge && __builtin_amdgcn_ballot_w64(bm != 0u) == 0) {
@@ -582,7 +585,7 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p) {
 
 // Shan-Chen psi(rho) of one node's post-streaming density, with the same
 // pull / boundary handling (and summation order) as k_step.
-template <int Q, class S, int MEM = 0>
+template <int Q, class S, int MEM = 0, bool GENERIC = false>
 LB_DEV typename S::T psi_of_node(const StepArgs<S> &a, const NodePos &p) {
     typedef typename S::T T;
     uint8_t fl;
@@ -594,7 +597,7 @@ LB_DEV typename S::T psi_of_node(const StepArgs<S> &a, const NodePos &p) {
         rho = (T)a.bcrho[fl >> 4];
     } else {
         T f[Q];
-        pull_node<Q, S, MEM>(a, p, bm, f);
+        pull_node<Q, S, MEM, GENERIC>(a, p, bm, f);
         apply_zou_he<Q, S>(a, p, f);
         T ux, uy, uz;
         if constexpr (S::DEV) {
@@ -614,6 +617,145 @@ LB_DEV void density_body(const StepArgs<S> &a, const NodePos &p) {
     if (!p.valid) return;
     stm(&a.psi[p.id], psi_of_node<Q, S, MEM>(a, p));
 }
+
+// ---------------------------------------------------------------------------
+// One-pass Shan-Chen step (D3Q19): a workgroup owns a bx x by column of
+// nodes over zc consecutive z-planes and marches up through them.  At plane
+// z it pulls the populations of plane z + 1 into registers (the next
+// iteration sweeps them), turns their density into psi and writes it to a
+// ring of four psi planes in LDS, together with the psi of the one-node frame
+// around the tile; after one workgroup barrier it sweeps plane z with the
+// psi of planes z - 1, z, z + 1 read from LDS.  Population loads per node:
+// one pull of its own plus the frame's share (2 (bx + 2) + 2 by nodes per
+// bx by), against two pulls (density pass + sweep) and 19 psi loads in the
+// two-pass form.  A frame or out-of-lattice position holds the psi of its
+// source node (periodic faces wrap, other faces clamp: the two-pass sweep's
+// neighbour index rule), and every psi and every swept node runs the same
+// functions as the density pass and the sweep, so the result is bit-identical
+// to them.  Four planes in the ring let one barrier per plane suffice: the
+// plane written at iteration z + 1 was last read at iteration z - 1, which
+// every wave has finished once it passed the barrier of iteration z.
+constexpr int kZThreads = 512;  // default workgroup (tiles of 256, 512 or 1,024 threads)
+constexpr int kZSlot = 1300;     // largest (bx + 2)(by + 2) of the tile shapes (128 x 8)
+
+// Source coordinate of the psi at position v along an axis of n nodes.
+LB_DEV int psi_src(int v, int n, int bc_lo, int bc_hi) {
+    if (v < 0) v = bc_lo == LB_BC_PERIODIC ? v + n : 0;
+    else if (v >= n) v = bc_hi == LB_BC_PERIODIC ? v - n : n - 1;
+    return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+
+// psi of the source node of lattice position (x, y, z), any position.
+template <int Q, class S, int MEM>
+LB_DEV typename S::T psi_at(const StepArgs<S> &a, int x, int y, int z) {
+    NodePos p;
+    p.x = psi_src(x, a.nx, a.bc[0], a.bc[1]);
+    p.y = psi_src(y, a.ny, a.bc[2], a.bc[3]);
+    p.z = Q == 19 ? psi_src(z, a.nz, a.bc[4], a.bc[5]) : 0;
+    p.id = (uint32_t)p.x + (uint32_t)a.nx * ((uint32_t)p.y + (uint32_t)a.ny * (uint32_t)p.z);
+    p.e = (uint32_t)p.x + a.sy * (uint32_t)p.y + a.sz * (uint32_t)p.z;
+    p.e0 = p.e;
+    p.valid = true;
+    p.step = 0;
+    p.seed = 0;
+    return psi_of_node<Q, S, MEM, true>(a, p);
+}
+
+// Pulled populations of a node and their moments (m0: rho, or the density
+// deviation of the deviation storages), kept from the density phase for the
+// sweep of the next iteration.
+template <int Q, class S> struct ZState {
+    typename S::T f[Q];
+    typename S::T m0, ux, uy, uz;
+    uint8_t fl;
+    uint32_t bm;
+};
+
+// Density phase of a node of the lattice: node word, pull and moments as in
+// step_body, and the node's psi as psi_of_node returns it.
+template <int Q, class S, int MEM>
+LB_DEV typename S::T zm_load(const StepArgs<S> &a, const NodePos &p, ZState<Q, S> &st) {
+    typedef typename S::T T;
+    node_word(a, p, st.fl, st.bm);
+    if (st.fl & F_SOLID) {
+        if (a.bb_full) pull_node<Q, S, MEM>(a, p, st.bm, st.f);
+        return (T)a.scpsi_solid;
+    }
+    if (st.fl & F_INLET) return sc_psi<T>((T)a.bcrho[st.fl >> 4], a.scrho0);
+    pull_node<Q, S, MEM>(a, p, st.bm, st.f);
+    apply_zou_he<Q, S>(a, p, st.f);
+    T rho;
+    if constexpr (S::DEV) {
+        moments_dev<Q, T>(st.f, st.m0, st.ux, st.uy, st.uz);
+        rho = T(1) + st.m0;
+    } else {
+        moments<Q, T>(st.f, st.m0, st.ux, st.uy, st.uz);
+        rho = st.m0;
+    }
+    return sc_psi<T>(rho, a.scrho0);
+}
+
+// Sweep phase: step_body's arithmetic from the kept state, the psi of the
+// node and its neighbours from the LDS planes pz[cz + 1] (node at c, rows W
+// apart).
+template <int Q, class S, bool MACROS, int MEM>
+LB_DEV void zm_sweep(const StepArgs<S> &a, const NodePos &p, ZState<Q, S> &st, const typename S::T *pzm,
+                     const typename S::T *pz0, const typename S::T *pzp, int c, int W) {
+    typedef typename S::T T;
+    typedef Vel<Q> L;
+    if (st.fl & F_SOLID) {
+        if (a.bb_full) store_node<Q, S, MEM, true>(a, p, st.f);
+        if (MACROS) store_macros(a, p.id, 0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    T rho, ux, uy, uz;
+    if (st.fl & F_INLET) {
+        const int face = st.fl >> 4;
+        rho = (T)a.bcrho[face]; ux = (T)a.bcu[face][0]; uy = (T)a.bcu[face][1]; uz = (T)a.bcu[face][2];
+        if constexpr (S::DEV) feq_dev<Q, T>(st.f, rho - T(1), ux, uy, uz);
+        else feq<Q, T>(st.f, rho, ux, uy, uz);
+    } else {
+        T drho = T(0);
+        if constexpr (S::DEV) {
+            drho = st.m0;
+            rho = T(1) + drho;
+        } else {
+            rho = st.m0;
+        }
+        ux = st.ux; uy = st.uy; uz = st.uz;
+        const T ti = (T)a.ti;
+        T ex = ux, ey = uy, ez = uz;
+        if (a.force[0] != 0.0f) ex = ux + (T)a.force[0] / (rho * ti);
+        if (a.force[1] != 0.0f) ey = uy + (T)a.force[1] / (rho * ti);
+        if (a.force[2] != 0.0f) ez = uz + (T)a.force[2] / (rho * ti);
+        T IF[3] = {T(0), T(0), T(0)};
+        static_for<1, Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+            const T *pl = cz < 0 ? pzm : (cz > 0 ? pzp : pz0);
+            T wp = weight<Q, T>(L::wc(i)) * pl[c + cx + cy * W];
+            if constexpr (cx > 0) IF[0] = IF[0] + wp; else if constexpr (cx < 0) IF[0] = IF[0] - wp;
+            if constexpr (L::pcy(i) > 0) IF[1] = IF[1] + wp; else if constexpr (L::pcy(i) < 0) IF[1] = IF[1] - wp;
+            if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
+        });
+        T gp = (T)(-a.scG) * pz0[c];
+        ex = ex + (gp * IF[0]) / (ti * rho);
+        ey = ey + (gp * IF[1]) / (ti * rho);
+        if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+        if constexpr (S::DEV) collide_dev<Q, T>(st.f, drho, ex, ey, ez, ti, a.rest_closure != 0);
+        else collide<Q, T>(st.f, rho, ex, ey, ez, ti, a.rest_closure != 0);
+    }
+    if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
+    store_node<Q, S, MEM>(a, p, st.f);
+}
+
+// Storages with the one-pass sweep, and its launch (lb_scz.hip): keep = the
+// pulled populations stay in registers across the barrier (else pulled again
+// for the sweep); threads = bx x by, 256, 512 or 1,024; args = the device
+// argument block.
+bool scz_storage(int storage);
+int launch_step_scz(int storage, bool macros, bool keep, int threads, unsigned nblocks, hipStream_t stream,
+                    const void *args, uint32_t dstep);
 
 // ---------------------------------------------------------------------------
 // Node flags on the host: solid, inlet plane (+ face in the high nibble), and

@@ -66,7 +66,7 @@ class LbConfig(C.Structure):
                 ("zh_rho_out", C.c_float), ("halo_z", C.c_int), ("block_x", C.c_int),
                 ("block_y", C.c_int), ("xcd_remap", C.c_int), ("graph_chunk", C.c_int),
                 ("rest_closure", C.c_int), ("mem_mode", C.c_int), ("layout", C.c_int), ("round_mode", C.c_int),
-                ("node_words", C.c_int), ("time_block", C.c_int)]
+                ("node_words", C.c_int), ("time_block", C.c_int), ("sc_sweep", C.c_int), ("sc_zchunk", C.c_int)]
 
 
 class LbBatchLattice(C.Structure):

@@ -27,8 +27,9 @@ W19 = np.array([12 / 36] + [2 / 36] * 6 + [1 / 36] * 12, dtype=np.float64)
 
 def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, bc_u=None, bc_rho=None,
                 force=None, sc_G=0.0, sc_rho0=1.0, sc_psi_solid=0.0, bb_full=False, zou_he=False,
-                zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, xcd_remap=-1, graph_chunk=0,
-                rest_closure=None, mem_mode=-1, layout=-1, round_mode="default", node_words=-1, time_block=-1):
+                zh_ux_in=0.0, zh_rho_out=1.0, halo_z=False, block_x=0, block_y=0, xcd_remap=-1, graph_chunk=0,
+                rest_closure=None, mem_mode=-1, layout=-1, round_mode="default", node_words=-1, time_block=-1,
+                sc_sweep=-1, sc_zchunk=0):
     """Plain-dict description of a lattice; the oracle binding reads the same dict.
 
     rest_closure (default: on unless bb_full) sets feq_0 = rho - sum_{i>0} feq_i so
@@ -56,20 +57,20 @@ def make_config(lattice, shape, tau=None, tau_inv=None, storage="f32", bc=None, 
                 force=[float(a) for a in (list(force or []) + [0, 0, 0])[:3]],
                 sc_G=float(sc_G), sc_rho0=float(sc_rho0), sc_psi_solid=float(sc_psi_solid),
                 bb_full=int(bool(bb_full)), zou_he=int(bool(zou_he)), zh_ux_in=float(zh_ux_in),
-                zh_rho_out=float(zh_rho_out), halo_z=int(bool(halo_z)), block_x=int(block_x),
+                zh_rho_out=float(zh_rho_out), halo_z=int(bool(halo_z)), block_x=int(block_x), block_y=int(block_y),
                 xcd_remap=int(xcd_remap), graph_chunk=int(graph_chunk), rest_closure=int(bool(rest_closure)),
                 mem_mode=int(mem_mode), layout=LAYOUT.get(layout, layout),
                 round_mode=ROUND.get(round_mode, round_mode), node_words=int(node_words),
-                time_block=int(time_block))
+                time_block=int(time_block), sc_sweep=int(sc_sweep), sc_zchunk=int(sc_zchunk))
 
 
 def to_lbconfig(d):
     c = LbConfig()
-    defaults = dict(block_x=0, xcd_remap=-1, graph_chunk=0, mem_mode=-1, layout=-1, round_mode=0, node_words=-1,
-                    time_block=-1)
+    defaults = dict(block_x=0, block_y=0, xcd_remap=-1, graph_chunk=0, mem_mode=-1, layout=-1, round_mode=0, node_words=-1,
+                    time_block=-1, sc_sweep=-1, sc_zchunk=0)
     for k in ("lattice", "nx", "ny", "nz", "storage", "tau_inv", "sc_G", "sc_rho0", "sc_psi_solid", "bb_full",
-              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "xcd_remap", "graph_chunk", "rest_closure",
-              "mem_mode", "layout", "round_mode", "node_words", "time_block"):
+              "zou_he", "zh_ux_in", "zh_rho_out", "halo_z", "block_x", "block_y", "xcd_remap", "graph_chunk", "rest_closure",
+              "mem_mode", "layout", "round_mode", "node_words", "time_block", "sc_sweep", "sc_zchunk"):
         setattr(c, k, d[k] if k in d else defaults[k])
     for f in range(6):
         c.bc[f] = d["bc"][f]
@@ -126,7 +127,7 @@ class Solver:
         check(lib.lb_get_config(self._h, C.byref(c)), "lb_get_config")
         return dict(block_x=c.block_x, block_y=c.block_y, xcd_remap=c.xcd_remap, mem_mode=c.mem_mode, layout=c.layout,
                     node_words=c.node_words, time_block=c.time_block,
-                    graph_chunk=c.graph_chunk,
+                    graph_chunk=c.graph_chunk, sc_sweep=c.sc_sweep, sc_zchunk=c.sc_zchunk,
                     round_mode={1: "stochastic", 2: "nearest"}.get(c.round_mode, "default"))
 
     def set_solid(self, solid):

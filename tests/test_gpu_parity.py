@@ -440,6 +440,58 @@ def test_full_size_sphere_channel_parity(gpu):
     sol.close()
 
 
+ONE_PASS_CASES = {
+    "spinodal24_f32": ("spinodal", dict(n=24), 300),
+    "spinodal24_bf16_sr": ("spinodal", dict(n=24, storage="bf16", round_mode=1), 300),
+    "spinodal24_f16_nearest": ("spinodal", dict(n=24, storage="f16", round_mode="nearest"), 300),
+    "sc19_walls_box_f32": ("sc", dict(name="sc19"), 300),
+    "sc19_walls_box_bf16_sr": ("sc", dict(name="sc19", storage="bf16", round_mode=1), 200),
+    "sc19_walls_box_f16": ("sc", dict(name="sc19", storage="f16"), 200),
+}
+
+
+@pytest.mark.parametrize("layout", ["soa", "rows"])
+@pytest.mark.parametrize("bx,by,zc,form", [(32, 16, 5, 1), (64, 8, 8, 2), (128, 4, 16, 1), (32, 8, 7, 2),
+                                           (64, 4, 3, 1), (64, 16, 6, 1), (128, 8, 4, 2), (32, 32, 9, 1)])
+@pytest.mark.parametrize("case", sorted(ONE_PASS_CASES))
+def test_one_pass_shan_chen_sweep_matches_oracle(gpu, case, bx, by, zc, form, layout):
+    """The z-marching sweep (psi planes in LDS) against the oracle's density
+    pass + sweep: tiles of 32 x 16, 64 x 8, 128 x 4 (512 threads), 32 x 8,
+    64 x 4 (256 threads), 64 x 16, 128 x 8 and 32 x 32 nodes (1,024), ragged at the +x / +y faces or wider than the
+    lattice, z-chunks that do not divide nz, periodic faces and walls /
+    outflow / moving wall / solids, both layouts, the populations kept in
+    registers (form 1) or pulled again for the sweep (form 2)."""
+    kind, kw, steps = ONE_PASS_CASES[case]
+    cfg, solid, init = _scene(kind, kw)
+    cfg.update(layout={"soa": 0, "rows": 1}[layout], sc_sweep=form, sc_zchunk=zc, block_x=bx, block_y=by)
+    print("case", case, "one-pass", bx, by, zc, form, layout, flush=True)
+    sol, orc = _run_pair(gpu, cfg, solid, init, steps)
+    lc = sol.launch_config
+    assert lc["sc_sweep"] == form and lc["sc_zchunk"] == zc and lc["block_x"] == bx and lc["block_y"] == by, lc
+    fluid = solid.reshape(orc.shape) == 0
+    _assert_pop_parity(sol.populations(), orc.A, cfg["storage"], fluid)
+    m, mo = sol.macros(), orc.macros()
+    for k in ("rho", "ux", "uy", "uz"):
+        np.testing.assert_array_equal(m[k][fluid], mo[k][fluid].astype(np.float32))
+    sol.close()
+
+
+def test_one_pass_sweep_rejected_where_it_does_not_apply(gpu):
+    from lattibol_amd import _lib, scenes
+    for kw, over in ((dict(n=16), dict(sc_sweep=1, storage=3)),          # FP64 storage
+                     (dict(n=16), dict(sc_sweep=1, mem_mode=2)),         # non-temporal accesses
+                     (dict(n=16), dict(sc_sweep=1, block_x=256)),        # no tile of 256 x 2
+                     (dict(n=16), dict(sc_sweep=1, block_x=32, block_y=64))):   # 2,048 threads
+        cfg, solid, init = scenes.spinodal(**kw)
+        cfg.update(over)
+        with pytest.raises(RuntimeError, match="unsupported|UNSUPPORTED|-3"):
+            gpu.Solver(cfg, solid)
+    cfg, solid, init = scenes.cavity(n=32)
+    cfg.update(sc_sweep=1)
+    with pytest.raises(RuntimeError):
+        gpu.Solver(cfg, solid)
+
+
 def test_full_size_spinodal_parity_and_mass(gpu):
     from lattibol_amd import scenes
     cfg, solid, init = scenes.spinodal()
