@@ -60,7 +60,7 @@ def parse():
 def spinodal_traffic():
     """HBM bytes per Shan-Chen step (density pass + sweep) measured with
     rocprofv3 --pmc on the 192^3 FP32 workload (committed summary)."""
-    p = os.path.join(ROOT, "profiles", "d3q19_spinodal_192_f32_traffic.json")
+    p = os.path.join(ROOT, "profiles", "d3q19_spinodal_192_traffic.json")
     try:
         with open(p) as fh:
             return json.load(fh).get("hbm_bytes_per_step")
@@ -270,7 +270,7 @@ def bench_spinodal(args):
     # psi and masks excluded.  The two-pass Shan-Chen step moves more: the
     # density pass reads the 19 populations again and stores psi, and the
     # sweep reads psi: 152 + 76 + 4 + 4 = 236 B; the traffic counters measure
-    # hbm_bytes_per_step (profiles/d3q19_spinodal_192_f32_traffic.json).
+    # hbm_bytes_per_step (profiles/d3q19_spinodal_192_traffic.json).
     n = 192 ** 3
     ms = out["f32"]["seconds"] / steps * 1e3
     achieved = 152.0 * n / (ms * 1e-3) / 1e9
