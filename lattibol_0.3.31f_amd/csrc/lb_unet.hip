@@ -204,11 +204,19 @@ template <> struct Piece8<float> {
     }
 };
 
-template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU, int SW = 32, bool SPLIT = false>
+// DPRE (input gradient of a conv whose input is the previous conv's ReLU
+// output act, TOut = bf16, no K split): the epilogue applies that conv's ReLU
+// backward, dpre = v * (act > 0), stores dpre as BF16 (nearest even) and
+// writes the bias-gradient partials of the workgroup's pixels,
+// bpart[blockIdx.x][co] = sum of the unrounded dpre (lane order, then the
+// lane pair holding the same channel, then the four wavefronts in order).
+template <class TIn, class TOut, int PM, int NB, bool BIAS_RELU, int SW = 32, bool SPLIT = false, bool DPRE = false>
 __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in, const TIn *__restrict__ low, int cl,
                                                       const bf16 *__restrict__ W, const float *__restrict__ bias,
                                                       TOut *__restrict__ out, float *__restrict__ part, int npix, int H,
-                                                      int Wd, int Ci, int Co) {
+                                                      int Wd, int Ci, int Co, const bf16 *__restrict__ act = nullptr,
+                                                      float *__restrict__ bpart = nullptr) {
+    static_assert(!DPRE || (!SPLIT && !BIAS_RELU), "ReLU-backward epilogue: no K split, no bias");
     constexpr int RPT = 32 / SW;                 // image rows per MFMA pixel tile
     constexpr int ROWS = 4 * PM * RPT, IR = ROWS + 2, IC = SW + 2;
     constexpr int NCO = 32 * NB;
@@ -316,22 +324,55 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
             }
         }
     }
+    if constexpr (DPRE) {
+        __shared__ float s_b[4][32 * NB];
+        float bs[NB];
 #pragma unroll
-    for (int j = 0; j < PM; j++) {
-        const int ty0 = y0 + (wave * PM + j) * RPT;
+        for (int b = 0; b < NB; b++) bs[b] = 0.f;
+#pragma unroll
+        for (int j = 0; j < PM; j++) {
+            const int ty0 = y0 + (wave * PM + j) * RPT;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int co = n0 + 32 * b + row;
+                for (int r = 0; r < 16; r++) {
+                    const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const size_t pix = ((size_t)img * H + ty0 + i / SW) * Wd + x0 + i % SW;
+                    const float av = lb::bf16_to_f32(__builtin_bit_cast(uint16_t, act[pix * Co + co]));
+                    const float v = av > 0.f ? acc[j][b][r] : 0.f;
+                    out[pix * Co + co] = from_f<TOut>(v);
+                    bs[b] += v;
+                }
+            }
+        }
 #pragma unroll
         for (int b = 0; b < NB; b++) {
-            const int co = n0 + 32 * b + row;
-            const float bv = (BIAS_RELU && !SPLIT) ? bias[co] : 0.f;
-            for (int r = 0; r < 16; r++) {
-                const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                const size_t pix = ((size_t)img * H + ty0 + i / SW) * Wd + x0 + i % SW;
-                float v = acc[j][b][r];
-                if (SPLIT) {
-                    part[((size_t)blockIdx.z * npix + pix) * Co + co] = v;
-                } else {
-                    if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
-                    out[pix * Co + co] = from_f<TOut>(v);
+            bs[b] += __shfl_xor(bs[b], 32);
+            if (lane < 32) s_b[wave][32 * b + lane] = bs[b];
+        }
+        __syncthreads();
+        if (threadIdx.x < 32 * NB) {
+            const int t = threadIdx.x;
+            bpart[(size_t)blockIdx.x * Co + n0 + t] = ((s_b[0][t] + s_b[1][t]) + s_b[2][t]) + s_b[3][t];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < PM; j++) {
+            const int ty0 = y0 + (wave * PM + j) * RPT;
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int co = n0 + 32 * b + row;
+                const float bv = (BIAS_RELU && !SPLIT) ? bias[co] : 0.f;
+                for (int r = 0; r < 16; r++) {
+                    const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                    const size_t pix = ((size_t)img * H + ty0 + i / SW) * Wd + x0 + i % SW;
+                    float v = acc[j][b][r];
+                    if (SPLIT) {
+                        part[((size_t)blockIdx.z * npix + pix) * Co + co] = v;
+                    } else {
+                        if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
+                        out[pix * Co + co] = from_f<TOut>(v);
+                    }
                 }
             }
         }
@@ -701,10 +742,12 @@ __global__ __launch_bounds__(256) void k_sum_chunks16(SumSeg a, SumSeg b, unsign
     }
 }
 
-// Bias-gradient chunking of k_relu_bwd_bias: at most 512 chunks of >= 64 pixels.
+// Bias-gradient chunking of k_relu_bwd_bias: at most 1,024 chunks of >= 64
+// pixels (more chunks keep more loads in flight here but cost more in the
+// chunk sum, whose threads add up to nchunk / 16 partials each).
 struct BiasChunks { int chunk, nchunk; };
 inline BiasChunks bias_chunks(int npix) {
-    const int chunk = npix > 64 * 512 ? (npix + 511) / 512 : 64;
+    const int chunk = npix > 64 * 1024 ? (npix + 1023) / 1024 : 64;
     return {chunk, (npix + chunk - 1) / chunk};
 }
 
@@ -1070,6 +1113,7 @@ struct lb_unet {
     bf16 *xin = nullptr;
     float *y = nullptr, *dy = nullptr;
     float *gA = nullptr, *gB = nullptr, *gC = nullptr;   // gradient scratch (largest tensor)
+    size_t big = 0;                                      // their size (elements)
     bf16 *tA = nullptr, *tB = nullptr;                   // channel-major copies for the weight gradient
     std::vector<float *> gskip;    // gradient of each encoder output (skip)
     float *partial = nullptr;      // chunk partials of the weight and head gradients, K-split conv shares
@@ -1192,6 +1236,49 @@ bool conv_tile_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bia
                                H, Wd, Ci, Co);
     }
     return true;
+}
+
+// Input gradient of a conv with the previous conv's ReLU backward and bias
+// partials fused into the epilogue (k_conv3x3_tile DPRE): dout = this conv's
+// pre-activation gradient (Ci = its output channels), Wt its flipped,
+// transposed weights, act = its input (the previous conv's output, Co
+// channels).  Writes dpre (BF16) and bpart[chunk][Co]; returns the chunk
+// count, 0 (nothing launched) when the shape takes no unsplit LDS-tile plan.
+int dgrad_dpre_chunks(int npix, int H, int Wd, int Ci, int Co) {
+    const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
+    if (!pl.sw || pl.ks > 1) return 0;
+    const int B = npix / (H * Wd);
+    return B * (H / (4 * pl.pm * (32 / pl.sw))) * (Wd / pl.sw);
+}
+int dgrad_dpre_launch(lb_unet *u, const bf16 *dout, const bf16 *Wt, const bf16 *act, bf16 *dpre, float *bpart, int npix,
+                      int H, int Wd, int Ci, int Co) {
+    const int nchunk = dgrad_dpre_chunks(npix, H, Wd, Ci, Co);
+    if (!nchunk) return 0;
+    const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
+    const dim3 grd((unsigned)nchunk, (unsigned)(Co / (32 * pl.nb)), 1u), blk(256);
+    const bf16 *nol = nullptr;
+    if (pl.sw == 32) {
+        if (pl.nb == 2 && pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 2, 2, false, 32, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+        else if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 1, 2, false, 32, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+        else if (pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 2, 1, false, 32, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 1, 1, false, 32, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+    } else {
+        if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 1, 2, false, 16, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, bf16, 1, 1, false, 16, false, true>), grd, blk, 0, u->stream, dout, nol, 0,
+                               Wt, nullptr, dpre, nullptr, npix, H, Wd, Ci, Co, act, bpart);
+    }
+    return nchunk;
 }
 
 template <class TIn, class TOut, bool BR>
@@ -1394,6 +1481,7 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         size_t cmax = l < Lv - 1 ? (size_t)(L[l + 1].C + L[l].C) : (size_t)L[l].C;
         big = std::max(big, (size_t)B * L[l].H * L[l].W * std::max(cmax, (size_t)u->cin_pad));
     }
+    u->big = big;
     u->gA = u->alloc<float>(big); u->gB = u->alloc<float>(big); u->gC = u->alloc<float>(big);
     u->tA = u->alloc<bf16>(big); u->tB = u->alloc<bf16>(big);
     u->xin = u->alloc<bf16>((size_t)B * L[0].H * L[0].W * u->cin_pad);
@@ -1411,6 +1499,9 @@ extern "C" int lb_unet_create(const lb_unet_desc *desc, uint64_t seed, lb_unet *
         const size_t nw = (size_t)u->convs[ci].co * 9 * u->convs[ci].ci;
         u->partial_n = std::max(u->partial_n, nchunk * nw);
         u->bpart_n = std::max(u->bpart_n, (size_t)bias_chunks((int)npix).nchunk * u->convs[ci].co);
+        // bias partials of the previous conv from this conv's fused input gradient
+        u->bpart_n = std::max(u->bpart_n, (size_t)dgrad_dpre_chunks((int)npix, L[l].H, L[l].W, u->convs[ci].co,
+                                                                    u->convs[ci].ci) * u->convs[ci].ci);
         // K-split scratch of the forward and input-gradient convs
         u->partial_n = std::max(u->partial_n, conv_split_scratch((int)npix, L[l].H, L[l].W, u->convs[ci].ci, u->convs[ci].co));
         u->partial_n = std::max(u->partial_n, conv_split_scratch((int)npix, L[l].H, L[l].W, u->convs[ci].co, u->convs[ci].ci));
@@ -1572,28 +1663,56 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
                        u->gA, npix0, C0, O);
     // gA = d(out of current conv), post-activation gradient
     float *gcur = u->gA, *gtmp = u->gB;
-    bf16 *gpre = (bf16 *)u->gC;    // pre-activation gradient, BF16
+    // pre-activation gradients (BF16): the two halves of gC, so a fused input
+    // gradient can write the next conv's while it reads this conv's
+    bf16 *gpre_buf[2] = {(bf16 *)u->gC, (bf16 *)u->gC + u->big};
+    int gsel = 0;
+    // set by a fused input gradient: the next conv's dpre (gpre_buf[gsel]) and
+    // its bias partials in bpart (pre_nchunk chunks) are ready
+    bool pre_ready = false;
+    int pre_nchunk = 0;
     int ci = last;
     // dpool: the incoming gradient of this conv's output is gcur (its skip
-    // gradient) plus the average-pool backward of dpool (see k_relu_bwd_bias)
+    // gradient) plus the average-pool backward of dpool (see k_relu_bwd_bias).
+    // fuse: this conv's input is the ReLU output of the conv processed next
+    // (same level): its input gradient also applies that conv's ReLU backward
+    // and bias partials (dgrad_dpre_launch), where the plan allows.
     auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din, const bf16 *low = nullptr, int cl = 0,
-                        const float *dpool = nullptr) {
+                        const float *dpool = nullptr, bool fuse = false) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
-        // ReLU backward + bias-gradient partials, summed in the weight
-        // gradient's chunk-sum launch
-        const BiasChunks bc = bias_chunks(npix);
-        hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, dpool ? u->gskip[l] : gcur,
-                           u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W);
-        wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1,
-                                 SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff}, low, cl);
-        if (need_din)
-            conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
+        bf16 *gpre = gpre_buf[gsel];
+        SumSeg bseg;
+        if (pre_ready) {
+            bseg = SumSeg{u->bpart, pre_nchunk, (size_t)c.co, u->g + c.boff};
+            pre_ready = false;
+        } else {
+            // ReLU backward + bias-gradient partials, summed in the weight
+            // gradient's chunk-sum launch
+            const BiasChunks bc = bias_chunks(npix);
+            hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, dpool ? u->gskip[l] : gcur,
+                               u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W);
+            bseg = SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff};
+        }
+        wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1, bseg,
+                                 low, cl);
+        if (!need_din) return;
+        if (fuse) {
+            const int nch = dgrad_dpre_launch(u, gpre, u->wt + u->wt_off[cidx], input, gpre_buf[1 - gsel], u->bpart, npix,
+                                              L[l].H, L[l].W, c.co, c.ci);
+            if (nch) {
+                gsel = 1 - gsel;
+                pre_ready = true;
+                pre_nchunk = nch;
+                return;
+            }
+        }
+        conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
         std::swap(gcur, gtmp);
     };
     for (int l = 0; l <= Lv - 2; l++) {
         const int lev = l;   // decoder levels in backward order: 0, 1, ..., Lv-2
-        conv_bwd(ci, u->act[ci - 1], lev, true);
+        conv_bwd(ci, u->act[ci - 1], lev, true, nullptr, 0, nullptr, true);
         ci--;
         if (u->cat_direct.size() > (size_t)lev && u->cat_direct[lev])
             conv_bwd(ci, u->act[enc_conv(lev) + 1], lev, true, u->cat_low[lev], L[lev + 1].C);
@@ -1616,7 +1735,7 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         const int c2 = 2 * l + 1, c1 = 2 * l;
         // the gradient of the encoder output act[c2] is its skip gradient
         // plus the pool backward of d(pool[l]), summed in the ReLU backward
-        conv_bwd(c2, u->act[c1], l, true, nullptr, 0, dpool);
+        conv_bwd(c2, u->act[c1], l, true, nullptr, 0, dpool, true);
         const bf16 *in1 = l == 0 ? u->xin : u->pool[l - 1];
         conv_bwd(c1, in1, l, l > 0);
         dpool = l > 0 ? gcur : nullptr;    // d(pool[l - 1])
