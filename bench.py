@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--layout", type=int, default=-1,
                     help="fix the population layout (0 [i][z][y][x], 1 [z][y][i][x]; -1 = autotune)")
     ap.add_argument("--sc-sweep", type=int, default=-1,
-                    help="Shan-Chen step form (0 density pass + sweep, 1 one z-marching sweep; -1 = autotune)")
+                    help="Shan-Chen step form (0 density pass + sweep, 1 / 2 one z-marching sweep on tiles, "
+                         "3 row-slab one-pass sweep of periodic boxes; -1 = autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--error-steps", type=int, default=5000,
                     help="steps of the FP64/FP32/FP16/BF16 error runs of the sphere workload (0 = skip)")

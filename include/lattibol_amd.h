@@ -234,10 +234,17 @@ typedef struct {
                                (two launches, psi through HBM), 1 or 2 one z-marching
                                sweep with the psi planes in LDS (one launch), keeping
                                the pulled populations in registers (1) or pulling them
-                               again from L2 for the sweep (2); all bit-identical.
-                               1 / 2 need Shan-Chen D3Q19, FP32 / FP16 / BF16 storage and
-                               plain accesses (mem_mode -1 or 0), else
-                               LB_ERR_UNSUPPORTED */
+                               again from L2 for the sweep (2), 3 one z-marching sweep
+                               over whole x-rows (row-slab tiles, no frame column) for
+                               fully periodic lattices; all bit-identical.
+                               1 / 2 / 3 need Shan-Chen D3Q19, FP32 / FP16 / BF16 storage
+                               and plain accesses (mem_mode -1 or 0), else
+                               LB_ERR_UNSUPPORTED.  3 also needs six periodic faces, no
+                               halo planes, Zou-He faces or solid nodes (lb_set_solid
+                               with a solid node then fails; under -1 the two passes
+                               take over), nx <= 512 and 19 N sizeof(population) < 4 GiB;
+                               block_x is then its workgroup (512, 768 or 1,024 threads)
+                               and block_y its rows per tile */
     int sc_zchunk;          /* z-planes per workgroup of the one-pass sweep, 0 = autotune */
 } lb_config;
 

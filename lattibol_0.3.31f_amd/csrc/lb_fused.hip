@@ -369,8 +369,15 @@ struct lb_ctx {
     // Shan-Chen D3Q19 step form: 0 density pass + sweep; 1 / 2 the one-pass
     // z-marching sweep (k_step_scz; the sweep's launch fields then hold its
     // tile shape), keeping the pulled populations in registers (1) or pulling
-    // them again for the sweep (2), with zc z-planes per workgroup
+    // them again for the sweep (2), with zc z-planes per workgroup; 3 the
+    // row-slab one-pass sweep of fully periodic lattices (k_step_scr; by =
+    // rows per tile, scr_v its wave / segment variant)
     int scz = 0, zc = 24;
+    int scr_v = 0;
+    int sc_req = -1;                  // the caller's sc_sweep
+    // the tuned two-pass sweep geometry, restored when solids make the
+    // row-slab form inapplicable (lb_set_solid)
+    int two_bx_shift = 6, two_by = 4, two_xcd = 0, two_mem = 0;
     void *dargs = nullptr;            // argument blocks: sweep from buffer 0 / 1, density pass from 0 / 1
     static constexpr size_t kArgBytes = 640;
 
@@ -400,6 +407,32 @@ int ilog2(int v) { int s = 0; while ((1 << s) < v) s++; return s; }
 bool scz_shape_ok(int bx, int by) {
     return (bx == 32 || bx == 64 || bx == 128) && (bx * by == 256 || bx * by == 512 || bx * by == 1024) &&
            2 * (bx + 2) + 2 * by <= bx * by && (bx + 2) * (by + 2) <= kZSlot;
+}
+
+// The row-slab one-pass sweep (k_step_scr) applies to Shan-Chen D3Q19
+// lattices with FP32 / FP16 / BF16 storage and plain accesses whose six
+// faces are periodic, with no halo planes, Zou-He faces or solid nodes, rows
+// of at most 512 nodes and a population buffer addressable with 32-bit byte
+// offsets.
+bool scr_lattice_ok(const lb_config &g, uint32_t N) {
+    if (g.lattice != 19 || g.sc_G == 0.0f || !scz_storage(g.storage) || g.mem_mode > 0) return false;
+    for (int k = 0; k < 6; k++) if (g.bc[k] != LB_BC_PERIODIC) return false;
+    if (g.halo_z || g.zou_he || g.nx > 512) return false;
+    const uint64_t es = g.storage == LB_STORE_F32 ? 4 : 2;
+    return (uint64_t)19 * N * es < (1ull << 32);
+}
+bool any_solid(const std::vector<uint8_t> &s) { return std::find(s.begin(), s.end(), 1) != s.end(); }
+// Largest tile (rows) of a variant: the rows plus the two frame rows, in
+// segments of 64 nodes, fit the variant's waves x segments per wave; its LDS
+// (four psi planes) stays within 64 KB.
+int scr_max_rows(int variant, int nx) {
+    const int nseg = (nx + 63) / 64;
+    int by = scr_variant_waves(variant) * scr_variant_spw(variant) / nseg - 2;
+    while (by >= 1 && scr_lds_bytes(nx, by) > 65536) by--;
+    return by;
+}
+bool scr_shape_ok(int variant, int nx, int by) {
+    return variant >= 0 && variant <= 2 && by >= 1 && by <= scr_max_rows(variant, nx);
 }
 
 // 16-bit stores round stochastically: asked for, or by default on Shan-Chen
@@ -451,7 +484,14 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src, bool dens = f
     a.nblocks = a.ntx * a.nty * c->cfg.nz;
     a.xcd = dens ? c->dxcd : c->xcd;
     a.zc = c->zc;
+    if (!dens && c->scz == 3) {
+        // row-slab tiles: ntx = 64-node segments per row, by rows per tile
+        a.bx_shift = 6;
+        a.ntx = (c->cfg.nx + 63) / 64;
+        a.nty = (c->cfg.ny + a.by - 1) / a.by;
+    }
     if (!dens && c->scz) a.nblocks = a.ntx * a.nty * ((c->cfg.nz + c->zc - 1) / c->zc);
+    if (!dens && c->scz == 3) a.nblocks = a.nty * ((c->cfg.nz + c->zc - 1) / c->zc);
     if (c->Q == 9) fast_path_args<9>(a, 1 << a.bx_shift, a.by);
     else fast_path_args<19>(a, 1 << a.bx_shift, a.by);
     return a;
@@ -491,6 +531,9 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     const StepArgs<S> *pa = (const StepArgs<S> *)((const char *)c->dargs + src * lb_ctx::kArgBytes);
     const bool sc = c->cfg.sc_G != 0.0f;
     if constexpr (Q == 19) {
+        if (sc && c->scz == 3)
+            return launch_step_scr(c->cfg.storage, macros, c->scr_v, (unsigned)a.nblocks,
+                                   scr_lds_bytes(c->cfg.nx, c->by), c->stream, pa, dstep);
         if (sc && c->scz)
             return launch_step_scz(c->cfg.storage, macros, c->scz == 1, (1 << c->bx_shift) * c->by,
                                    (unsigned)a.nblocks, c->stream, pa, dstep);
@@ -843,15 +886,18 @@ int autotune(lb_ctx *c) {
             (void)hipEventElapsedTime(&ms, e0, e1);
             return ms;
         };
-        struct ZCand { int bx, by, zc, xcd, form, layout; };
+        struct ZCand { int bx, by, zc, xcd, form, layout, v; };
         const int two_bx = c->bx_shift, two_by = c->by, two_xcd = c->xcd, two_mem = c->mem, two_lay = c->layout;
+        c->two_bx_shift = two_bx; c->two_by = two_by; c->two_xcd = two_xcd; c->two_mem = two_mem;
         const float two = c->cfg.sc_sweep > 0 ? 1e30f : time_steps();
         float zbest = 1e30f;
-        ZCand zb{64, 8, 24, 0, 1, c->layout};
+        ZCand zb{64, 8, 24, 0, 1, c->layout, 0};
         const int zcs[5] = {8, 16, 24, 32, 48};
         // tiles of 256 / 512 / 1,024 threads whose frame (2 (bx + 2) + 2 by) fits the workgroup
         const int shapes[8][2] = {{32, 8}, {64, 4}, {32, 16}, {64, 8}, {128, 4}, {32, 32}, {64, 16}, {128, 8}};
+        const bool forms12 = c->cfg.sc_sweep < 0 || c->cfg.sc_sweep == 1 || c->cfg.sc_sweep == 2;
         for (const auto &sh : shapes) {
+            if (!forms12) break;
             const int bx = sh[0], by = sh[1];
             if (c->cfg.block_x > 0 ? bx != c->cfg.block_x : (bx > 2 * c->cfg.nx && bx != 32)) continue;
             if (c->cfg.block_x > 0 && c->cfg.block_y > 0 && by != c->cfg.block_y) continue;
@@ -869,7 +915,7 @@ int autotune(lb_ctx *c) {
                             rc = upload_args(c);
                             if (rc) break;
                             const float ms = time_steps();
-                            if (ms < zbest) { zbest = ms; zb = {bx, by, zcv, xcd, form, lay}; }
+                            if (ms < zbest) { zbest = ms; zb = {bx, by, zcv, xcd, form, lay, 0}; }
                         }
                         if (rc) break;
                     }
@@ -879,8 +925,37 @@ int autotune(lb_ctx *c) {
             }
             if (rc) break;
         }
+        // the row-slab sweep of fully periodic lattices: variant x rows per
+        // tile (the three largest) x z-chunk x XCD remap, layout kept
+        if (!rc && (c->cfg.sc_sweep < 0 || c->cfg.sc_sweep == 3) && scr_lattice_ok(c->cfg, c->N) &&
+            !any_solid(c->solid)) {
+            const int rzcs[6] = {6, 8, 12, 16, 24, 32};
+            for (int v = 0; v < 3 && !rc; v++) {
+                if (c->cfg.block_x > 0 && c->cfg.block_x != 64 * scr_variant_waves(v)) continue;
+                const int bmax = scr_max_rows(v, c->cfg.nx);
+                const int bhi = c->cfg.block_y > 0 ? c->cfg.block_y : bmax;
+                const int blo = c->cfg.block_y > 0 ? c->cfg.block_y : std::max(1, bmax - 2);
+                for (int by = bhi; by >= blo && !rc; by--) {
+                    if (!scr_shape_ok(v, c->cfg.nx, by)) continue;
+                    for (int zc : rzcs) {
+                        if (c->cfg.sc_zchunk > 0 ? zc != rzcs[0] : (zc > c->cfg.nz && zc != rzcs[0])) continue;
+                        const int zcv = c->cfg.sc_zchunk > 0 ? c->cfg.sc_zchunk : zc;
+                        for (int xcd = 0; xcd < 2; xcd++) {
+                            if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
+                            c->scz = 3; c->zc = zcv; c->scr_v = v;
+                            c->bx_shift = 6; c->by = by; c->xcd = xcd; c->mem = 0; c->layout = two_lay;
+                            rc = upload_args(c);
+                            if (rc) break;
+                            const float ms = time_steps();
+                            if (ms < zbest) { zbest = ms; zb = {64, by, zcv, xcd, 3, two_lay, v}; }
+                        }
+                        if (rc) break;
+                    }
+                }
+            }
+        }
         if (!rc && zbest < two) {
-            c->scz = zb.form; c->zc = zb.zc;
+            c->scz = zb.form; c->zc = zb.zc; c->scr_v = zb.v;
             c->bx_shift = ilog2(zb.bx); c->by = zb.by; c->xcd = zb.xcd; c->mem = 0; c->layout = zb.layout;
         } else {
             c->scz = 0;
@@ -953,9 +1028,23 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     // Shan-Chen reads the psi of z-neighbours, which the halo planes do not
     // carry: a ring of several slabs would need a psi-plane exchange too
     if (cfg->halo_z && cfg->lattice == 19 && cfg->sc_G != 0.0f) return LB_ERR_UNSUPPORTED;
-    if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) return LB_ERR_INVALID;
-    if (cfg->sc_sweep < -1 || cfg->sc_sweep > 2 || cfg->sc_zchunk < 0) return LB_ERR_INVALID;
-    if (cfg->sc_sweep > 0 && (cfg->lattice != 19 || cfg->sc_G == 0.0f || !scz_storage(cfg->storage) ||
+    if (cfg->sc_sweep < -1 || cfg->sc_sweep > 3 || cfg->sc_zchunk < 0) return LB_ERR_INVALID;
+    // the row-slab one-pass sweep: block_x = its workgroup (512, 768 or 1,024
+    // threads), block_y = rows per tile
+    if (cfg->sc_sweep == 3) {
+        if (!scr_lattice_ok(*cfg, (uint32_t)N64)) return LB_ERR_UNSUPPORTED;
+        int v = -1;
+        for (int k = 0; k < 3; k++) if (cfg->block_x == 64 * scr_variant_waves(k)) v = k;
+        if (cfg->block_x > 0 && v < 0) return LB_ERR_UNSUPPORTED;
+        if (cfg->block_y > 0 && (v >= 0 ? !scr_shape_ok(v, cfg->nx, cfg->block_y)
+                                         : !(scr_shape_ok(0, cfg->nx, cfg->block_y) ||
+                                             scr_shape_ok(1, cfg->nx, cfg->block_y) ||
+                                             scr_shape_ok(2, cfg->nx, cfg->block_y))))
+            return LB_ERR_UNSUPPORTED;
+    } else if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) {
+        return LB_ERR_INVALID;
+    }
+    if ((cfg->sc_sweep == 1 || cfg->sc_sweep == 2) && (cfg->lattice != 19 || cfg->sc_G == 0.0f || !scz_storage(cfg->storage) ||
                                cfg->mem_mode > 0 ||
                                (cfg->block_x > 0 && !scz_shape_ok(cfg->block_x, cfg->block_y > 0 ? cfg->block_y
                                                                                        : kZThreads / cfg->block_x))))
@@ -967,6 +1056,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     lb_ctx *c = new (std::nothrow) lb_ctx();
     if (!c) return LB_ERR_INVALID;
     c->cfg = *cfg;
+    c->sc_req = cfg->sc_sweep;
     c->Q = cfg->lattice;
     c->N = (uint32_t)N64;
     c->P = (uint32_t)cfg->nx * cfg->ny;
@@ -1006,9 +1096,15 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             (!scq || cfg->sc_sweep == 0 || (cfg->sc_sweep > 0 && cfg->sc_zchunk > 0))) {
             c->scz = scq ? std::max(0, cfg->sc_sweep) : 0;
             if (c->scz) c->zc = cfg->sc_zchunk;
-            c->bx_shift = ilog2(cfg->block_x);
-            c->by = c->scz ? (cfg->block_y > 0 ? cfg->block_y : kZThreads / cfg->block_x)
-                           : std::max(1, kThreads / cfg->block_x);
+            if (c->scz == 3) {
+                for (int k = 0; k < 3; k++) if (cfg->block_x == 64 * scr_variant_waves(k)) c->scr_v = k;
+                c->bx_shift = 6;
+                c->by = cfg->block_y > 0 ? cfg->block_y : scr_max_rows(c->scr_v, cfg->nx);
+            } else {
+                c->bx_shift = ilog2(cfg->block_x);
+                c->by = c->scz ? (cfg->block_y > 0 ? cfg->block_y : kZThreads / cfg->block_x)
+                               : std::max(1, kThreads / cfg->block_x);
+            }
             c->xcd = cfg->xcd_remap;
             c->mem = cfg->mem_mode;
             c->layout = cfg->layout;
@@ -1027,7 +1123,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     if (rc) { delete c; return rc; }
     c->cfg.time_block = c->tb;
-    c->cfg.block_x = 1 << c->bx_shift;
+    c->cfg.block_x = c->scz == 3 ? 64 * scr_variant_waves(c->scr_v) : 1 << c->bx_shift;
     c->cfg.block_y = c->by;
     c->cfg.xcd_remap = c->xcd;
     c->cfg.mem_mode = c->mem;
@@ -1059,6 +1155,20 @@ extern "C" int lb_set_solid(lb_ctx *c, const uint8_t *solid) {
     if (!c || !solid) return LB_ERR_INVALID;
     for (uint32_t k = 0; k < c->N; k++) c->solid[k] = solid[k] ? 1 : 0;
     c->drop_graphs();
+    if (c->scz == 3 && any_solid(c->solid)) {
+        // the row-slab sweep has no solid nodes: the two passes take over
+        // (same population layout), or the caller's explicit choice fails
+        if (c->sc_req == 3) return LB_ERR_UNSUPPORTED;
+        c->scz = 0;
+        c->bx_shift = c->two_bx_shift; c->by = c->two_by; c->xcd = c->two_xcd; c->mem = c->two_mem;
+        c->cfg.sc_sweep = 0;
+        c->cfg.sc_zchunk = 0;
+        c->cfg.block_x = 1 << c->bx_shift;
+        c->cfg.block_y = c->by;
+        c->cfg.xcd_remap = c->xcd;
+        c->cfg.mem_mode = c->mem;
+        if (int rc = upload_args(c)) return rc;
+    }
     return upload_flags(c);
 }
 

@@ -757,6 +757,15 @@ bool scz_storage(int storage);
 int launch_step_scz(int storage, bool macros, bool keep, int threads, unsigned nblocks, hipStream_t stream,
                     const void *args, uint32_t dstep);
 
+// The row-slab one-pass sweep of fully periodic lattices (lb_scr.hip):
+// variant 0 / 1 / 2 = 8 / 12 / 16 waves handling up to 3 / 2 / 2 row
+// segments of 64 nodes each per plane; lds = scr_lds_bytes(nx, rows per tile).
+int scr_variant_waves(int variant);
+int scr_variant_spw(int variant);
+size_t scr_lds_bytes(int nx, int by);
+int launch_step_scr(int storage, bool macros, int variant, unsigned nblocks, size_t lds, hipStream_t stream,
+                    const void *args, uint32_t dstep);
+
 // ---------------------------------------------------------------------------
 // Node flags on the host: solid, inlet plane (+ face in the high nibble), and
 // F_SLOW with a link mask (lm) for nodes with a link that bounces back: bit i

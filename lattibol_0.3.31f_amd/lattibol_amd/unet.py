@@ -57,7 +57,8 @@ def make_targets(rho, ux, uy, u_in):
     """Physics-scaled targets (ux/u_in, uy/u_in, (rho-1)/u_in), (B, H, W, 3)."""
     u = np.asarray(u_in, np.float64).reshape(-1, 1, 1)
     t = np.stack([ux / u, uy / u, (rho - 1.0) / u], axis=-1)
-    return t.astype(np.float32)
+    with np.errstate(over="ignore", invalid="ignore"):   # diverged channels: callers drop non-finite targets
+        return t.astype(np.float32)
 
 
 class UNet:

@@ -1,0 +1,141 @@
+"""CPU restatement of the row-slab one-pass Shan-Chen sweep's indexing
+(k_step_scr in csrc/lb_scr.hip): workgroups of whole x-rows over z-chunks,
+waves handling 64-node row segments, 19 buffer loads per node with a scalar
+row / plane offset and a per-lane x offset, stores at the node, and four psi
+planes in LDS with one padding element per row end.
+
+For every workgroup, wave, segment and lane of a set of lattice shapes,
+launch shapes and both population layouts it checks that
+  * every load reads population i of the upstream node (x - c_i, periodic),
+    inside the population buffer;
+  * the sweeps store every node of the lattice exactly once per step;
+  * every psi the sweep reads from LDS is the psi of the neighbour x + c_i,
+    written by the march for the right plane and not yet overwritten, and
+    every LDS index stays inside the four slots the host allocates.
+The kernel's parity is checked on the GPU; this test pins the address
+arithmetic without one."""
+import numpy as np
+import pytest
+
+C19 = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (1, -1, 0),
+       (1, 0, 1), (1, 0, -1), (-1, 1, 0), (-1, -1, 0), (-1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, 1, -1),
+       (0, -1, 1), (0, -1, -1)]
+VARIANTS = {512: (8, 3), 768: (12, 2), 1024: (16, 2)}   # threads: (waves, segments per wave)
+
+
+def wrap1(v, n):
+    return np.where(v < 0, v + n, np.where(v >= n, v - n, v))
+
+
+def layout(kind, nx, ny, nz):
+    N = nx * ny * nz
+    if kind == "rows":
+        return nx, 19 * nx, 19 * nx * ny          # ps, sy, sz
+    return N, nx, nx * ny
+
+
+def max_rows(threads, nx):
+    nw, spw = VARIANTS[threads]
+    nseg = (nx + 63) // 64
+    by = nw * spw // nseg - 2
+    while by >= 1 and 4 * (by + 2) * (nx + 2) * 4 > 65536:
+        by -= 1
+    return by
+
+
+def emulate(nx, ny, nz, threads, BY, zc, kind):
+    nw, spw = VARIANTS[threads]
+    ps, sy, sz = layout(kind, nx, ny, nz)
+    N = nx * ny * nz
+    nseg = (nx + 63) // 64
+    W = nx + 2
+    slot = (BY + 2) * W
+    nty = -(-ny // BY)
+    nblocks = nty * -(-nz // zc)
+    stored = np.zeros(N, np.int64)
+    lane = np.arange(64)
+    for b in range(nblocks):
+        ty, zch = b % nty, b // nty
+        y0 = ty * BY
+        bye = min(BY, ny - y0)
+        z0 = zch * zc
+        z1 = min(nz, z0 + zc)
+        # LDS: (node id, plane k) written at each position of the four slots
+        lds_node = np.full(4 * slot, -1, np.int64)
+        lds_k = np.full(4 * slot, -99, np.int64)
+        segs = []
+        for w in range(nw):
+            for j in range(spw):
+                s = w + j * nw
+                r, xs = s // nseg, s % nseg
+                x = xs * 64 + lane
+                segs.append(dict(r=r, active=r < bye + 2, interior=1 <= r <= bye,
+                                 y=int(wrap1(np.array(y0 - 1 + r), ny)), x=x, ok=x < nx))
+        for k in range(z0 - 1, z1 + 2):
+            zk = int(wrap1(np.array(k), nz))
+
+            def pull_check(g, z):
+                x, ok = g["x"][g["ok"]], g["ok"]
+                for i, (cx, cy, cz) in enumerate(C19):
+                    so = i * ps + int(wrap1(np.array(g["y"] - cy), ny)) * sy + int(wrap1(np.array(z - cz), nz)) * sz
+                    e = so + wrap1(x - cx, nx)
+                    want = i * ps + wrap1(x - cx, nx) + ((g["y"] - cy) % ny) * sy + ((z - cz) % nz) * sz
+                    assert (e == want).all() and (e >= 0).all() and (e < 19 * N).all()
+                return ok
+
+            q = k - 2
+            if q >= z0:
+                for g in segs:
+                    if not g["interior"]:
+                        continue
+                    pull_check(g, q)
+                    x = g["x"][g["ok"]]
+                    ids = x + nx * (g["y"] + ny * q)
+                    stored[ids] += 1
+                    c = g["r"] * W + x + 1
+                    for (cx, cy, cz) in C19[1:] + [(0, 0, 0)]:
+                        pl = (q + cz + 4) & 3
+                        pos = pl * slot + c + cx + cy * W
+                        assert (pos >= 0).all() and (pos < 4 * slot).all()
+                        want = ((x + cx) % nx) + nx * (((g["y"] + cy) % ny) + ny * ((q + cz) % nz))
+                        assert (lds_node[pos] == want).all(), (b, k, cx, cy, cz)
+                        assert (lds_k[pos] == q + cz).all()
+            if k <= z1:
+                for g in segs:
+                    if not g["active"]:
+                        continue
+                    pull_check(g, zk)
+                    x = g["x"][g["ok"]]
+                    ids = x + nx * (g["y"] + ny * zk)
+                    base = ((k + 4) & 3) * slot + g["r"] * W
+                    for pos, sel in ((base + x + 1, slice(None)), (base + 0 * x, x == nx - 1),
+                                     (base + nx + 1 + 0 * x, x == 0)):
+                        assert (pos[sel] >= 0).all() and (pos[sel] < 4 * slot).all()
+                        lds_node[pos[sel]] = ids[sel]
+                        lds_k[pos[sel]] = k
+    assert (stored == 1).all(), (stored.min(), stored.max())
+
+
+# (shape, workgroup threads, rows per tile (0: the largest), z-chunk): one to
+# eight 64-node segments per row (partial last segments), tiles and chunks
+# that do not divide the lattice, a single plane, one-row tiles
+CASES = [((24, 24, 24), 512, 0, 5), ((24, 24, 24), 1024, 1, 1), ((70, 13, 11), 768, 3, 4),
+         ((70, 13, 11), 1024, 0, 16), ((130, 5, 3), 512, 0, 2), ((130, 5, 3), 768, 2, 1),
+         ((64, 7, 1), 512, 4, 3), ((200, 3, 4), 1024, 0, 3), ((512, 2, 3), 1024, 0, 2)]
+
+
+@pytest.mark.parametrize("kind", ["soa", "rows"])
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_row_slab_indexing(case, kind):
+    (nx, ny, nz), threads, by, zc = CASES[case]
+    emulate(nx, ny, nz, threads, by or max_rows(threads, nx), zc, kind)
+
+
+@pytest.mark.parametrize("threads", sorted(VARIANTS))
+def test_row_slab_c3_launch_shapes(threads):
+    """Rows of 192 nodes (the 192^3 spinodal box) at each variant's largest
+    tile and two z-chunks, on a lattice cut to three row tiles and two
+    z-chunks plus one plane (every tile and chunk runs the same march)."""
+    by = max_rows(threads, 192)
+    for zc in (6, 12):
+        emulate(192, 2 * by + 3, 2 * zc + 1, threads, by, zc, "rows")
