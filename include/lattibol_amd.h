@@ -243,7 +243,7 @@ typedef struct {
                                halo planes, Zou-He faces or solid nodes (lb_set_solid
                                with a solid node then fails; under -1 the two passes
                                take over), nx <= 512 and 19 N sizeof(population) < 4 GiB;
-                               block_x is then its workgroup (512, 768 or 1,024 threads)
+                               block_x is then its workgroup (384, 512, 768 or 1,024 threads)
                                and block_y its rows per tile */
     int sc_zchunk;          /* z-planes per workgroup of the one-pass sweep, 0 = autotune */
 } lb_config;

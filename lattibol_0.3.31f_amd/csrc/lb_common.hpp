@@ -129,10 +129,15 @@ LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16,
 
 // Stochastic rounding of 16-bit stores.  Random bits per (step, node, slot):
 // the SplitMix64 finaliser of step * golden + node, then Wellons' lowbias32
-// of its low word plus slot * 0x9E3779B9 (a counter-based stream: the same
-// bits on any device, launch shape or thread count).  The value rounds up in
-// magnitude with probability (distance to the lower neighbour) / spacing, so
-// the stored value is unbiased and increments below half a spacing survive.
+// of its low word plus (slot / 2) * 0x9E3779B9, whose high 16 bits serve
+// the even slot and low 16 bits the odd one (a counter-based stream: the same
+// bits on any device, launch shape or thread count; one hash per two
+// populations).  The 16 bits sit in the top half of the word the rounding
+// functions read.  The value rounds up in magnitude with probability
+// (distance to the lower neighbour) / spacing, so the stored value is
+// unbiased (exactly for BF16 and FP16 normals, whose dropped fractions have
+// 16 and 13 bits; FP16 subnormals round up with that probability quantised
+// to 2^-16) and increments below half a spacing survive.
 // The node's word (64-bit arithmetic, once per node) and each slot's bits
 // from it; sr_bits is the two together.
 LB_DEV uint32_t sr_seed(uint32_t step, uint32_t id) {
@@ -141,16 +146,18 @@ LB_DEV uint32_t sr_seed(uint32_t step, uint32_t id) {
     return (uint32_t)z;
 }
 LB_DEV uint32_t sr_mix(uint32_t seed, int slot) {
-    uint32_t x = seed + (uint32_t)slot * 0x9E3779B9u;
+    uint32_t x = seed + (uint32_t)(slot >> 1) * 0x9E3779B9u;
     x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
-    return x;
+    return (slot & 1) ? x << 16 : x & 0xffff0000u;
 }
 LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) { return sr_mix(sr_seed(step, id), slot); }
 // BF16: add 16 random bits below the kept ones and truncate.
+// Both results are formed and one selected (no branch per population).
 LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
     const uint32_t u = __builtin_bit_cast(uint32_t, f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return f32_to_bf16(f);
-    return (uint16_t)((u + (r >> 16)) >> 16);
+    const uint16_t t = (uint16_t)((u + (r >> 16)) >> 16);
+    const uint16_t n = f32_to_bf16(f);
+    return (u & 0x7f800000u) == 0x7f800000u ? n : t;
 }
 // FP16: the same with 13 dropped bits in the normal range (the truncated
 // value is exact in FP16); below 2^-14 the spacing is 2^-24:
@@ -160,21 +167,28 @@ LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
 // exactly: for k > 23 the fraction dropped from m 2^(23-k) cannot carry,
 // since m 2^(23-k) + r23 crosses a multiple of 2^23 only through its
 // integer part.  Integer arithmetic only.
+// The normal-range result is formed for every lane; the subnormal and
+// overflow results only when a lane of the wave needs them (a wave-uniform
+// branch, no divergence), then one is selected per lane.  For k <= 23 the sum
+// splits as (m >> k) + (((m mod 2^k) 2^(23-k) + r23) >> 23), whose second
+// term stays below 2^24, so 32-bit words suffice; k is clamped to [14, 47]
+// (at 47 and above q is 0 either way).
 LB_DEV uint16_t f32_to_f16_sr(float f, uint32_t r) {
     const uint32_t x = __builtin_bit_cast(uint32_t, f);
     const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
-    if (ax >= 0x477ff000u) return f32_to_f16(f);
-    if (ax < 0x38800000u) {
-        const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | (e ? 0x800000u : 0u);
-        const int k = 126 - (int)(e ? e : 1u);
-        const uint32_t r23 = r >> 9;
-        uint32_t q;
-        if (k <= 23) q = (uint32_t)((((uint64_t)m << (23 - k)) + r23) >> 23);
-        else q = ((k - 23 >= 32 ? 0u : (m >> (k - 23))) + r23) >> 23;
-        return (uint16_t)(sign | q);
-    }
     const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
-    return (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
+    const uint16_t hn = (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
+    const bool rare = ax >= 0x477ff000u || ax < 0x38800000u;
+    if (__builtin_amdgcn_ballot_w64(rare) == 0) return hn;
+    const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | (e ? 0x800000u : 0u);
+    const int k0 = 126 - (int)(e ? e : 1u);
+    const int k = k0 < 14 ? 14 : (k0 > 47 ? 47 : k0);
+    const uint32_t r23 = r >> 9;
+    const uint32_t ql = (m >> (k & 31)) + ((((m & ((1u << (k & 31)) - 1u)) << ((23 - k) & 31)) + r23) >> 23);
+    const uint32_t qh = ((m >> ((k - 23) & 31)) + r23) >> 23;
+    const uint16_t hs = (uint16_t)(sign | (k <= 23 ? ql : qh));
+    const uint16_t ho = f32_to_f16(f);
+    return ax >= 0x477ff000u ? ho : (ax < 0x38800000u ? hs : hn);
 }
 
 // DEV = false: the population itself is stored and the arithmetic follows the
@@ -246,13 +260,45 @@ LB_DEV double exp_cw(double x) {
     return p * __builtin_bit_cast(double, bits);
 }
 
-// psi = rho0 (1 - exp(-rho/rho0)) in double, rounded to T.
-template <class T> LB_DEV T sc_psi(T rho, float rho0) {
-    // rho0 = 1 (a uniform branch) skips the FP64 division and product, which
-    // are exact there: the same bits either way
-    if (rho0 == 1.0f) return (T)(1.0 - exp_cw(-(double)rho));
-    double r = (double)rho / (double)rho0;
-    return (T)((double)rho0 * (1.0 - exp_cw(-r)));
+// exp in single precision: Cody-Waite reduction x = k ln2 + r with ln2 split
+// into a 15-bit head (k * head exact for |k| < 2^9) and a tail, a degree-7
+// Taylor polynomial in Horner form with fused multiply-adds on |r| <= ln2/2
+// (truncation below 1e-8 relative), and scaling by 2^k through the exponent
+// field.  Below -87 the result is 0, above 88 +inf, NaN stays NaN; within
+// [-87, 88] it is within 2 ulp of exp.  Selects instead of branches, so a
+// wave runs one straight-line sequence.  The host oracle evaluates the same
+// IEEE operations.
+LB_DEV float exp_f(float x) {
+    const float xc = x < -87.0f ? -87.0f : (x > 88.0f ? 88.0f : x);
+    const float kf = __builtin_rintf(xc * 0x1.715476p+0f);        // 1 / ln2
+    float r = __builtin_fmaf(-kf, 0x1.62e4p-1f, xc);                // ln2 head
+    r = __builtin_fmaf(-kf, 0x1.7f7d1cp-20f, r);                    // ln2 tail
+    float p = 1.0f / 5040.0f;
+    p = __builtin_fmaf(p, r, 1.0f / 720.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 120.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 24.0f);
+    p = __builtin_fmaf(p, r, 1.0f / 6.0f);
+    p = __builtin_fmaf(p, r, 0.5f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    p = __builtin_fmaf(p, r, 1.0f);
+    const float v = p * __builtin_bit_cast(float, (uint32_t)((int)kf + 127) << 23);
+    const float w = x < -87.0f ? 0.0f : (x > 88.0f ? __builtin_huge_valf() : v);
+    return x != x ? x : w;
+}
+
+// psi = rho0 (1 - exp(-rho/rho0)): FP32 collisions evaluate it in single
+// precision (exp_f), FP64 collisions in double (exp_cw).
+template <class T> LB_DEV T sc_psi(T rho, float rho0);
+template <> LB_DEV float sc_psi<float>(float rho, float rho0) {
+    if (rho0 == 1.0f) return 1.0f - exp_f(-rho);
+    return rho0 * (1.0f - exp_f(-(rho / rho0)));
+}
+template <> LB_DEV double sc_psi<double>(double rho, float rho0) {
+    // rho0 = 1 (a uniform branch) skips the division and product, which are
+    // exact there: the same bits either way
+    if (rho0 == 1.0f) return 1.0 - exp_cw(-rho);
+    double r = rho / (double)rho0;
+    return (double)rho0 * (1.0 - exp_cw(-r));
 }
 
 // ---------------------------------------------------------------------------

@@ -371,9 +371,9 @@ struct lb_ctx {
     // tile shape), keeping the pulled populations in registers (1) or pulling
     // them again for the sweep (2), with zc z-planes per workgroup; 3 the
     // row-slab one-pass sweep of fully periodic lattices (k_step_scr; by =
-    // rows per tile, scr_v its wave / segment variant)
+    // rows per tile, scr_nw its waves per workgroup)
     int scz = 0, zc = 24;
-    int scr_v = 0;
+    int scr_nw = 8;
     int sc_req = -1;                  // the caller's sc_sweep
     // the tuned two-pass sweep geometry, restored when solids make the
     // row-slab form inapplicable (lb_set_solid)
@@ -422,17 +422,8 @@ bool scr_lattice_ok(const lb_config &g, uint32_t N) {
     return (uint64_t)19 * N * es < (1ull << 32);
 }
 bool any_solid(const std::vector<uint8_t> &s) { return std::find(s.begin(), s.end(), 1) != s.end(); }
-// Largest tile (rows) of a variant: the rows plus the two frame rows, in
-// segments of 64 nodes, fit the variant's waves x segments per wave; its LDS
-// (four psi planes) stays within 64 KB.
-int scr_max_rows(int variant, int nx) {
-    const int nseg = (nx + 63) / 64;
-    int by = scr_variant_waves(variant) * scr_variant_spw(variant) / nseg - 2;
-    while (by >= 1 && scr_lds_bytes(nx, by) > 65536) by--;
-    return by;
-}
-bool scr_shape_ok(int variant, int nx, int by) {
-    return variant >= 0 && variant <= 2 && by >= 1 && by <= scr_max_rows(variant, nx);
+bool scr_shape_ok(int nw, int nx, int by, int storage) {
+    return (nw == 6 || nw == 8 || nw == 12 || nw == 16) && by >= 1 && by <= scr_max_rows(nw, nx, storage);
 }
 
 // 16-bit stores round stochastically: asked for, or by default on Shan-Chen
@@ -532,8 +523,8 @@ int launch_step_t(lb_ctx *c, int src, bool macros, uint32_t dstep) {
     const bool sc = c->cfg.sc_G != 0.0f;
     if constexpr (Q == 19) {
         if (sc && c->scz == 3)
-            return launch_step_scr(c->cfg.storage, macros, c->scr_v, (unsigned)a.nblocks,
-                                   scr_lds_bytes(c->cfg.nx, c->by), c->stream, pa, dstep);
+            return launch_step_scr(c->cfg.storage, macros, c->scr_nw, c->by, c->cfg.nx, (unsigned)a.nblocks,
+                                   c->stream, pa, dstep);
         if (sc && c->scz)
             return launch_step_scz(c->cfg.storage, macros, c->scz == 1, (1 << c->bx_shift) * c->by,
                                    (unsigned)a.nblocks, c->stream, pa, dstep);
@@ -891,7 +882,7 @@ int autotune(lb_ctx *c) {
         c->two_bx_shift = two_bx; c->two_by = two_by; c->two_xcd = two_xcd; c->two_mem = two_mem;
         const float two = c->cfg.sc_sweep > 0 ? 1e30f : time_steps();
         float zbest = 1e30f;
-        ZCand zb{64, 8, 24, 0, 1, c->layout, 0};
+        ZCand zb{64, 8, 24, 0, 1, c->layout, 8};
         const int zcs[5] = {8, 16, 24, 32, 48};
         // tiles of 256 / 512 / 1,024 threads whose frame (2 (bx + 2) + 2 by) fits the workgroup
         const int shapes[8][2] = {{32, 8}, {64, 4}, {32, 16}, {64, 8}, {128, 4}, {32, 32}, {64, 16}, {128, 8}};
@@ -925,37 +916,43 @@ int autotune(lb_ctx *c) {
             }
             if (rc) break;
         }
-        // the row-slab sweep of fully periodic lattices: variant x rows per
-        // tile (the three largest) x z-chunk x XCD remap, layout kept
+        // the row-slab sweep of fully periodic lattices: waves per workgroup x
+        // rows per tile x z-chunk with the XCD remap on, then the remap off for
+        // the best shape; the population layout kept
         if (!rc && (c->cfg.sc_sweep < 0 || c->cfg.sc_sweep == 3) && scr_lattice_ok(c->cfg, c->N) &&
             !any_solid(c->solid)) {
-            const int rzcs[6] = {6, 8, 12, 16, 24, 32};
-            for (int v = 0; v < 3 && !rc; v++) {
-                if (c->cfg.block_x > 0 && c->cfg.block_x != 64 * scr_variant_waves(v)) continue;
-                const int bmax = scr_max_rows(v, c->cfg.nx);
+            const int rzcs[8] = {6, 8, 10, 12, 16, 20, 24, 32};
+            float rbest = 1e30f;
+            ZCand rb{64, 0, 0, 1, 3, two_lay, 8};
+            auto try_r = [&](int nw, int by, int zcv, int xcd) {
+                c->scz = 3; c->zc = zcv; c->scr_nw = nw;
+                c->bx_shift = 6; c->by = by; c->xcd = xcd; c->mem = 0; c->layout = two_lay;
+                rc = upload_args(c);
+                if (rc) return;
+                const float ms = time_steps();
+                if (ms < rbest) { rbest = ms; rb = {64, by, zcv, xcd, 3, two_lay, nw}; }
+            };
+            const int xcd0 = c->cfg.xcd_remap >= 0 ? c->cfg.xcd_remap : 1;
+            for (int nw : {6, 8, 12, 16}) {
+                if (rc) break;
+                if (c->cfg.block_x > 0 && c->cfg.block_x != 64 * nw) continue;
+                const int bmax = scr_max_rows(nw, c->cfg.nx, c->cfg.storage);
                 const int bhi = c->cfg.block_y > 0 ? c->cfg.block_y : bmax;
-                const int blo = c->cfg.block_y > 0 ? c->cfg.block_y : std::max(1, bmax - 2);
+                const int blo = c->cfg.block_y > 0 ? c->cfg.block_y : std::max(1, bmax - 7);
                 for (int by = bhi; by >= blo && !rc; by--) {
-                    if (!scr_shape_ok(v, c->cfg.nx, by)) continue;
+                    if (!scr_shape_ok(nw, c->cfg.nx, by, c->cfg.storage)) continue;
                     for (int zc : rzcs) {
                         if (c->cfg.sc_zchunk > 0 ? zc != rzcs[0] : (zc > c->cfg.nz && zc != rzcs[0])) continue;
-                        const int zcv = c->cfg.sc_zchunk > 0 ? c->cfg.sc_zchunk : zc;
-                        for (int xcd = 0; xcd < 2; xcd++) {
-                            if (c->cfg.xcd_remap >= 0 && xcd != c->cfg.xcd_remap) continue;
-                            c->scz = 3; c->zc = zcv; c->scr_v = v;
-                            c->bx_shift = 6; c->by = by; c->xcd = xcd; c->mem = 0; c->layout = two_lay;
-                            rc = upload_args(c);
-                            if (rc) break;
-                            const float ms = time_steps();
-                            if (ms < zbest) { zbest = ms; zb = {64, by, zcv, xcd, 3, two_lay, v}; }
-                        }
+                        try_r(nw, by, c->cfg.sc_zchunk > 0 ? c->cfg.sc_zchunk : zc, xcd0);
                         if (rc) break;
                     }
                 }
             }
+            if (!rc && rbest < 1e30f && c->cfg.xcd_remap < 0) try_r(rb.v, rb.by, rb.zc, 1 - xcd0);
+            if (!rc && rbest < zbest) { zbest = rbest; zb = rb; }
         }
         if (!rc && zbest < two) {
-            c->scz = zb.form; c->zc = zb.zc; c->scr_v = zb.v;
+            c->scz = zb.form; c->zc = zb.zc; c->scr_nw = zb.v;
             c->bx_shift = ilog2(zb.bx); c->by = zb.by; c->xcd = zb.xcd; c->mem = 0; c->layout = zb.layout;
         } else {
             c->scz = 0;
@@ -1029,17 +1026,18 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     // carry: a ring of several slabs would need a psi-plane exchange too
     if (cfg->halo_z && cfg->lattice == 19 && cfg->sc_G != 0.0f) return LB_ERR_UNSUPPORTED;
     if (cfg->sc_sweep < -1 || cfg->sc_sweep > 3 || cfg->sc_zchunk < 0) return LB_ERR_INVALID;
-    // the row-slab one-pass sweep: block_x = its workgroup (512, 768 or 1,024
+    // the row-slab one-pass sweep: block_x = its workgroup (384, 512, 768 or 1,024
     // threads), block_y = rows per tile
     if (cfg->sc_sweep == 3) {
         if (!scr_lattice_ok(*cfg, (uint32_t)N64)) return LB_ERR_UNSUPPORTED;
-        int v = -1;
-        for (int k = 0; k < 3; k++) if (cfg->block_x == 64 * scr_variant_waves(k)) v = k;
-        if (cfg->block_x > 0 && v < 0) return LB_ERR_UNSUPPORTED;
-        if (cfg->block_y > 0 && (v >= 0 ? !scr_shape_ok(v, cfg->nx, cfg->block_y)
-                                         : !(scr_shape_ok(0, cfg->nx, cfg->block_y) ||
-                                             scr_shape_ok(1, cfg->nx, cfg->block_y) ||
-                                             scr_shape_ok(2, cfg->nx, cfg->block_y))))
+        const int nw = cfg->block_x / 64;
+        if (cfg->block_x > 0 && (cfg->block_x % 64 || (nw != 6 && nw != 8 && nw != 12 && nw != 16)))
+            return LB_ERR_UNSUPPORTED;
+        if (cfg->block_y > 0 && (cfg->block_x > 0 ? !scr_shape_ok(nw, cfg->nx, cfg->block_y, cfg->storage)
+                                                   : !(scr_shape_ok(6, cfg->nx, cfg->block_y, cfg->storage) ||
+                                                       scr_shape_ok(8, cfg->nx, cfg->block_y, cfg->storage) ||
+                                                       scr_shape_ok(12, cfg->nx, cfg->block_y, cfg->storage) ||
+                                                       scr_shape_ok(16, cfg->nx, cfg->block_y, cfg->storage))))
             return LB_ERR_UNSUPPORTED;
     } else if (cfg->block_x && (cfg->block_x & (cfg->block_x - 1) || cfg->block_x > kThreads)) {
         return LB_ERR_INVALID;
@@ -1097,9 +1095,9 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->scz = scq ? std::max(0, cfg->sc_sweep) : 0;
             if (c->scz) c->zc = cfg->sc_zchunk;
             if (c->scz == 3) {
-                for (int k = 0; k < 3; k++) if (cfg->block_x == 64 * scr_variant_waves(k)) c->scr_v = k;
+                c->scr_nw = cfg->block_x / 64;
                 c->bx_shift = 6;
-                c->by = cfg->block_y > 0 ? cfg->block_y : scr_max_rows(c->scr_v, cfg->nx);
+                c->by = cfg->block_y > 0 ? cfg->block_y : scr_max_rows(c->scr_nw, cfg->nx, cfg->storage);
             } else {
                 c->bx_shift = ilog2(cfg->block_x);
                 c->by = c->scz ? (cfg->block_y > 0 ? cfg->block_y : kZThreads / cfg->block_x)
@@ -1123,7 +1121,7 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
     if (!rc) rc = (int)hipStreamSynchronize(c->stream);
     if (rc) { delete c; return rc; }
     c->cfg.time_block = c->tb;
-    c->cfg.block_x = c->scz == 3 ? 64 * scr_variant_waves(c->scr_v) : 1 << c->bx_shift;
+    c->cfg.block_x = c->scz == 3 ? 64 * c->scr_nw : 1 << c->bx_shift;
     c->cfg.block_y = c->by;
     c->cfg.xcd_remap = c->xcd;
     c->cfg.mem_mode = c->mem;

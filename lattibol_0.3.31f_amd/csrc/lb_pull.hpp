@@ -757,13 +757,12 @@ bool scz_storage(int storage);
 int launch_step_scz(int storage, bool macros, bool keep, int threads, unsigned nblocks, hipStream_t stream,
                     const void *args, uint32_t dstep);
 
-// The row-slab one-pass sweep of fully periodic lattices (lb_scr.hip):
-// variant 0 / 1 / 2 = 8 / 12 / 16 waves handling up to 3 / 2 / 2 row
-// segments of 64 nodes each per plane; lds = scr_lds_bytes(nx, rows per tile).
-int scr_variant_waves(int variant);
-int scr_variant_spw(int variant);
+// The row-slab one-pass sweep of fully periodic lattices (lb_scr.hip): a
+// workgroup of nw = 8, 12 or 16 waves over tiles of `by` whole rows; the
+// largest tile a storage admits, its LDS bytes, and the launch.
+int scr_max_rows(int nw, int nx, int storage);
 size_t scr_lds_bytes(int nx, int by);
-int launch_step_scr(int storage, bool macros, int variant, unsigned nblocks, size_t lds, hipStream_t stream,
+int launch_step_scr(int storage, bool macros, int nw, int by, int nx, unsigned nblocks, hipStream_t stream,
                     const void *args, uint32_t dstep);
 
 // ---------------------------------------------------------------------------

@@ -2,26 +2,37 @@
 // inlets, Zou-He faces or halo planes (a multiphase box such as spinodal
 // decomposition): the sweep of one time step in one launch, with psi in LDS.
 //
-// A workgroup owns whole x-rows: BY rows of the lattice (plus one frame row on
-// each side, periodic in y) over a chunk of zc z-planes, and marches up through
-// the chunk.  Each of its NW waves handles up to SPW row segments of 64 nodes
-// per plane (segment s = wave + j NW, row s / nseg, x-segment s % nseg), so
-// the frame rows are spread over the waves and there is no frame column at
-// all: the x-wrap stays inside the row (LDS rows carry one padding element per
-// side holding the psi of the opposite end).
+// A workgroup of NW waves owns whole x-rows: BY rows of the lattice plus one
+// frame row on each side (periodic in y), over a chunk of zc z-planes, and
+// marches up through the chunk.  Rows are cut into segments of 64 nodes (one
+// wave each).  Wave w holds the interior segments s = w + j NW (j < SPI, row
+// 1 + s / nseg) and at most one frame segment f = w (row 0 or BY + 1), so
+// there is no frame column at all: the x-wrap stays inside the row (LDS rows
+// carry one padding element per side holding the psi of the opposite end).
 //
 // Iteration k of the march:
-//   A  issue the population loads of plane k (every segment, frame included);
-//   B  sweep plane k - 2 (interior segments): pull its populations again (the
-//      workgroup read them two iterations before, so they come from L2),
-//      moments, the Shan-Chen force from the psi of planes k - 3 .. k - 1 in
-//      LDS, BGK, stores;
-//   C  wait for the loads of A, psi of plane k into LDS slot k & 3;
+//   A   issue the population loads of plane k (the wave's interior and frame
+//       segments), one register per value, no wait;
+//   B1  sweep plane k - 2 from the populations the wave loaded two iterations
+//       before and kept in registers: moments, the Shan-Chen force from the
+//       psi of planes k - 3 .. k - 1 in LDS, BGK; the values to store stay
+//       in registers (two 16-bit values per register);
+//   C   wait for the loads of A, psi of plane k into LDS slot k & 3, and the
+//       interior populations packed for their sweep;
+//   B2  issue the stores of plane k - 2;
 //   one workgroup barrier.
-// The loads of A are in flight while B computes and stores, so the memory
-// pipe does not drain at every plane.  Four LDS slots suffice: the slot
-// written at iteration k was last read by the sweep of iteration k - 1,
-// which every wave finished before that iteration's barrier.
+// The loads of A are in flight while B1 computes.  The stores come after C:
+// vector memory counts loads and stores together, so stores issued between A
+// and C would make C wait for them too (measured: issuing them at the start
+// of the next iteration instead, beside the loads, is 8 % slower).  Every
+// population crosses HBM once
+// per step (the frame rows and the planes just outside the chunk are loaded
+// again by the workgroups that own them, mostly from L2).  Three planes of
+// populations live in registers per interior segment; the 16-bit storages
+// keep the two older ones with two populations per register.  Four LDS
+// slots suffice: the slot written at iteration k was last read by the sweep
+// of iteration k - 1, which every wave finished before that iteration's
+// barrier.
 //
 // Every psi and every swept node runs the arithmetic of the two-pass step
 // (psi_of_node / step_body through the same moment, force, collision and
@@ -40,11 +51,8 @@
 namespace lb {
 namespace {
 
-template <class R> LB_DEV R scr_ld(rsrc_t r, uint32_t vo, uint32_t so);
-template <> LB_DEV uint16_t scr_ld<uint16_t>(rsrc_t r, uint32_t vo, uint32_t so) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0);
-}
-template <> LB_DEV float scr_ld<float>(rsrc_t r, uint32_t vo, uint32_t so) {
+LB_DEV uint16_t scr_ld16(rsrc_t r, uint32_t vo, uint32_t so) { return __builtin_amdgcn_raw_buffer_load_b16(r, vo, so, 0); }
+LB_DEV float scr_ld32(rsrc_t r, uint32_t vo, uint32_t so) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
 }
 LB_DEV void scr_st(uint16_t v, rsrc_t r, uint32_t vo, uint32_t so) {
@@ -57,30 +65,107 @@ LB_DEV void scr_st(float v, rsrc_t r, uint32_t vo, uint32_t so) {
 // Wrapped coordinate v in [-1, n] (one step outside at most).
 LB_DEV int wrap1(int v, int n) { return v < 0 ? v + n : (v >= n ? v - n : v); }
 
-// A row segment of the tile: row r of the (BY + 2)-row window (0 and
-// rows - 1 are the frame), lattice row y, and the lane's x.
-struct Seg {
-    int r, y, x;
-    bool active;     // the segment exists in this tile (uniform per wave)
-    bool interior;   // a row the workgroup sweeps (uniform per wave)
-    bool lane;       // the lane's x lies in the lattice
+// The 19 raw populations of one node, held in registers: two 16-bit values
+// per register, or one FP32 value.
+template <class S, bool P = sizeof(typename S::raw) == 2> struct Held;
+template <class S> struct Held<S, true> {
+    typedef unsigned short us2 __attribute__((ext_vector_type(2)));
+    uint32_t w[10];
+    template <int I> LB_DEV uint16_t get() const {
+        return (I & 1) ? (uint16_t)(w[I >> 1] >> 16) : (uint16_t)(w[I >> 1] & 0xffffu);
+    }
+    // loads slot i into its half (odd slots high)
+    template <class F> LB_DEV void fill(F &&ld) {
+        static_for<0, 9>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            us2 v;
+            v.x = ld(std::integral_constant<int, 2 * m>{});
+            v.y = ld(std::integral_constant<int, 2 * m + 1>{});
+            w[m] = __builtin_bit_cast(uint32_t, v);
+        });
+        w[9] = (uint32_t)ld(std::integral_constant<int, 18>{});
+    }
+};
+template <class S> struct Held<S, false> {
+    float w[19];
+    template <int I> LB_DEV float get() const { return w[I]; }
+    template <class F> LB_DEV void fill(F &&ld) {
+        static_for<0, 19>([&](auto I) { w[decltype(I)::value] = ld(I); });
+    }
 };
 
-// Raw populations of the node at (s.x, s.y, z) after streaming: slot i comes
-// from (x - c_x, y - c_y, z - c_z), periodic.  ysb / zsb: element offsets of
-// the source rows / planes for c = +1, 0, -1 (index 1 - c); xb: per-lane byte
-// offsets of the source columns, same indexing.
+// The 19 raw populations of one node as loaded, one register each: the loads
+// stay in flight until the values are used (packing them into Held at load
+// time would wait for every load at once).
+// (16-bit values widened to 32 bits: an array of 16-bit elements is kept as
+// packed pairs, and inserting a loaded value into a pair waits for it.)
+template <class S> struct Raw {
+    typedef typename std::conditional<sizeof(typename S::raw) == 2, uint32_t, typename S::raw>::type E;
+    E v[19];
+};
+
+// Marks the loaded values as used here (an empty asm that reads and rewrites
+// them, ordered after the preceding memory operations): the conversions and
+// the packing after it cannot be hoisted next to the loads, where they would
+// wait for them before the sweep has run.
+template <class S> LB_DEV void scr_pin(Raw<S> &r) {
+    auto &v = r.v;
+    __asm__ volatile("" : "+v"(v[0]), "+v"(v[1]), "+v"(v[2]), "+v"(v[3]), "+v"(v[4]), "+v"(v[5]), "+v"(v[6]),
+                          "+v"(v[7]), "+v"(v[8]), "+v"(v[9]), "+v"(v[10]), "+v"(v[11]), "+v"(v[12]), "+v"(v[13]),
+                          "+v"(v[14]), "+v"(v[15]), "+v"(v[16]), "+v"(v[17]), "+v"(v[18])
+                        :
+                        : "memory");
+}
+
+template <class S> LB_DEV void scr_pack(const Raw<S> &r, Held<S> &h) {
+    if constexpr (sizeof(typename S::raw) == 2) {
+        static_for<0, 9>([&](auto M) {
+            constexpr int m = decltype(M)::value;
+            h.w[m] = r.v[2 * m] | (r.v[2 * m + 1] << 16);
+        });
+        h.w[9] = r.v[18];
+    } else {
+        static_for<0, 19>([&](auto I) { h.w[decltype(I)::value] = r.v[decltype(I)::value]; });
+    }
+}
+
+// Loads the populations of the node at (x, y, z) after streaming into h: slot
+// i comes from (x - c_x, y - c_y, z - c_z), periodic.  ysb / zsb: element
+// offsets of the source rows / planes for c = +1, 0, -1 (index 1 - c); xb:
+// per-lane byte offsets of the source columns, same indexing.
 template <class S>
 LB_DEV void scr_pull(const StepArgs<S> &a, rsrc_t rs, const uint32_t ysb[3], const uint32_t zsb[3],
-                     const uint32_t xb[3], typename S::raw *raw) {
+                     const uint32_t xb[3], Held<S> &h) {
     typedef Vel<19> L;
-    typedef typename S::raw R;
-    constexpr uint32_t ES = sizeof(R);
+    constexpr uint32_t ES = sizeof(typename S::raw);
+    h.fill([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
+        const uint32_t so = ((uint32_t)i * a.ps + ysb[1 - cy] + zsb[1 - cz]) * ES;
+        if constexpr (ES == 2) return scr_ld16(rs, xb[1 - cx], so);
+        else return scr_ld32(rs, xb[1 - cx], so);
+    });
+}
+
+template <class S>
+LB_DEV void scr_pull_raw(const StepArgs<S> &a, rsrc_t rs, const uint32_t ysb[3], const uint32_t zsb[3],
+                         const uint32_t xb[3], Raw<S> &r) {
+    typedef Vel<19> L;
+    constexpr uint32_t ES = sizeof(typename S::raw);
     static_for<0, 19>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
         const uint32_t so = ((uint32_t)i * a.ps + ysb[1 - cy] + zsb[1 - cz]) * ES;
-        raw[i] = scr_ld<R>(rs, xb[1 - cx], so);
+        if constexpr (ES == 2) r.v[i] = scr_ld16(rs, xb[1 - cx], so);
+        else r.v[i] = scr_ld32(rs, xb[1 - cx], so);
+    });
+}
+
+// Populations in compute form, as pull_node converts them.
+template <class S> LB_DEV void scr_unpack(const Held<S> &h, typename S::T *f) {
+    static_for<0, 19>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        f[i] = S::ld(h.template get<i>(), weight<19, float>(Vel<19>::wc(i)));
     });
 }
 
@@ -91,13 +176,55 @@ template <class S> LB_DEV void scr_planes(const StepArgs<S> &a, int z, uint32_t 
     zsb[2] = (uint32_t)wrap1(z + 1, a.nz) * a.sz;
 }
 
-template <class S, bool MACROS, int NW, int SPW>
-__global__ __launch_bounds__(NW * 64, 4) void k_step_scr(const StepArgs<S> *__restrict__ pa, uint32_t dstep) {
+// psi of a node from its pulled populations (psi_of_node's arithmetic).
+template <class S> LB_DEV typename S::T scr_psi(const StepArgs<S> &a, const Raw<S> &r) {
+    typedef typename S::T T;
+    T f[19];
+    static_for<0, 19>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        f[i] = S::ld((typename S::raw)r.v[i], weight<19, float>(Vel<19>::wc(i)));
+    });
+    T m0, ux, uy, uz, rho;
+    if constexpr (S::DEV) {
+        moments_dev<19, T>(f, m0, ux, uy, uz);
+        rho = T(1) + m0;
+    } else {
+        moments<19, T>(f, m0, ux, uy, uz);
+        rho = m0;
+    }
+    return sc_psi<T>(rho, a.scrho0);
+}
+
+// A row segment: row r of the (BY + 2)-row window, lattice row y, the lane's
+// x, and the offsets of its source rows and columns.
+struct Seg {
+    int r, y, x;
+    bool on;        // the segment exists in this tile (uniform per wave)
+    bool lane;      // the lane's x lies in the lattice
+    uint32_t ysb[3], xb[3];
+};
+template <class S> LB_DEV void scr_seg(const StepArgs<S> &a, Seg &g, int r, int xs, int y0, bool on) {
+    g.r = r;
+    g.on = on;
+    g.y = wrap1(y0 - 1 + r, a.ny);
+    g.x = xs * 64 + (int)(threadIdx.x & 63);
+    g.lane = g.x < a.nx;
+    for (int c = 0; c < 3; c++) {
+        g.ysb[c] = (uint32_t)wrap1(g.y + c - 1, a.ny) * a.sy;                              // row y - c_y, c_y = 1 - c
+        g.xb[c] = (uint32_t)wrap1(g.x + c - 1, a.nx) * (uint32_t)sizeof(typename S::raw);   // column x - c_x
+    }
+}
+
+// Waves per SIMD each workgroup shape is compiled for: two workgroups of 6
+// waves, or one of 8, 12 or 16, per CU.
+constexpr int scr_min_waves(int nw) { return nw == 6 ? 3 : nw / 4; }
+
+template <class S, bool MACROS, int NW, int SPI>
+__global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const StepArgs<S> *__restrict__ pa, uint32_t dstep) {
     constexpr int Q = 19;
     typedef typename S::T T;
-    typedef typename S::raw R;
     typedef Vel<19> L;
-    constexpr uint32_t ES = sizeof(R);
+    constexpr uint32_t ES = sizeof(typename S::raw);
     extern __shared__ float scr_lds[];
     T *psl = (T *)scr_lds;
     const StepArgs<S> &a = *pa;
@@ -108,63 +235,68 @@ __global__ __launch_bounds__(NW * 64, 4) void k_step_scr(const StepArgs<S> *__re
     const int ty = b % a.nty, zch = b / a.nty;
     const int y0 = ty * BY, bye = min(BY, ny - y0);
     const int z0 = zch * a.zc, z1 = min(nz, z0 + a.zc);
-    const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
 
-    Seg sg[SPW];
-    uint32_t ysb[SPW][3], xb[SPW][3];
-    static_for<0, SPW>([&](auto J) {
+    Seg si[SPI], sf;
+    static_for<0, SPI>([&](auto J) {
         constexpr int j = decltype(J)::value;
         const int s = w + j * NW;
-        const int r = s / nseg, xs = s - r * nseg;
-        Seg &g = sg[j];
-        g.r = r;
-        g.active = r < bye + 2;
-        g.interior = r >= 1 && r <= bye;
-        g.y = wrap1(y0 - 1 + r, ny);
-        g.x = xs * 64 + lane;
-        g.lane = g.x < nx;
-        for (int c = 0; c < 3; c++) {
-            ysb[j][c] = (uint32_t)wrap1(g.y + c - 1, ny) * a.sy;      // source row y - c_y, c_y = 1 - c
-            xb[j][c] = (uint32_t)wrap1(g.x + c - 1, nx) * ES;         // source column x - c_x
-        }
+        scr_seg(a, si[j], 1 + s / nseg, s % nseg, y0, s < bye * nseg);
     });
+    scr_seg(a, sf, w < nseg ? 0 : bye + 1, w % nseg, y0, w < 2 * nseg);
     const rsrc_t rsa = make_rsrc(a.A, (uint64_t)Q * a.N * ES);
     const rsrc_t rsb = make_rsrc(a.B, (uint64_t)Q * a.N * ES);
     uint32_t step = 0;
     if (a.round_mode) step = ldm(&a.ctr[0]) + dstep;
 
-    for (int k = z0 - 1; k <= z1 + 1; k++) {
-        // A: loads of plane k for its psi
-        R dr[SPW][19];
-        const bool dens = k <= z1;
-        const int zk = wrap1(k, nz);
-        uint32_t zsk[3];
-        scr_planes(a, zk, zsk);
-        if (dens) {
-            static_for<0, SPW>([&](auto J) {
-                constexpr int j = decltype(J)::value;
-                if (sg[j].active && sg[j].lane) scr_pull<S>(a, rsa, ysb[j], zsk, xb[j], dr[j]);
+    Held<S> h1[SPI], h2[SPI];   // populations of planes k - 1 and k - 2 (interior segments)
+    Held<S> res[SPI];               // values to store of the plane being swept
+    float mac[SPI][4];
+    // B2: the stores of plane q
+    auto store_plane = [&](int q) {
+        const uint32_t qe = (uint32_t)q * a.sz;
+        static_for<0, SPI>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const Seg &g = si[j];
+            if (!(g.on && g.lane)) return;
+            const uint32_t vo = (uint32_t)g.x * ES;
+            const uint32_t rowe = (uint32_t)g.y * a.sy + qe;
+            static_for<0, Q>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                scr_st(res[j].template get<i>(), rsb, vo, ((uint32_t)i * a.ps + rowe) * ES);
             });
+            if (MACROS) {
+                const uint32_t id = (uint32_t)g.x + (uint32_t)nx * ((uint32_t)g.y + (uint32_t)ny * (uint32_t)q);
+                store_macros(a, id, mac[j][0], mac[j][1], mac[j][2], mac[j][3]);
+            }
+        });
+    };
+    for (int k = z0 - 1; k <= z1 + 1; k++) {
+        // A: loads of plane k for its psi (and, interior segments, its sweep)
+        Raw<S> di[SPI], df;
+        const bool dens = k <= z1;
+        uint32_t zsk[3];
+        scr_planes(a, wrap1(k, nz), zsk);
+        if (dens) {
+            static_for<0, SPI>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if (si[j].on && si[j].lane) scr_pull_raw<S>(a, rsa, si[j].ysb, zsk, si[j].xb, di[j]);
+            });
+            if (sf.on && sf.lane) scr_pull_raw<S>(a, rsa, sf.ysb, zsk, sf.xb, df);
         }
-        // B: sweep plane q = k - 2 with the psi of planes q - 1, q, q + 1
+        // B1: sweep plane q = k - 2 with the psi of planes q - 1, q, q + 1;
+        // the values to store wait in registers for B2
         const int q = k - 2;
-        if (q >= z0) {
+        const bool sw = q >= z0;
+        if (sw) {
             const T *pzm = psl + ((q + 3) & 3) * slot, *pz0 = psl + (q & 3) * slot,
                     *pzp = psl + ((q + 1) & 3) * slot;
-            uint32_t zsq[3];
-            scr_planes(a, q, zsq);
-            static_for<0, SPW>([&](auto J) {
+            static_for<0, SPI>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                const Seg &g = sg[j];
-                if (!(g.interior && g.lane)) return;
-                R rr[19];
-                scr_pull<S>(a, rsa, ysb[j], zsq, xb[j], rr);
+                const Seg &g = si[j];
+                if (!(g.on && g.lane)) return;
                 T f[19];
-                static_for<0, 19>([&](auto I) {
-                    constexpr int i = decltype(I)::value;
-                    f[i] = S::ld(rr[i], weight<Q, float>(L::wc(i)));
-                });
+                scr_unpack<S>(h2[j], f);
                 T m0, ux, uy, uz, rho, drho = T(0);
                 if constexpr (S::DEV) {
                     moments_dev<Q, T>(f, m0, ux, uy, uz);
@@ -196,19 +328,16 @@ __global__ __launch_bounds__(NW * 64, 4) void k_step_scr(const StepArgs<S> *__re
                 ez = ez + (gp * IF[2]) / (ti * rho);
                 if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
                 else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
+                if (MACROS) { mac[j][0] = (float)rho; mac[j][1] = (float)ux; mac[j][2] = (float)uy; mac[j][3] = (float)uz; }
                 NodePos p;
                 p.x = g.x; p.y = g.y; p.z = q;
                 p.id = (uint32_t)g.x + (uint32_t)nx * ((uint32_t)g.y + (uint32_t)ny * (uint32_t)q);
                 p.step = step;
                 p.seed = a.round_mode ? sr_seed(step, p.id) : 0u;
-                if (MACROS) store_macros(a, p.id, (float)rho, (float)ux, (float)uy, (float)uz);
-                const uint32_t vo = (uint32_t)g.x * ES;
-                const uint32_t rowe = (uint32_t)g.y * a.sy + (uint32_t)q * a.sz;
                 auto put = [&](auto sr) {
-                    static_for<0, Q>([&](auto I) {
+                    res[j].fill([&](auto I) {
                         constexpr int i = decltype(I)::value;
-                        auto v = store_val_m<S, decltype(sr)::value>(p, f[i], weight<Q, float>(L::wc(i)), i);
-                        scr_st(v, rsb, vo, ((uint32_t)i * a.ps + rowe) * ES);
+                        return store_val_m<S, decltype(sr)::value>(p, f[i], weight<Q, float>(L::wc(i)), i);
                     });
                 };
                 if constexpr (kRoundable<S>) {
@@ -219,68 +348,107 @@ __global__ __launch_bounds__(NW * 64, 4) void k_step_scr(const StepArgs<S> *__re
                 }
             });
         }
-        // C: psi of plane k into slot k & 3 (and the row's padding elements)
+        // C: psi of plane k into slot k & 3 (and the rows' padding elements);
+        // the interior populations packed for their sweep two iterations on
         if (dens) {
-            T *ps = psl + ((k + 4) & 3) * slot;
-            static_for<0, SPW>([&](auto J) {
+            static_for<0, SPI>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                const Seg &g = sg[j];
-                if (!(g.active && g.lane)) return;
-                T f[19];
-                static_for<0, 19>([&](auto I) {
-                    constexpr int i = decltype(I)::value;
-                    f[i] = S::ld(dr[j][i], weight<Q, float>(L::wc(i)));
-                });
-                T m0, ux, uy, uz, rho;
-                if constexpr (S::DEV) {
-                    moments_dev<Q, T>(f, m0, ux, uy, uz);
-                    rho = T(1) + m0;
-                } else {
-                    moments<Q, T>(f, m0, ux, uy, uz);
-                    rho = m0;
-                }
-                const T v = sc_psi<T>(rho, a.scrho0);
+                if (si[j].on && si[j].lane) scr_pin<S>(di[j]);
+            });
+            if (sf.on && sf.lane) scr_pin<S>(df);
+            T *ps = psl + ((k + 4) & 3) * slot;
+            auto put_psi = [&](const Seg &g, const Raw<S> &r) {
+                const T v = scr_psi<S>(a, r);
                 const int row = g.r * W;
                 ps[row + g.x + 1] = v;
                 if (g.x == nx - 1) ps[row] = v;
                 if (g.x == 0) ps[row + nx + 1] = v;
+            };
+            static_for<0, SPI>([&](auto J) {
+                constexpr int j = decltype(J)::value;
+                if (si[j].on && si[j].lane) put_psi(si[j], di[j]);
             });
+            if (sf.on && sf.lane) put_psi(sf, df);
         }
+        // B2: the stores of plane q, issued after C has used the loads of A
+        if (sw) store_plane(q);
+        static_for<0, SPI>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            h2[j] = h1[j];
+            if (dens) scr_pack<S>(di[j], h1[j]);
+        });
         __syncthreads();
     }
 }
 
-template <class S, int NW, int SPW>
+template <class S, int NW, int SPI>
 int launch_v(bool macros, unsigned nb, size_t lds, hipStream_t st, const StepArgs<S> *a, uint32_t dstep) {
-    if (macros) hipLaunchKernelGGL((k_step_scr<S, true, NW, SPW>), dim3(nb), dim3(NW * 64), lds, st, a, dstep);
-    else hipLaunchKernelGGL((k_step_scr<S, false, NW, SPW>), dim3(nb), dim3(NW * 64), lds, st, a, dstep);
+    if (macros) hipLaunchKernelGGL((k_step_scr<S, true, NW, SPI>), dim3(nb), dim3(NW * 64), lds, st, a, dstep);
+    else hipLaunchKernelGGL((k_step_scr<S, false, NW, SPI>), dim3(nb), dim3(NW * 64), lds, st, a, dstep);
     return (int)hipGetLastError();
 }
 
 template <class S>
-int launch_s(bool macros, int variant, unsigned nb, size_t lds, hipStream_t st, const void *pa, uint32_t dstep) {
+int launch_s(bool macros, int nw, int spi, unsigned nb, size_t lds, hipStream_t st, const void *pa, uint32_t dstep) {
     const StepArgs<S> *a = (const StepArgs<S> *)pa;
-    switch (variant) {
-    case 0: return launch_v<S, 8, 3>(macros, nb, lds, st, a, dstep);
-    case 1: return launch_v<S, 12, 2>(macros, nb, lds, st, a, dstep);
-    case 2: return launch_v<S, 16, 2>(macros, nb, lds, st, a, dstep);
-    default: return LB_ERR_UNSUPPORTED;
+    switch (nw * 4 + spi) {
+    case 6 * 4 + 1: return launch_v<S, 6, 1>(macros, nb, lds, st, a, dstep);
+    case 8 * 4 + 1: return launch_v<S, 8, 1>(macros, nb, lds, st, a, dstep);
+    case 8 * 4 + 2: return launch_v<S, 8, 2>(macros, nb, lds, st, a, dstep);
+    case 12 * 4 + 1: return launch_v<S, 12, 1>(macros, nb, lds, st, a, dstep);
+    case 16 * 4 + 1: return launch_v<S, 16, 1>(macros, nb, lds, st, a, dstep);
+    default: break;
     }
+    if constexpr (std::is_same<S, StBF16>::value) {
+        switch (nw * 4 + spi) {
+        case 6 * 4 + 2: return launch_v<S, 6, 2>(macros, nb, lds, st, a, dstep);
+        case 8 * 4 + 3: return launch_v<S, 8, 3>(macros, nb, lds, st, a, dstep);
+        case 12 * 4 + 2: return launch_v<S, 12, 2>(macros, nb, lds, st, a, dstep);
+        default: break;
+        }
+    }
+    return LB_ERR_UNSUPPORTED;
 }
 
 }  // namespace
 
-int scr_variant_waves(int variant) { return variant == 0 ? 8 : (variant == 1 ? 12 : (variant == 2 ? 16 : 0)); }
-int scr_variant_spw(int variant) { return variant == 0 ? 3 : (variant == 1 ? 2 : (variant == 2 ? 2 : 0)); }
+// Interior segments per wave a workgroup of nw waves holds: three planes of
+// populations per segment live in registers (one value per register as
+// loaded, the two older planes of the 16-bit storages packed two per
+// register), and each instance fits the register budget of scr_min_waves(nw)
+// waves per SIMD without spilling (checked at build time with
+// -Rpass-analysis=kernel-resource-usage).
+int scr_max_spi(int nw, int storage) {
+    const bool bf = storage == LB_STORE_BF16;
+    if (nw == 6) return bf ? 2 : 1;
+    if (nw == 8) return bf ? 3 : 2;
+    if (nw == 12) return bf ? 2 : 1;
+    if (nw == 16) return 1;
+    return 0;
+}
+
+// Largest tile (rows) for nw waves: the interior segments fit nw x the
+// segments per wave, the two frame rows one segment per wave, the four psi
+// planes 64 KB of LDS.
+int scr_max_rows(int nw, int nx, int storage) {
+    const int nseg = (nx + 63) / 64;
+    if (2 * nseg > nw) return 0;
+    int by = nw * scr_max_spi(nw, storage) / nseg;
+    while (by >= 1 && scr_lds_bytes(nx, by) > 65536) by--;
+    return by;
+}
 
 size_t scr_lds_bytes(int nx, int by) { return (size_t)4 * (by + 2) * (nx + 2) * sizeof(float); }
 
-int launch_step_scr(int storage, bool macros, int variant, unsigned nblocks, size_t lds, hipStream_t stream,
+int launch_step_scr(int storage, bool macros, int nw, int by, int nx, unsigned nblocks, hipStream_t stream,
                     const void *args, uint32_t dstep) {
+    const int nseg = (nx + 63) / 64;
+    const int spi = (by * nseg + nw - 1) / nw;
+    const size_t lds = scr_lds_bytes(nx, by);
     switch (storage) {
-    case LB_STORE_F32: return launch_s<StF32>(macros, variant, nblocks, lds, stream, args, dstep);
-    case LB_STORE_F16: return launch_s<StF16>(macros, variant, nblocks, lds, stream, args, dstep);
-    case LB_STORE_BF16: return launch_s<StBF16>(macros, variant, nblocks, lds, stream, args, dstep);
+    case LB_STORE_F32: return launch_s<StF32>(macros, nw, spi, nblocks, lds, stream, args, dstep);
+    case LB_STORE_F16: return launch_s<StF16>(macros, nw, spi, nblocks, lds, stream, args, dstep);
+    case LB_STORE_BF16: return launch_s<StBF16>(macros, nw, spi, nblocks, lds, stream, args, dstep);
     default: return LB_ERR_UNSUPPORTED;
     }
 }

@@ -23,6 +23,10 @@
  * How each function is pinned (tests/test_oracle_*.py):
  *   orc_exp                 vs math.exp on 20k points (<= 2 ulp) and exp(0)=1,
  *                           exp(ln2)=2 exactly-to-rounding.
+ *   orc_expf                single-precision exp of the FP32 Shan-Chen psi:
+ *                           vs math.exp rounded to float on 30k points over
+ *                           [-87, 88] (<= 2 ulp), exp(0)=1, 0 / inf / NaN edges;
+ *                           psi in single vs double precision within 2 ulp.
  *   orc_splitmix64_next     published SplitMix64 outputs for seed 0.
  *   orc_f32_to_f16          numpy astype(float16) (round-to-nearest-even) on
  *                           normals, subnormals, ties, overflow.
@@ -135,6 +139,34 @@ double exp_cw(double x) {
     return p * scale;
 }
 
+/* ------------------------------------------------------------------------
+ * exp in single precision: Cody-Waite reduction with ln2 split into a 15-bit
+ * head (0x1.62e4p-1) and a tail (0x1.7f7d1cp-20), degree-7 Taylor polynomial
+ * in Horner form with fused multiply-adds on |r| <= ln2/2, scaling by 2^k
+ * through the exponent field; 0 below -87, +inf above 88, NaN kept.  The GPU
+ * path evaluates the same IEEE single-precision operations.
+ * ---------------------------------------------------------------------- */
+float exp_f(float x) {
+    const float xc = x < -87.0f ? -87.0f : (x > 88.0f ? 88.0f : x);
+    const float kf = std::nearbyintf(xc * 0x1.715476p+0f);
+    float r = std::fmaf(-kf, 0x1.62e4p-1f, xc);
+    r = std::fmaf(-kf, 0x1.7f7d1cp-20f, r);
+    float p = 1.0f / 5040.0f;
+    p = std::fmaf(p, r, 1.0f / 720.0f);
+    p = std::fmaf(p, r, 1.0f / 120.0f);
+    p = std::fmaf(p, r, 1.0f / 24.0f);
+    p = std::fmaf(p, r, 1.0f / 6.0f);
+    p = std::fmaf(p, r, 0.5f);
+    p = std::fmaf(p, r, 1.0f);
+    p = std::fmaf(p, r, 1.0f);
+    const uint32_t sb = (uint32_t)((int)kf + 127) << 23;
+    float sc;
+    std::memcpy(&sc, &sb, 4);
+    const float v = p * sc;
+    const float w = x < -87.0f ? 0.0f : (x > 88.0f ? HUGE_VALF : v);
+    return x != x ? x : w;
+}
+
 inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
@@ -181,18 +213,21 @@ float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
 /* ------------------------------------------------------------------------
  * Stochastic rounding of the 16-bit stores (round_mode 1).  Random bits:
  * one SplitMix64 finaliser per (step, node), then Wellons' lowbias32 per
- * population slot -- a counter-based stream, so every run, thread count and
- * device produces the same bits.  v rounds up (in magnitude) to the next
- * representable value with probability equal to its distance from the lower
- * one over the spacing: the expected stored value is v, so increments below
- * half a spacing are not lost (nearest-even rounding drops them every step).
+ * pair of population slots (high 16 bits for the even slot, low 16 for the
+ * odd one, placed in the top half of the word) -- a counter-based stream, so
+ * every run, thread count and device produces the same bits.  v rounds up
+ * (in magnitude) to the next representable value with probability equal to
+ * its distance from the lower one over the spacing (for FP16 subnormals
+ * quantised to 2^-16 by the 16 random bits): the expected stored value is v,
+ * so increments below half a spacing are not lost (nearest-even rounding
+ * drops them every step).
  * ---------------------------------------------------------------------- */
 uint32_t sr_bits(uint32_t step, uint32_t id, int slot) {
     uint64_t z = 0x5EEDull + (uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)id;
     z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
-    uint32_t x = (uint32_t)z + (uint32_t)slot * 0x9E3779B9u;
+    uint32_t x = (uint32_t)z + (uint32_t)(slot >> 1) * 0x9E3779B9u;
     x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
-    return x;
+    return (slot & 1) ? x << 16 : x & 0xffff0000u;   /* 16 bits per slot, two slots per hash */
 }
 /* BF16: add 16 random bits below the kept ones, truncate. */
 uint16_t f2bf_sr(float f, uint32_t r) {
@@ -682,10 +717,16 @@ typename S::T node_density(const orc_cfg *cfg, const Geo &g, const typename S::r
     return rho;
 }
 
-/* psi = rho0 * (1 - exp(-rho/rho0)) evaluated in double, rounded to T. */
-template <class T> inline T sc_psi(T rho, float rho0) {
-    double r = (double)rho / (double)rho0;
-    return (T)((double)rho0 * (1.0 - exp_cw(-r)));
+/* psi = rho0 * (1 - exp(-rho/rho0)): single precision (exp_f) for FP32
+ * collisions, double (exp_cw) for FP64 ones. */
+template <class T> inline T sc_psi(T rho, float rho0);
+template <> inline float sc_psi<float>(float rho, float rho0) {
+    if (rho0 == 1.0f) return 1.0f - exp_f(-rho);
+    return rho0 * (1.0f - exp_f(-(rho / rho0)));
+}
+template <> inline double sc_psi<double>(double rho, float rho0) {
+    double r = rho / (double)rho0;
+    return (double)rho0 * (1.0 - exp_cw(-r));
 }
 
 template <int Q, class S>
@@ -909,6 +950,9 @@ void reduce_impl(const orc_cfg *cfg, const typename S::raw *A, const uint8_t *so
 extern "C" {
 
 double orc_exp(double x) { return exp_cw(x); }
+float orc_expf(float x) { return exp_f(x); }
+float orc_psi_f32(float rho, float rho0) { return sc_psi<float>(rho, rho0); }
+double orc_psi_f64(double rho, float rho0) { return sc_psi<double>(rho, rho0); }
 
 /* SplitMix64 (Steele, Lea, Flood 2014): state += golden gamma, then the
  * variant-13 finaliser. */

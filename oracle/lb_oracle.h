@@ -66,6 +66,9 @@ typedef struct {
 
 /* ---- shared numerics -------------------------------------------------- */
 double   orc_exp(double x);
+float    orc_expf(float x);
+float    orc_psi_f32(float rho, float rho0);
+double   orc_psi_f64(double rho, float rho0);
 uint64_t orc_splitmix64_next(uint64_t *state);
 void     orc_uniform_noise(uint64_t seed, int64_t n, double base, double amp, float *out);
 uint16_t orc_f32_to_f16(float f);

@@ -59,6 +59,9 @@ i_ = C.c_int
 f_ = C.c_float
 _PROTOS = {
     "orc_exp": (C.c_double, [C.c_double]),
+    "orc_expf": (C.c_float, [C.c_float]),
+    "orc_psi_f32": (C.c_float, [C.c_float, C.c_float]),
+    "orc_psi_f64": (C.c_double, [C.c_double, C.c_float]),
     "orc_splitmix64_next": (C.c_uint64, [C.POINTER(C.c_uint64)]),
     "orc_uniform_noise": (None, [C.c_uint64, C.c_int64, C.c_double, C.c_double, vp]),
     "orc_f32_to_f16": (C.c_uint16, [f_]),

@@ -514,14 +514,21 @@ def _periodic_box(shape, storage, round_mode, force):
     return cfg, solid, dict(rho=rho, ux=0.0, uy=0.0, uz=0.0)
 
 
+ROW_SLAB_SHAPES = [(384, 2, 5), (512, 5, 3), (768, 4, 8), (1024, 5, 16), (512, 1, 1), (768, 2, 3)]
+# tiles holding two or three interior segments per wave (BF16 only)
+ROW_SLAB_BF16_SHAPES = [("cube24_bf16_sr", 512, 20, 4), ("cube24_bf16_sr", 384, 9, 2),
+                        ("thin64x7x1_bf16_nearest", 768, 20, 6), ("thin64x7x1_bf16_nearest", 384, 12, 1)]
+
+
 @pytest.mark.parametrize("layout", ["soa", "rows"])
-@pytest.mark.parametrize("threads,by,zc", [(512, 6, 5), (768, 3, 8), (1024, 8, 16), (512, 1, 1), (768, 2, 3)])
-@pytest.mark.parametrize("case", sorted(ROW_SLAB_CASES))
+@pytest.mark.parametrize("case,threads,by,zc",
+                         [(c,) + t for c in sorted(ROW_SLAB_CASES) for t in ROW_SLAB_SHAPES] + ROW_SLAB_BF16_SHAPES)
 def test_row_slab_shan_chen_sweep_matches_oracle(gpu, case, threads, by, zc, layout):
     """The row-slab one-pass sweep (whole x-rows per workgroup, the frame rows
     spread over the waves, psi planes in LDS) against the oracle's density
     pass + sweep: rows of 24 to 130 nodes (one to three 64-node segments, the
-    last one partial), tiles of 1 to 8 rows that do not divide ny, z-chunks
+    last one partial), workgroups of 6 to 16 waves with one to three interior
+    segments per wave, tiles of 1 to 20 rows that do not divide ny, z-chunks
     of 1 to 16 planes that do not divide nz, a single z-plane, a body force,
     FP32 and 16-bit storage with both roundings, both layouts, several
     graph-replayed launches."""
@@ -566,7 +573,7 @@ def test_row_slab_sweep_applies_only_to_periodic_boxes(gpu):
         with pytest.raises(RuntimeError):
             gpu.Solver(cfg, solid)
     cfg, solid, init = scenes.spinodal(n=16)
-    cfg.update(sc_sweep=3, block_x=768, block_y=4, sc_zchunk=4)
+    cfg.update(sc_sweep=3, block_x=768, block_y=2, sc_zchunk=4)
     sol = gpu.Solver(cfg, solid)
     assert sol.launch_config["sc_sweep"] == 3
     sol.close()

@@ -29,6 +29,41 @@ def test_exp_matches_libm_within_2ulp():
     assert O.lib.orc_exp(-750.0) == 0.0
 
 
+def f32_ulp_diff(a, b):
+    ia = int(np.float32(a).view(np.int32))
+    ib = int(np.float32(b).view(np.int32))
+    return abs(ia - ib)
+
+
+def test_expf_matches_libm_within_2ulp():
+    """The single-precision exp of the FP32 Shan-Chen psi against libm's
+    double exp rounded to float: a dense grid over [-87, 88] (which covers
+    every reduction interval k), random points, the psi range [-10, 0] and
+    the edges; 0 below -87, +inf above 88, NaN kept."""
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([np.linspace(-87.0, 88.0, 20001), rng.uniform(-87, 88, 5000), rng.uniform(-10, 0, 5000),
+                         [0.0, -0.0, 1.0, -1.0, math.log(2.0), -math.log(2.0) / 2, 1e-8, -1e-8]]).astype(np.float32)
+    worst = max(f32_ulp_diff(O.lib.orc_expf(float(x)), np.float32(math.exp(float(x)))) for x in xs)
+    assert worst <= 2, worst
+    assert O.lib.orc_expf(0.0) == 1.0
+    assert O.lib.orc_expf(-87.5) == 0.0 and O.lib.orc_expf(-1e30) == 0.0
+    assert O.lib.orc_expf(88.5) == math.inf
+    assert math.isnan(O.lib.orc_expf(math.nan))
+
+
+def test_psi_single_precision_matches_double():
+    """psi = rho0 (1 - exp(-rho / rho0)) in single precision against the
+    double evaluation: within 2 ulp of 1 over rho in [0, 20] for rho0 = 1
+    and 2.5; psi(0) = 0 and psi -> rho0 for large rho."""
+    rng = np.random.default_rng(2)
+    rhos = np.concatenate([np.linspace(0.0, 20.0, 4001), rng.uniform(0.0, 3.0, 3000)]).astype(np.float32)
+    for rho0 in (1.0, 2.5):
+        worst = max(abs(O.lib.orc_psi_f32(float(r), rho0) - O.lib.orc_psi_f64(float(r), rho0)) for r in rhos)
+        assert worst <= 2 * 2.0 ** -23 * rho0, (rho0, worst)
+        assert O.lib.orc_psi_f32(0.0, rho0) == 0.0
+        assert O.lib.orc_psi_f32(200.0, rho0) == np.float32(rho0)
+
+
 def test_splitmix64_published_outputs():
     # SplitMix64 with state 0 (Vigna's reference splitmix64.c): first outputs
     st = C.c_uint64(0)
@@ -142,9 +177,16 @@ def test_f16_stochastic_rounding_is_exactly_unbiased():
 
 
 def test_sr_bits_uniform_and_counter_based():
+    """16 random bits per population slot, in the top half of the word; slots
+    2m and 2m + 1 share one 32-bit hash of (node word, m), high half to the
+    even slot."""
     a = np.array([O.lib.orc_sr_bits(s, i, q) for s in range(4) for i in range(1024) for q in range(4)], np.uint64)
-    assert len(np.unique(a)) == a.size                      # no repeats in 16k draws of 32 bits
-    h = np.bincount((a >> np.uint64(24)).astype(np.int64), minlength=256)
-    chi2 = float(((h - a.size / 256) ** 2 / (a.size / 256)).sum())
-    assert chi2 < 340, chi2                                  # 255 dof, p ~ 0.0003 threshold
+    assert int((a & np.uint64(0xFFFF)).max()) == 0
+    words = a[0::2] | (a[1::2] >> np.uint64(16))             # the hash of each (step, node, pair)
+    assert len(np.unique(words)) == words.size              # no repeats in 8k draws of 32 bits
+    for bits in (a >> np.uint64(24), (a >> np.uint64(16)) & np.uint64(0xFF)):   # high and low byte of each draw
+        h = np.bincount(bits.astype(np.int64), minlength=256)
+        chi2 = float(((h - a.size / 256) ** 2 / (a.size / 256)).sum())
+        assert chi2 < 340, chi2                              # 255 dof, p ~ 0.0003 threshold
     assert O.lib.orc_sr_bits(7, 11, 3) == O.lib.orc_sr_bits(7, 11, 3)
+    assert O.lib.orc_sr_bits(7, 11, 2) != O.lib.orc_sr_bits(7, 11, 3)

@@ -20,7 +20,7 @@ import pytest
 C19 = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (1, -1, 0),
        (1, 0, 1), (1, 0, -1), (-1, 1, 0), (-1, -1, 0), (-1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, 1, -1),
        (0, -1, 1), (0, -1, -1)]
-VARIANTS = {512: (8, 3), 768: (12, 2), 1024: (16, 2)}   # threads: (waves, segments per wave)
+WAVES = (6, 8, 12, 16)
 
 
 def wrap1(v, n):
@@ -34,17 +34,21 @@ def layout(kind, nx, ny, nz):
     return N, nx, nx * ny
 
 
-def max_rows(threads, nx):
-    nw, spw = VARIANTS[threads]
+def max_spi(nw, bf16):
+    return {6: 2 if bf16 else 1, 8: 3 if bf16 else 2, 12: 2 if bf16 else 1, 16: 1}[nw]
+
+
+def max_rows(nw, nx, bf16=True):
     nseg = (nx + 63) // 64
-    by = nw * spw // nseg - 2
+    if 2 * nseg > nw:
+        return 0
+    by = nw * max_spi(nw, bf16) // nseg
     while by >= 1 and 4 * (by + 2) * (nx + 2) * 4 > 65536:
         by -= 1
     return by
 
 
-def emulate(nx, ny, nz, threads, BY, zc, kind):
-    nw, spw = VARIANTS[threads]
+def emulate(nx, ny, nz, nw, BY, zc, kind):
     ps, sy, sz = layout(kind, nx, ny, nz)
     N = nx * ny * nz
     nseg = (nx + 63) // 64
@@ -63,14 +67,20 @@ def emulate(nx, ny, nz, threads, BY, zc, kind):
         # LDS: (node id, plane k) written at each position of the four slots
         lds_node = np.full(4 * slot, -1, np.int64)
         lds_k = np.full(4 * slot, -99, np.int64)
+        # wave w: interior segments s = w + j nw (row 1 + s / nseg), frame
+        # segment f = w (row 0 or bye + 1)
+        spi = -(-BY * nseg // nw)
         segs = []
+
+        def seg(r, xs, on, interior):
+            x = xs * 64 + lane
+            return dict(r=r, active=on, interior=interior and on, y=int(wrap1(np.array(y0 - 1 + r), ny)), x=x,
+                        ok=x < nx)
         for w in range(nw):
-            for j in range(spw):
+            for j in range(spi):
                 s = w + j * nw
-                r, xs = s // nseg, s % nseg
-                x = xs * 64 + lane
-                segs.append(dict(r=r, active=r < bye + 2, interior=1 <= r <= bye,
-                                 y=int(wrap1(np.array(y0 - 1 + r), ny)), x=x, ok=x < nx))
+                segs.append(seg(1 + s // nseg, s % nseg, s < bye * nseg, True))
+            segs.append(seg(0 if w < nseg else bye + 1, w % nseg, w < 2 * nseg, False))
         for k in range(z0 - 1, z1 + 2):
             zk = int(wrap1(np.array(k), nz))
 
@@ -116,26 +126,26 @@ def emulate(nx, ny, nz, threads, BY, zc, kind):
     assert (stored == 1).all(), (stored.min(), stored.max())
 
 
-# (shape, workgroup threads, rows per tile (0: the largest), z-chunk): one to
-# eight 64-node segments per row (partial last segments), tiles and chunks
+# (shape, waves per workgroup, rows per tile (0: the largest), z-chunk): one
+# to four 64-node segments per row (partial last segments), tiles and chunks
 # that do not divide the lattice, a single plane, one-row tiles
-CASES = [((24, 24, 24), 512, 0, 5), ((24, 24, 24), 1024, 1, 1), ((70, 13, 11), 768, 3, 4),
-         ((70, 13, 11), 1024, 0, 16), ((130, 5, 3), 512, 0, 2), ((130, 5, 3), 768, 2, 1),
-         ((64, 7, 1), 512, 4, 3), ((200, 3, 4), 1024, 0, 3), ((512, 2, 3), 1024, 0, 2)]
+CASES = [((24, 24, 24), 8, 0, 5), ((24, 24, 24), 16, 1, 1), ((70, 13, 11), 12, 3, 4),
+         ((70, 13, 11), 16, 0, 16), ((130, 5, 3), 8, 0, 2), ((130, 5, 3), 12, 2, 1),
+         ((64, 7, 1), 8, 4, 3), ((192, 14, 5), 6, 6, 4), ((130, 9, 3), 6, 0, 2), ((200, 3, 4), 16, 0, 3), ((256, 2, 3), 8, 0, 2), ((256, 9, 3), 12, 5, 2)]
 
 
 @pytest.mark.parametrize("kind", ["soa", "rows"])
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_row_slab_indexing(case, kind):
-    (nx, ny, nz), threads, by, zc = CASES[case]
-    emulate(nx, ny, nz, threads, by or max_rows(threads, nx), zc, kind)
+    (nx, ny, nz), nw, by, zc = CASES[case]
+    emulate(nx, ny, nz, nw, by or max_rows(nw, nx), zc, kind)
 
 
-@pytest.mark.parametrize("threads", sorted(VARIANTS))
-def test_row_slab_c3_launch_shapes(threads):
-    """Rows of 192 nodes (the 192^3 spinodal box) at each variant's largest
+@pytest.mark.parametrize("nw", WAVES)
+def test_row_slab_c3_launch_shapes(nw):
+    """Rows of 192 nodes (the 192^3 spinodal box) at each workgroup's largest
     tile and two z-chunks, on a lattice cut to three row tiles and two
     z-chunks plus one plane (every tile and chunk runs the same march)."""
-    by = max_rows(threads, 192)
+    by = max_rows(nw, 192)
     for zc in (6, 12):
-        emulate(192, 2 * by + 3, 2 * zc + 1, threads, by, zc, "rows")
+        emulate(192, 2 * by + 3, 2 * zc + 1, nw, by, zc, "rows")
