@@ -128,12 +128,13 @@ LB_DEV uint16_t f32_to_f16(float f) {
 LB_DEV float f16_to_f32(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
 
 // Stochastic rounding of 16-bit stores.  Random bits per (step, node, slot):
-// the SplitMix64 finaliser of step * golden + node, then Wellons' lowbias32
-// of its low word plus (slot / 2) * 0x9E3779B9, whose high 16 bits serve
-// the even slot and low 16 bits the odd one (a counter-based stream: the same
-// bits on any device, launch shape or thread count; one hash per two
-// populations).  The 16 bits sit in the top half of the word the rounding
-// functions read.  The value rounds up in magnitude with probability
+// the SplitMix64 finaliser of step * golden + node gives the node's word;
+// Marsaglia's xorshift32 (13, 17, 5) run from that word with its low bit set
+// gives word m + 1 for slots 2m and 2m + 1, high 16 bits to the even slot,
+// low 16 bits to the odd one (a counter-based stream: the same bits on any
+// device, launch shape or thread count; shifts and exclusive ors only, ten
+// words per node, no multiply).  The 16 bits sit in the top half of the word
+// the rounding functions read.  The value rounds up in magnitude with probability
 // (distance to the lower neighbour) / spacing, so the stored value is
 // unbiased (exactly for BF16 and FP16 normals, whose dropped fractions have
 // 16 and 13 bits; FP16 subnormals round up with that probability quantised
@@ -145,9 +146,13 @@ LB_DEV uint32_t sr_seed(uint32_t step, uint32_t id) {
     z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
     return (uint32_t)z;
 }
+// (slot is a compile-time constant at every call: the chains of the 19
+// slots share their common prefix.)
 LB_DEV uint32_t sr_mix(uint32_t seed, int slot) {
-    uint32_t x = seed + (uint32_t)(slot >> 1) * 0x9E3779B9u;
-    x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
+    uint32_t x = seed | 1u;
+    for (int m = 0; m <= (slot >> 1); m++) {
+        x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+    }
     return (slot & 1) ? x << 16 : x & 0xffff0000u;
 }
 LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) { return sr_mix(sr_seed(step, id), slot); }

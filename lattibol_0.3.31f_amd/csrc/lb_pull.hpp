@@ -572,9 +572,10 @@ LB_DEV void step_body(const StepArgs<S> &a, const NodePos &p) {
                 if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
             });
             T gp = (T)(-a.scG) * (T)ldm(&a.psi[p.id]);
-            ex = ex + (gp * IF[0]) / (ti * rho);
-            ey = ey + (gp * IF[1]) / (ti * rho);
-            if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+            const T kf = gp / (ti * rho);   // one division for the three components
+            ex = fma_t(kf, IF[0], ex);
+            ey = fma_t(kf, IF[1], ey);
+            if constexpr (Q == 19) ez = fma_t(kf, IF[2], ez);
         }
         if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
         else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
@@ -739,9 +740,10 @@ LB_DEV void zm_sweep(const StepArgs<S> &a, const NodePos &p, ZState<Q, S> &st, c
             if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
         });
         T gp = (T)(-a.scG) * pz0[c];
-        ex = ex + (gp * IF[0]) / (ti * rho);
-        ey = ey + (gp * IF[1]) / (ti * rho);
-        if constexpr (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+        const T kf = gp / (ti * rho);
+        ex = fma_t(kf, IF[0], ex);
+        ey = fma_t(kf, IF[1], ey);
+        if constexpr (Q == 19) ez = fma_t(kf, IF[2], ez);
         if constexpr (S::DEV) collide_dev<Q, T>(st.f, drho, ex, ey, ez, ti, a.rest_closure != 0);
         else collide<Q, T>(st.f, rho, ex, ey, ez, ti, a.rest_closure != 0);
     }

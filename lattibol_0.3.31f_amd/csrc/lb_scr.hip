@@ -323,9 +323,10 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
                     if constexpr (cz > 0) IF[2] = IF[2] + wp; else if constexpr (cz < 0) IF[2] = IF[2] - wp;
                 });
                 T gp = (T)(-a.scG) * pz0[c];
-                ex = ex + (gp * IF[0]) / (ti * rho);
-                ey = ey + (gp * IF[1]) / (ti * rho);
-                ez = ez + (gp * IF[2]) / (ti * rho);
+                const T kf = gp / (ti * rho);
+                ex = fma_t(kf, IF[0], ex);
+                ey = fma_t(kf, IF[1], ey);
+                ez = fma_t(kf, IF[2], ez);
                 if constexpr (S::DEV) collide_dev<Q, T>(f, drho, ex, ey, ez, ti, a.rest_closure != 0);
                 else collide<Q, T>(f, rho, ex, ey, ez, ti, a.rest_closure != 0);
                 if (MACROS) { mac[j][0] = (float)rho; mac[j][1] = (float)ux; mac[j][2] = (float)uy; mac[j][3] = (float)uz; }

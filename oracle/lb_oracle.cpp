@@ -212,9 +212,10 @@ float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
 
 /* ------------------------------------------------------------------------
  * Stochastic rounding of the 16-bit stores (round_mode 1).  Random bits:
- * one SplitMix64 finaliser per (step, node), then Wellons' lowbias32 per
- * pair of population slots (high 16 bits for the even slot, low 16 for the
- * odd one, placed in the top half of the word) -- a counter-based stream, so
+ * one SplitMix64 finaliser per (step, node), then a xorshift32 (13, 17, 5)
+ * sequence from it with the low bit set, one word per pair of population
+ * slots (high 16 bits for the even slot, low 16 for the odd one, placed in
+ * the top half of the word) -- a counter-based stream, so
  * every run, thread count and device produces the same bits.  v rounds up
  * (in magnitude) to the next representable value with probability equal to
  * its distance from the lower one over the spacing (for FP16 subnormals
@@ -225,9 +226,9 @@ float bf2f(uint16_t h) { return bitsf((uint32_t)h << 16); }
 uint32_t sr_bits(uint32_t step, uint32_t id, int slot) {
     uint64_t z = 0x5EEDull + (uint64_t)step * 0x9E3779B97F4A7C15ull + (uint64_t)id;
     z ^= z >> 30; z *= 0xBF58476D1CE4E5B9ull; z ^= z >> 27; z *= 0x94D049BB133111EBull; z ^= z >> 31;
-    uint32_t x = (uint32_t)z + (uint32_t)(slot >> 1) * 0x9E3779B9u;
-    x ^= x >> 16; x *= 0x7FEB352Du; x ^= x >> 15; x *= 0x846CA68Bu; x ^= x >> 16;
-    return (slot & 1) ? x << 16 : x & 0xffff0000u;   /* 16 bits per slot, two slots per hash */
+    uint32_t x = (uint32_t)z | 1u;                    /* xorshift32 (13, 17, 5) from the node word */
+    for (int m = 0; m <= (slot >> 1); m++) { x ^= x << 13; x ^= x >> 17; x ^= x << 5; }
+    return (slot & 1) ? x << 16 : x & 0xffff0000u;   /* 16 bits per slot, one word per two slots */
 }
 /* BF16: add 16 random bits below the kept ones, truncate. */
 uint16_t f2bf_sr(float f, uint32_t r) {
@@ -842,9 +843,13 @@ void pull_step_impl(const orc_cfg *cfg, const typename S::raw *A, typename S::ra
                         }
                     }
                     T gp = (T)(-cfg->sc_G) * psi[id];
-                    ex = ex + (gp * IF[0]) / (ti * rho);
-                    ey = ey + (gp * IF[1]) / (ti * rho);
-                    if (Q == 19) ez = ez + (gp * IF[2]) / (ti * rho);
+                    /* Shan-Chen velocity shift: one division, then
+                     * u_eq += (-G psi / (tau^-1 rho)) F_sum with fused
+                     * multiply-adds (the GPU sweeps' sequence) */
+                    const T kf = gp / (ti * rho);
+                    ex = std::fma(kf, IF[0], ex);
+                    ey = std::fma(kf, IF[1], ey);
+                    if (Q == 19) ez = std::fma(kf, IF[2], ez);
                 }
                 if (S::DEV) {
                     T eq[19];
