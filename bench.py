@@ -36,6 +36,10 @@ def parse():
                     help="fix the load/store mode (0 plain, 1 non-temporal stores, 2 non-temporal loads+stores)")
     ap.add_argument("--layout", type=int, default=-1,
                     help="fix the population layout (0 [i][z][y][x], 1 [z][y][i][x]; -1 = autotune)")
+    ap.add_argument("--block-y", type=int, default=0,
+                    help="rows per tile of the one-pass Shan-Chen sweeps (0 = autotune)")
+    ap.add_argument("--sc-zchunk", type=int, default=0,
+                    help="z-planes per workgroup of the one-pass Shan-Chen sweeps (0 = autotune)")
     ap.add_argument("--sc-sweep", type=int, default=-1,
                     help="Shan-Chen step form (0 density pass + sweep, 1 / 2 one z-marching sweep on tiles, "
                          "3 row-slab one-pass sweep of periodic boxes; -1 = autotune)")
@@ -58,13 +62,15 @@ def parse():
     return ap.parse_args()
 
 
-def spinodal_traffic():
-    """HBM bytes per Shan-Chen step (density pass + sweep) measured with
-    rocprofv3 --pmc on the 192^3 FP32 workload (committed summary)."""
+def spinodal_traffic(storage):
+    """HBM bytes per Shan-Chen step of the tuned form, measured with
+    rocprofv3 --pmc on the 192^3 workload (committed summary; the 16-bit
+    storages are profiled with nearest-even rounding, which moves the same
+    bytes)."""
     p = os.path.join(ROOT, "profiles", "d3q19_spinodal_192_traffic.json")
     try:
         with open(p) as fh:
-            return json.load(fh).get("hbm_bytes_per_step")
+            return json.load(fh)["hbm_bytes_per_step"][storage]
     except Exception:
         return None
 
@@ -234,7 +240,8 @@ def bench_spinodal(args):
     for st, rm in runs:
         key = st + ("_nearest" if rm == "nearest" else "")
         cfg, solid, init = scenes.spinodal(storage=st, round_mode=rm, block_x=args.block_x, sc_sweep=args.sc_sweep,
-                                           xcd_remap=args.xcd_remap, mem_mode=args.mem_mode, layout=args.layout)
+                                           xcd_remap=args.xcd_remap, mem_mode=args.mem_mode, layout=args.layout,
+                                           block_y=args.block_y, sc_zchunk=args.sc_zchunk)
         sol = lb.Solver(cfg, solid)
         sol.init_equilibrium(init["rho"], 0.0, 0.0, 0.0)
         m0 = sol.reduce()[0]
@@ -247,11 +254,16 @@ def bench_spinodal(args):
         m1 = sol.reduce()[0]
         rho = sol.macros()["rho"]
         fields[key] = rho.astype(np.float64)
+        # the step's kernels alone, from HIP events on the solver's stream (a
+        # graph of back-to-back steps; advances the state, after the fields)
+        kms = sol.time_step_kernel(args.kernel_launches)
         n = rho.size
         p5, p95 = np.percentile(rho, [5, 95])
         esz = {"f64": 8, "f32": 4}.get(st, 2)
         out[key] = {"mlups": round(n * steps / dt / 1e6, 1), "seconds": round(dt, 3),
                     "pct_hbm_at_2Q_bytes": round(100 * n * steps / dt * 38 * esz / (PEAK_HBM_GBS * 1e9), 2),
+                    "kernel_us_per_step": round(kms * 1e3, 2),
+                    "kernel_pct_hbm_at_2Q_bytes": round(100 * n / (kms * 1e-3) * 38 * esz / (PEAK_HBM_GBS * 1e9), 2),
                     "round_mode": sol.launch_config["round_mode"] if esz == 2 else None,
                     "mass_drift_rel": float((m1 - m0) / m0), "rho_min": float(rho.min()), "rho_max": float(rho.max()),
                     "rho_p5": float(p5), "rho_p95": float(p95), "launch": sol.launch_config}
@@ -268,13 +280,26 @@ def bench_spinodal(args):
         out[key]["phase_density_err_vs_fp64"] = [out[key]["rho_p5"] - out["f64"]["rho_p5"],
                                                  out[key]["rho_p95"] - out["f64"]["rho_p95"]]
     # Algorithmic bytes per update are 2*Q*sizeof(population) = 152 B (FP32),
-    # psi and masks excluded.  The two-pass Shan-Chen step moves more: the
-    # density pass reads the 19 populations again and stores psi, and the
-    # sweep reads psi: 152 + 76 + 4 + 4 = 236 B; the traffic counters measure
-    # hbm_bytes_per_step (profiles/d3q19_spinodal_192_traffic.json).
+    # 76 B (16-bit), psi and masks excluded; `achieved` divides them by the
+    # step's kernel time from HIP events.  The row-slab one-pass sweep
+    # (sc_sweep 3) moves about that; the two-pass step moves 236 B (FP32: the
+    # density pass reads the 19 populations again and stores psi, the sweep
+    # reads psi).  Measured HBM bytes per step for each storage's tuned form:
+    # profiles/d3q19_spinodal_192_traffic.json.
     n = 192 ** 3
     ms = out["f32"]["seconds"] / steps * 1e3
-    achieved = 152.0 * n / (ms * 1e-3) / 1e9
+    kms = out["f32"]["kernel_us_per_step"] * 1e-3
+    achieved = 152.0 * n / (kms * 1e-3) / 1e9
+    forms = {0: "density pass + sweep (k_density + k_step)", 1: "tiled one-pass sweep (k_step_scz)",
+             2: "tiled one-pass sweep (k_step_scz)", 3: "row-slab one-pass sweep (k_step_scr)"}
+    rl16 = None
+    if "bf16" in out:
+        k16 = out["bf16"]["kernel_us_per_step"] * 1e-3
+        a16 = 76.0 * n / (k16 * 1e-3) / 1e9
+        rl16 = {"storage": "bf16, stochastic rounding (the Shan-Chen default)", "bound": "hbm",
+                "achieved": round(a16, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(a16 / PEAK_HBM_GBS, 4),
+                "traffic": spinodal_traffic("bf16"), "bytes_per_update": 76,
+                "kernel": forms.get(out["bf16"]["launch"]["sc_sweep"]), "kernel_us": out["bf16"]["kernel_us_per_step"]}
     cpu = None
     if not args.no_cpu_baseline:
         cfg, solid, init = scenes.spinodal(storage="f32")
@@ -285,11 +310,10 @@ def bench_spinodal(args):
                   {"workload": "D3Q19 Shan-Chen spinodal decomposition 192^3 periodic, psi=1-exp(-rho), G=-5, "
                                "tau=1, rho=0.7+U(-0.01,0.01) seed 0"}, storage=out,
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": spinodal_traffic(),
-                            "kernel": "k_density<19,f32> + k_step<19,f32,SC> per step", "bytes_per_update": 152,
-                            "two_pass_bytes_per_update": 236,
-                            "frac_at_two_pass_bytes": round(236.0 * n / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
-                  cpu_baseline=cpu)
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": spinodal_traffic("f32"),
+                            "kernel": forms.get(out["f32"]["launch"]["sc_sweep"]), "bytes_per_update": 152,
+                            "kernel_us": out["f32"]["kernel_us_per_step"]},
+                  roofline_16bit=rl16, cpu_baseline=cpu)
 
 
 def bench_ensemble(args):
