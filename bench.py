@@ -75,6 +75,15 @@ def spinodal_traffic(storage):
         return None
 
 
+def profile_field(name, key):
+    """A figure of a committed rocprofv3 summary under profiles/ (None if absent)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as fh:
+            return json.load(fh).get(key)
+    except Exception:
+        return None
+
+
 def traffic_from_profiles(storage):
     """HBM bytes per sweep measured with rocprofv3 --pmc (committed summary)."""
     p = os.path.join(ROOT, "profiles", "d3q19_sphere_256x128x128_%s_traffic.json" % storage)
@@ -357,7 +366,8 @@ def bench_ensemble(args):
                   {"workload": "512 D2Q9 channels, sizes {64x64,128x64,128x128,256x128}, 1-4 circles, "
                                "u_in in [0.02,0.08], tau in [0.52,0.9], one launch per step", "nodes": n},
                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": profile_field(
+                                "d2q9_ensemble_512_traffic.json", "hbm_bytes_per_launch"),
                             "kernel": "k_batch_step (all lattices, one launch per step)", "bytes_per_update": 72},
                   cpu_baseline=cpu,
                   steady={"converged": int(conv.sum()), "of": len(specs), "wall_s": round(wall, 2),
