@@ -129,33 +129,24 @@ template <class S> LB_DEV void scr_pack(const Raw<S> &r, Held<S> &h) {
     }
 }
 
-// Loads the populations of the node at (x, y, z) after streaming into h: slot
-// i comes from (x - c_x, y - c_y, z - c_z), periodic.  ysb / zsb: element
-// offsets of the source rows / planes for c = +1, 0, -1 (index 1 - c); xb:
-// per-lane byte offsets of the source columns, same indexing.
+// Loads the populations of the node at (x, y, z) after streaming: slot i
+// comes from (x - c_x, y - c_y, z - c_z), periodic.  po: byte offsets of the
+// population slots (i ps sizeof); ysb / zsb: byte offsets of the source rows
+// / planes for c = +1, 0, -1 (index 1 - c); xb: per-lane byte offsets of the
+// source columns, same indexing.  The nine row + plane sums are formed once,
+// so each load costs one scalar add.
 template <class S>
-LB_DEV void scr_pull(const StepArgs<S> &a, rsrc_t rs, const uint32_t ysb[3], const uint32_t zsb[3],
-                     const uint32_t xb[3], Held<S> &h) {
-    typedef Vel<19> L;
-    constexpr uint32_t ES = sizeof(typename S::raw);
-    h.fill([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-        const uint32_t so = ((uint32_t)i * a.ps + ysb[1 - cy] + zsb[1 - cz]) * ES;
-        if constexpr (ES == 2) return scr_ld16(rs, xb[1 - cx], so);
-        else return scr_ld32(rs, xb[1 - cx], so);
-    });
-}
-
-template <class S>
-LB_DEV void scr_pull_raw(const StepArgs<S> &a, rsrc_t rs, const uint32_t ysb[3], const uint32_t zsb[3],
+LB_DEV void scr_pull_raw(rsrc_t rs, const uint32_t po[19], const uint32_t ysb[3], const uint32_t zsb[3],
                          const uint32_t xb[3], Raw<S> &r) {
     typedef Vel<19> L;
     constexpr uint32_t ES = sizeof(typename S::raw);
+    uint32_t rz[3][3];
+    for (int u = 0; u < 3; u++)
+        for (int v = 0; v < 3; v++) rz[u][v] = ysb[u] + zsb[v];
     static_for<0, 19>([&](auto I) {
         constexpr int i = decltype(I)::value;
         constexpr int cx = L::cx(i), cy = L::cy(i), cz = L::cz(i);
-        const uint32_t so = ((uint32_t)i * a.ps + ysb[1 - cy] + zsb[1 - cz]) * ES;
+        const uint32_t so = po[i] + rz[1 - cy][1 - cz];
         if constexpr (ES == 2) r.v[i] = scr_ld16(rs, xb[1 - cx], so);
         else r.v[i] = scr_ld32(rs, xb[1 - cx], so);
     });
@@ -169,11 +160,12 @@ template <class S> LB_DEV void scr_unpack(const Held<S> &h, typename S::T *f) {
     });
 }
 
-// Element offsets of the planes z - c_z for c_z = +1, 0, -1.
+// Byte offsets of the planes z - c_z for c_z = +1, 0, -1.
 template <class S> LB_DEV void scr_planes(const StepArgs<S> &a, int z, uint32_t zsb[3]) {
-    zsb[0] = (uint32_t)wrap1(z - 1, a.nz) * a.sz;
-    zsb[1] = (uint32_t)z * a.sz;
-    zsb[2] = (uint32_t)wrap1(z + 1, a.nz) * a.sz;
+    const uint32_t bz = a.sz * (uint32_t)sizeof(typename S::raw);
+    zsb[0] = (uint32_t)wrap1(z - 1, a.nz) * bz;
+    zsb[1] = (uint32_t)z * bz;
+    zsb[2] = (uint32_t)wrap1(z + 1, a.nz) * bz;
 }
 
 // psi of a node from its pulled populations (psi_of_node's arithmetic).
@@ -196,7 +188,7 @@ template <class S> LB_DEV typename S::T scr_psi(const StepArgs<S> &a, const Raw<
 }
 
 // A row segment: row r of the (BY + 2)-row window, lattice row y, the lane's
-// x, and the offsets of its source rows and columns.
+// x, and the byte offsets of its source rows and columns.
 struct Seg {
     int r, y, x;
     bool on;        // the segment exists in this tile (uniform per wave)
@@ -210,7 +202,7 @@ template <class S> LB_DEV void scr_seg(const StepArgs<S> &a, Seg &g, int r, int 
     g.x = xs * 64 + (int)(threadIdx.x & 63);
     g.lane = g.x < a.nx;
     for (int c = 0; c < 3; c++) {
-        g.ysb[c] = (uint32_t)wrap1(g.y + c - 1, a.ny) * a.sy;                              // row y - c_y, c_y = 1 - c
+        g.ysb[c] = (uint32_t)wrap1(g.y + c - 1, a.ny) * a.sy * (uint32_t)sizeof(typename S::raw);   // row y - c_y, c_y = 1 - c
         g.xb[c] = (uint32_t)wrap1(g.x + c - 1, a.nx) * (uint32_t)sizeof(typename S::raw);   // column x - c_x
     }
 }
@@ -244,6 +236,8 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
         scr_seg(a, si[j], 1 + s / nseg, s % nseg, y0, s < bye * nseg);
     });
     scr_seg(a, sf, w < nseg ? 0 : bye + 1, w % nseg, y0, w < 2 * nseg);
+    uint32_t po[19];                // byte offsets of the population slots
+    for (int i = 0; i < Q; i++) po[i] = (uint32_t)i * a.ps * ES;
     const rsrc_t rsa = make_rsrc(a.A, (uint64_t)Q * a.N * ES);
     const rsrc_t rsb = make_rsrc(a.B, (uint64_t)Q * a.N * ES);
     uint32_t step = 0;
@@ -254,16 +248,16 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
     float mac[SPI][4];
     // B2: the stores of plane q
     auto store_plane = [&](int q) {
-        const uint32_t qe = (uint32_t)q * a.sz;
+        const uint32_t qe = (uint32_t)q * a.sz * ES;
         static_for<0, SPI>([&](auto J) {
             constexpr int j = decltype(J)::value;
             const Seg &g = si[j];
             if (!(g.on && g.lane)) return;
             const uint32_t vo = (uint32_t)g.x * ES;
-            const uint32_t rowe = (uint32_t)g.y * a.sy + qe;
+            const uint32_t rowe = (uint32_t)g.y * a.sy * ES + qe;
             static_for<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                scr_st(res[j].template get<i>(), rsb, vo, ((uint32_t)i * a.ps + rowe) * ES);
+                scr_st(res[j].template get<i>(), rsb, vo, po[i] + rowe);
             });
             if (MACROS) {
                 const uint32_t id = (uint32_t)g.x + (uint32_t)nx * ((uint32_t)g.y + (uint32_t)ny * (uint32_t)q);
@@ -280,9 +274,9 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
         if (dens) {
             static_for<0, SPI>([&](auto J) {
                 constexpr int j = decltype(J)::value;
-                if (si[j].on && si[j].lane) scr_pull_raw<S>(a, rsa, si[j].ysb, zsk, si[j].xb, di[j]);
+                if (si[j].on && si[j].lane) scr_pull_raw<S>(rsa, po, si[j].ysb, zsk, si[j].xb, di[j]);
             });
-            if (sf.on && sf.lane) scr_pull_raw<S>(a, rsa, sf.ysb, zsk, sf.xb, df);
+            if (sf.on && sf.lane) scr_pull_raw<S>(rsa, po, sf.ysb, zsk, sf.xb, df);
         }
         // B1: sweep plane q = k - 2 with the psi of planes q - 1, q, q + 1;
         // the values to store wait in registers for B2
