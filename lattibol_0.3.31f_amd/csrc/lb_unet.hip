@@ -742,6 +742,56 @@ __global__ __launch_bounds__(256) void k_sum_chunks16(SumSeg a, SumSeg b, unsign
     }
 }
 
+// The same sums for segments whose n is a multiple of 4 (16-byte aligned
+// arrays), four adjacent elements per thread, with the same per-element
+// order: a segment of more than 16 chunks takes blocks of 16 element quads x
+// 16 phases as above; one of at most 16 chunks (where the phase sums above
+// are the plain in-order sum) takes 256 quads per block, each thread adding
+// the chunks in order with no LDS step.
+inline unsigned sum4_blocks(const SumSeg &g) {
+    const size_t q = g.n / 4;
+    return (unsigned)(g.nchunk > 16 ? (q + 15) / 16 : (q + 255) / 256);
+}
+__global__ __launch_bounds__(256) void k_sum_chunks16_x4(SumSeg a, SumSeg b, unsigned nblk_a) {
+    const bool first = blockIdx.x < nblk_a;
+    const SumSeg g = first ? a : b;
+    const unsigned blk = first ? blockIdx.x : blockIdx.x - nblk_a;
+    const size_t nq = g.n / 4;
+    const float4 *part = (const float4 *)g.partial;
+    if (g.nchunk <= 16) {
+        const size_t e = (size_t)blk * 256 + threadIdx.x;
+        if (e >= nq) return;
+        float4 s = float4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < g.nchunk; c++) {
+            const float4 v = part[(size_t)c * nq + e];
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        }
+        ((float4 *)g.out)[e] = s;
+        return;
+    }
+    const int el = threadIdx.x & 15, ph = threadIdx.x >> 4;
+    const size_t e = (size_t)blk * 16 + el;
+    float4 s = float4{0.f, 0.f, 0.f, 0.f};
+    if (e < nq) {
+#pragma unroll 8
+        for (int c = ph; c < g.nchunk; c += 16) {
+            const float4 v = part[(size_t)c * nq + e];
+            s.x += v.x; s.y += v.y; s.z += v.z; s.w += v.w;
+        }
+    }
+    __shared__ float4 sh[16][17];
+    sh[ph][el] = s;
+    __syncthreads();
+    if (threadIdx.x < 16 && e < nq) {
+        float4 v = float4{0.f, 0.f, 0.f, 0.f};
+        for (int p = 0; p < 16; p++) {
+            const float4 w = sh[p][el];
+            v.x += w.x; v.y += w.y; v.z += w.z; v.w += w.w;
+        }
+        ((float4 *)g.out)[e] = v;
+    }
+}
+
 // Bias-gradient chunking of k_relu_bwd_bias: at most 1,024 chunks of >= 64
 // pixels (more chunks keep more loads in flight here but cost more in the
 // chunk sum, whose threads add up to nchunk / 16 partials each).
@@ -1149,7 +1199,16 @@ constexpr int kHeadChunk = 512;
 
 // Fixed-order sum of nchunk partial arrays of n floats (and, in the same
 // launch, of a second set when b.n > 0).
+inline bool quad_ok(const SumSeg &g) {
+    return g.n % 4 == 0 && (uintptr_t)g.partial % 16 == 0 && (uintptr_t)g.out % 16 == 0;
+}
 void sum_chunks2(lb_unet *u, SumSeg a, SumSeg b) {
+    if (quad_ok(a) && quad_ok(b)) {
+        const unsigned na = sum4_blocks(a), nb = sum4_blocks(b);
+        if (na + nb == 0) return;
+        hipLaunchKernelGGL(k_sum_chunks16_x4, dim3(na + nb), dim3(256), 0, u->stream, a, b, na);
+        return;
+    }
     const unsigned na = (unsigned)((a.n + 15) / 16), nb = (unsigned)((b.n + 15) / 16);
     if (na + nb == 0) return;
     hipLaunchKernelGGL(k_sum_chunks16, dim3(na + nb), dim3(256), 0, u->stream, a, b, na);
