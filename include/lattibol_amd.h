@@ -419,6 +419,12 @@ int lb_conv3x3_wgrad_bf16(int method, const float *d_dout, const void *d_in, flo
  * round-to-nearest-even of the oracle; *mismatches = non-NaN inputs where
  * the two differ (0 expected).  Synchronises; allocates 8 bytes. */
 int lb_check_bf16_rounding(uint64_t *mismatches);
+/* Diagnostic: for every 32-bit pattern and two random words derived from
+ * (seed, pattern), the stochastic-rounding conversions of the 16-bit stores
+ * (round_mode 1: the gfx950 conversion instructions) against the oracle's
+ * bit-pattern formulas; mismatches[0] = BF16, mismatches[1] = FP16 pairs
+ * that differ (0 expected).  Synchronises; allocates 16 bytes. */
+int lb_check_sr_rounding(uint32_t seed, uint64_t *mismatches);
 
 /* ---------------------------------------------------------------------- */
 /* 5. Input decks and VTK output (host only)                                */

@@ -156,13 +156,31 @@ LB_DEV uint32_t sr_mix(uint32_t seed, int slot) {
     return (slot & 1) ? x << 16 : x & 0xffff0000u;
 }
 LB_DEV uint32_t sr_bits(uint32_t step, uint32_t id, int slot) { return sr_mix(sr_seed(step, id), slot); }
-// BF16: add 16 random bits below the kept ones and truncate.
-// Both results are formed and one selected (no branch per population).
-LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
+// sr_mix for a conversion that reads only the top 16 bits (BF16): the even
+// slot's word is passed unmasked, the same result without the mask.
+LB_DEV uint32_t sr_mix_hi(uint32_t seed, int slot) {
+    uint32_t x = seed | 1u;
+    for (int m = 0; m <= (slot >> 1); m++) {
+        x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+    }
+    return (slot & 1) ? x << 16 : x;
+}
+// BF16: add the top 16 bits of r below the kept bits and truncate;
+// infinities and NaN (quieted) as in round to nearest even.  The formula
+// on the bit pattern; the stores use the instruction below.
+LB_DEV uint16_t f32_to_bf16_sr_bits(float f, uint32_t r) {
     const uint32_t u = __builtin_bit_cast(uint32_t, f);
-    const uint16_t t = (uint16_t)((u + (r >> 16)) >> 16);
-    const uint16_t n = f32_to_bf16(f);
-    return (u & 0x7f800000u) == 0x7f800000u ? n : t;
+    if ((u & 0x7f800000u) == 0x7f800000u) return f32_to_bf16_bits(f);
+    return (uint16_t)((u + (r >> 16)) >> 16);
+}
+// v_cvt_sr_bf16_f32 (gfx950) rounds exactly so: one instruction per value
+// (checked against f32_to_bf16_sr_bits over every FP32 pattern,
+// lb_check_sr_rounding).
+typedef __bf16 lb_bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 lb_f16x2 __attribute__((ext_vector_type(2)));
+LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
+    const lb_bf16x2 v = __builtin_amdgcn_cvt_sr_bf16_f32(lb_bf16x2{}, f, r, false);
+    return (uint16_t)(__builtin_bit_cast(uint32_t, v) & 0xffffu);
 }
 // FP16: the same with 13 dropped bits in the normal range (the truncated
 // value is exact in FP16); below 2^-14 the spacing is 2^-24:
@@ -178,11 +196,23 @@ LB_DEV uint16_t f32_to_bf16_sr(float f, uint32_t r) {
 // splits as (m >> k) + (((m mod 2^k) 2^(23-k) + r23) >> 23), whose second
 // term stays below 2^24, so 32-bit words suffice; k is clamped to [14, 47]
 // (at 47 and above q is 0 either way).
-LB_DEV uint16_t f32_to_f16_sr(float f, uint32_t r) {
+// HW = false: the normal-range result from the bit pattern (the form the
+// instruction is checked against); HW = true: from v_cvt_sr_f16_f32
+// (gfx950), which equals it on 2^-14 <= |v| < 65520 (checked over every
+// FP32 pattern, lb_check_sr_rounding); the instruction's subnormal and
+// overflow results differ (it truncates after adding the same 13 bits), so
+// those lanes keep the integer forms.
+template <bool HW = true> LB_DEV uint16_t f32_to_f16_sr(float f, uint32_t r) {
     const uint32_t x = __builtin_bit_cast(uint32_t, f);
     const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7fffffffu;
-    const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
-    const uint16_t hn = (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
+    uint16_t hn;
+    if constexpr (HW) {
+        const lb_f16x2 v = __builtin_amdgcn_cvt_sr_f16_f32(lb_f16x2{}, f, r, false);
+        hn = (uint16_t)(__builtin_bit_cast(uint32_t, v) & 0xffffu);
+    } else {
+        const uint32_t t = (ax + (r >> 19)) & ~0x1fffu;
+        hn = (uint16_t)(sign | f32_to_f16(__builtin_bit_cast(float, t)));
+    }
     const bool rare = ax >= 0x477ff000u || ax < 0x38800000u;
     if (__builtin_amdgcn_ballot_w64(rare) == 0) return hn;
     const uint32_t e = ax >> 23, m = (ax & 0x7fffffu) | (e ? 0x800000u : 0u);

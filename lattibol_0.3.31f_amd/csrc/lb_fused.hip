@@ -284,7 +284,52 @@ __global__ __launch_bounds__(256) void k_check_bf16(unsigned long long *bad) {
     if ((threadIdx.x & 63) == 0 && n) atomicAdd(bad, n);
 }
 
+// Every FP32 pattern u with two random words (the node word sr_seed(seed, u)
+// and its complement): the stochastic-rounding conversions of the stores
+// against their bit-pattern forms, BF16 and FP16 mismatches counted apart.
+// (2^32 is a multiple of the grid, so every lane runs every iteration and
+// the FP16 conversion's wave-wide vote sees full waves.)
+__global__ __launch_bounds__(256) void k_check_sr(uint32_t seed, unsigned long long *bad) {
+    unsigned long long nb = 0, nh = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < (1ull << 32); i += (uint64_t)gridDim.x * 256) {
+        const uint32_t u = (uint32_t)i;
+        const float f = __builtin_bit_cast(float, u);
+        const uint32_t r0 = sr_seed(seed, u);
+        for (int k = 0; k < 2; k++) {
+            const uint32_t r = k ? ~r0 : r0;
+            if (f32_to_bf16_sr(f, r) != f32_to_bf16_sr_bits(f, r)) nb++;
+            if (f32_to_f16_sr<true>(f, r) != f32_to_f16_sr<false>(f, r)) nh++;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        nb += __shfl_xor(nb, off, 64);
+        nh += __shfl_xor(nh, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nb) atomicAdd(bad, nb);
+        if (nh) atomicAdd(bad + 1, nh);
+    }
+}
+
 }  // namespace
+
+extern "C" int lb_check_sr_rounding(uint32_t seed, uint64_t *mismatches) {
+    if (!mismatches) return LB_ERR_INVALID;
+    unsigned long long *d = nullptr;
+    int rc = (int)hipMalloc((void **)&d, 2 * sizeof(*d));
+    if (rc) return rc;
+    rc = (int)hipMemset(d, 0, 2 * sizeof(*d));
+    if (!rc) {
+        hipLaunchKernelGGL(k_check_sr, dim3(16384), dim3(256), 0, 0, seed, d);
+        rc = (int)hipGetLastError();
+    }
+    unsigned long long h[2] = {0, 0};
+    if (!rc) rc = (int)hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    mismatches[0] = (uint64_t)h[0];
+    mismatches[1] = (uint64_t)h[1];
+    return rc;
+}
 
 extern "C" int lb_check_bf16_rounding(uint64_t *mismatches) {
     if (!mismatches) return LB_ERR_INVALID;

@@ -324,7 +324,7 @@ template <class S> constexpr bool kRoundable = std::is_same<S, StF16>::value || 
 template <class S, bool SR>
 LB_DEV typename S::raw store_val_m(const NodePos &p, typename S::T v, float w, int slot) {
     if constexpr (SR && std::is_same<S, StF16>::value) return f32_to_f16_sr(v, sr_mix(p.seed, slot));
-    else if constexpr (SR && std::is_same<S, StBF16>::value) return f32_to_bf16_sr(v, sr_mix(p.seed, slot));
+    else if constexpr (SR && std::is_same<S, StBF16>::value) return f32_to_bf16_sr(v, sr_mix_hi(p.seed, slot));
     else return S::st(v, w);
 }
 template <class S>

@@ -185,6 +185,7 @@ _PROTOS = {
     "lb_conv3x3_bf16_noalloc": (_i, [_i, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_conv3x3_wgrad_bf16": (_i, [_i, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]),
     "lb_check_bf16_rounding": (_i, [C.POINTER(C.c_uint64)]),
+    "lb_check_sr_rounding": (_i, [C.c_uint32, C.POINTER(C.c_uint64)]),
     "lb_deck_read_params": (_i, [C.c_char_p, C.POINTER(LbDeckParams)]),
     "lb_deck_read_bmp": (_i, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),
     "lb_deck_voxelize_stl": (_i, [C.c_char_p, _i, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), _vp]),

@@ -183,6 +183,20 @@ def test_gpu_bf16_conversion_instruction_matches_bit_rounding(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("seed", [1, 0xC0FFEE])
+def test_gpu_stochastic_rounding_instructions_match_bit_formulas(gpu, seed):
+    # every FP32 pattern with two random words each: v_cvt_sr_bf16_f32 ==
+    # the oracle's BF16 formula (NaN, infinities and denormals included), and
+    # the FP16 conversion built on v_cvt_sr_f16_f32 == its integer form, so
+    # stochastic-rounding stores stay bit-exact against the oracle
+    import ctypes as C
+    from lattibol_amd import _lib
+    bad = (C.c_uint64 * 2)(7, 7)
+    _lib.check(_lib.lib.lb_check_sr_rounding(seed, bad), "lb_check_sr_rounding")
+    assert (bad[0], bad[1]) == (0, 0)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("ci,co,H,W", [(16, 32, 16, 16), (32, 64, 8, 32), (96, 32, 16, 16), (384, 128, 16, 16),
                                        (64, 32, 8, 8), (16, 32, 8, 64), (96, 32, 16, 32), (384, 128, 8, 32),
                                        (64, 64, 24, 32), (32, 32, 128, 128), (256, 256, 16, 16), (16, 32, 32, 48),
