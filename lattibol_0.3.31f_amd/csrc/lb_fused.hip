@@ -435,7 +435,10 @@ struct lb_ctx {
         if (stream) (void)hipStreamDestroy(stream);
     }
     PopLayout pl() const { return pop_layout(layout, Q, cfg.nx, cfg.ny, cfg.nz); }
+    // (the stream is drained first: a graph launched by an earlier lb_step
+    // may still be queued or running)
     void drop_graphs() {
+        if (stream && (graph[0] || graph[1] || !tails.empty())) (void)hipStreamSynchronize(stream);
         for (auto &g : graph) if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
         for (auto &t : tails) (void)hipGraphExecDestroy(t.g);
         tails.clear();
@@ -786,6 +789,11 @@ int tail_graph(lb_ctx *c, int src, int r, lb_ctx::Tail *out) {
     if (g) (void)hipGraphDestroy(g);
     if (rc) return rc;
     if (c->tails.size() >= lb_ctx::kMaxTails) {
+        // the evicted graph may still be queued on the stream
+        if (int src_rc = (int)hipStreamSynchronize(c->stream)) {
+            (void)hipGraphExecDestroy(ge);
+            return src_rc;
+        }
         (void)hipGraphExecDestroy(c->tails.front().g);
         c->tails.erase(c->tails.begin());
     }
