@@ -149,3 +149,28 @@ def test_row_slab_c3_launch_shapes(nw):
     by = max_rows(nw, 192)
     for zc in (6, 12):
         emulate(192, 2 * by + 3, 2 * zc + 1, nw, by, zc, "rows")
+
+
+@pytest.mark.parametrize("kind", ["soa", "rows"])
+@pytest.mark.parametrize("es", [4, 2])
+def test_row_slab_byte_offsets_stay_32_bit(kind, es):
+    """The kernel forms every load and store offset in 32-bit byte units:
+    slot i ps es, row y sy es, plane z sz es (each product in 32 bits), their
+    sums, plus x es per lane.  The host admits the form only for
+    19 N es < 2^32 (scr_lattice_ok, csrc/lb_fused.hip); on the largest such
+    lattices (nx up to 512) every product and partial sum of the largest
+    offset stays below 2^32, and one node more per plane row breaks the rule."""
+    for nx in (512, 192, 130):
+        n_max = ((1 << 32) - 1) // (19 * es)             # largest N the host admits
+        ny = 64
+        nz = n_max // (nx * ny)
+        N = nx * ny * nz
+        assert 19 * N * es < (1 << 32) and 19 * nx * ny * (nz + 1) * es >= (1 << 32)
+        ps, sy, sz = layout(kind, nx, ny, nz)
+        terms = [18 * ps * es, (ny - 1) * sy * es, (nz - 1) * sz * es, (nx - 1) * es, sz * es, sy * es]
+        assert all(t < (1 << 32) for t in terms)
+        rz = (ny - 1) * sy * es + (nz - 1) * sz * es       # the row + plane sum
+        assert rz < (1 << 32) and 18 * ps * es + rz < (1 << 32)
+        top = 18 * ps * es + rz + (nx - 1) * es            # the last population of the last node
+        assert top == (19 * N - 1) * es if kind == "rows" else top == (18 * N + N - 1) * es
+        assert top < (1 << 32)
