@@ -215,7 +215,8 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
                                                       const bf16 *__restrict__ W, const float *__restrict__ bias,
                                                       TOut *__restrict__ out, float *__restrict__ part, int npix, int H,
                                                       int Wd, int Ci, int Co, const bf16 *__restrict__ act = nullptr,
-                                                      float *__restrict__ bpart = nullptr) {
+                                                      float *__restrict__ bpart = nullptr, TOut *__restrict__ out2 = nullptr,
+                                                      int csplit = 0) {
     static_assert(!DPRE || (!SPLIT && !BIAS_RELU), "ReLU-backward epilogue: no K split, no bias");
     constexpr int RPT = 32 / SW;                 // image rows per MFMA pixel tile
     constexpr int ROWS = 4 * PM * RPT, IR = ROWS + 2, IC = SW + 2;
@@ -363,6 +364,17 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
             for (int b = 0; b < NB; b++) {
                 const int co = n0 + 32 * b + row;
                 const float bv = (BIAS_RELU && !SPLIT) ? bias[co] : 0.f;
+                // out2: channels [0, csplit) go to out ([pixel][csplit]),
+                // the rest to out2 ([pixel][Co - csplit]); csplit % 32 == 0,
+                // so a 32-channel group has one destination
+                TOut *dst = out;
+                int dc = Co, dco = co;
+                if (out2) {
+                    const bool hi = n0 + 32 * b >= csplit;
+                    dst = hi ? out2 : out;
+                    dc = hi ? Co - csplit : csplit;
+                    dco = hi ? co - csplit : co;
+                }
                 for (int r = 0; r < 16; r++) {
                     const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                     const size_t pix = ((size_t)img * H + ty0 + i / SW) * Wd + x0 + i % SW;
@@ -371,7 +383,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(const TIn *__restrict__ in
                         part[((size_t)blockIdx.z * npix + pix) * Co + co] = v;
                     } else {
                         if (BIAS_RELU) { v += bv; v = v > 0.f ? v : 0.f; }
-                        out[pix * Co + co] = from_f<TOut>(v);
+                        dst[pix * dc + dco] = from_f<TOut>(v);
                     }
                 }
             }
@@ -812,10 +824,13 @@ inline BiasChunks bias_chunks(int npix) {
 // dpool != null: the incoming gradient is d[m] + dpool[parent of m] / 4 (the
 // skip gradient plus the 2x2 average-pool backward of an H x W level), summed
 // here instead of in a pass of its own.
+// dup != null (d unused): the incoming gradient is the nearest-upsampling
+// backward of dup, a 2H x 2W gradient with Co channels: the sum of m's four
+// children, (c00 + c01) + (c10 + c11).
 __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__ d, const bf16 *__restrict__ act,
                                                        bf16 *__restrict__ dpre, int npix, int Co, int chunk,
                                                        float *__restrict__ partial, const float *__restrict__ dpool,
-                                                       int H, int Wd) {
+                                                       int H, int Wd, const float *__restrict__ dup = nullptr) {
     const int tpp = Co >> 2, nph = 256 / tpp;
     const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
     const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
@@ -824,7 +839,17 @@ __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__
 #pragma unroll 4
         for (int m = pbeg + ph; m < pend; m += nph) {
             const size_t o = (size_t)m * Co + 4 * cg;
-            float4 g = *(const float4 *)(d + o);
+            float4 g;
+            if (dup) {
+                const int x = m % Wd, r = m / Wd, y = r % H, b = r / H;
+                const float *q = dup + (((size_t)b * (2 * H) + 2 * y) * (2 * Wd) + 2 * x) * Co + 4 * cg;
+                const float4 a = *(const float4 *)q, bb = *(const float4 *)(q + Co);
+                const float4 cc = *(const float4 *)(q + (size_t)2 * Wd * Co), dd = *(const float4 *)(q + (size_t)2 * Wd * Co + Co);
+                g = float4{(a.x + bb.x) + (cc.x + dd.x), (a.y + bb.y) + (cc.y + dd.y), (a.z + bb.z) + (cc.z + dd.z),
+                           (a.w + bb.w) + (cc.w + dd.w)};
+            } else {
+                g = *(const float4 *)(d + o);
+            }
             if (dpool) {
                 const int x = m % Wd, r = m / Wd, y = r % H, b = r / H;
                 const float4 sp = *(const float4 *)(dpool + (((size_t)b * (H / 2) + y / 2) * (Wd / 2) + x / 2) * Co + 4 * cg);
@@ -1340,6 +1365,42 @@ int dgrad_dpre_launch(lb_unet *u, const bf16 *dout, const bf16 *Wt, const bf16 *
     return nchunk;
 }
 
+// Input gradient of a decoder level's first conv (Co = cl upsampled-low
+// channels + the skip channels) written as two tensors: channels [0, cl) to
+// out_low ([pixel][cl]), the rest to out_skip ([pixel][Co - cl]), so neither
+// half is copied out of a concatenated gradient.  False (nothing launched)
+// unless the shape takes an unsplit LDS-tile plan and cl % 32 == 0.
+bool dgrad_split_launch(lb_unet *u, const bf16 *in, const bf16 *W, float *out_low, float *out_skip, int npix, int H,
+                        int Wd, int Ci, int Co, int cl) {
+    const ConvPlan pl = conv_plan(npix, H, Wd, Ci, Co);
+    if (!pl.sw || pl.ks > 1 || cl <= 0 || cl >= Co || cl % 32) return false;
+    const int B = npix / (H * Wd);
+    const int rows = 4 * pl.pm * (32 / pl.sw);
+    const dim3 grd((unsigned)(B * (H / rows) * (Wd / pl.sw)), (unsigned)(Co / (32 * pl.nb)), 1u), blk(256);
+    const bf16 *nl = nullptr;
+    if (pl.sw == 32) {
+        if (pl.nb == 2 && pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 2, 2, false>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                               out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+        else if (pl.nb == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 1, 2, false>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                               out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+        else if (pl.pm == 2)
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 2, 1, false>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                               out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+        else
+            hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 1, 1, false>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                               out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+    } else if (pl.nb == 2) {
+        hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 1, 2, false, 16>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                           out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+    } else {
+        hipLaunchKernelGGL((k_conv3x3_tile<bf16, float, 1, 1, false, 16>), grd, blk, 0, u->stream, in, nl, 0, W, nullptr,
+                           out_low, nullptr, npix, H, Wd, Ci, Co, nullptr, nullptr, out_skip, cl);
+    }
+    return true;
+}
+
 template <class TIn, class TOut, bool BR>
 void conv_launch(lb_unet *u, const TIn *in, const bf16 *W, const float *bias, TOut *out, int npix, int H, int Wd,
                  int Ci, int Co, bool gather_only = false) {
@@ -1731,13 +1792,22 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
     bool pre_ready = false;
     int pre_nchunk = 0;
     int ci = last;
+    // set after a decoder level's first conv wrote its input gradient as
+    // [low | skip] tensors: gcur holds the full-resolution gradient of the
+    // upsampled low part, whose upsampling backward the next ReLU backward
+    // sums itself
+    bool up_pending = false;
     // dpool: the incoming gradient of this conv's output is gcur (its skip
     // gradient) plus the average-pool backward of dpool (see k_relu_bwd_bias).
     // fuse: this conv's input is the ReLU output of the conv processed next
     // (same level): its input gradient also applies that conv's ReLU backward
     // and bias partials (dgrad_dpre_launch), where the plan allows.
+    // skip_out: a decoder level's first conv; its input gradient goes to
+    // gcur (low part, skip_cl channels) and skip_out (skip part) where the
+    // plan allows (split_done), else to gcur as the whole [low | skip].
+    bool split_done = false;
     auto conv_bwd = [&](int cidx, const bf16 *input, int l, bool need_din, const bf16 *low = nullptr, int cl = 0,
-                        const float *dpool = nullptr, bool fuse = false) {
+                        const float *dpool = nullptr, bool fuse = false, float *skip_out = nullptr, int skip_cl = 0) {
         const UConv &c = u->convs[cidx];
         const int npix = B * L[l].H * L[l].W;
         bf16 *gpre = gpre_buf[gsel];
@@ -1750,7 +1820,9 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
             // gradient's chunk-sum launch
             const BiasChunks bc = bias_chunks(npix);
             hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, dpool ? u->gskip[l] : gcur,
-                               u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W);
+                               u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W,
+                               up_pending ? (const float *)gcur : nullptr);
+            up_pending = false;
             bseg = SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff};
         }
         wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1, bseg,
@@ -1766,7 +1838,11 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
                 return;
             }
         }
-        conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co, c.ci);
+        split_done = skip_out && dgrad_split_launch(u, gpre, u->wt + u->wt_off[cidx], gtmp, skip_out, npix, L[l].H,
+                                                    L[l].W, c.co, c.ci, skip_cl);
+        if (!split_done)
+            conv_launch<bf16, float, false>(u, gpre, u->wt + u->wt_off[cidx], nullptr, gtmp, npix, L[l].H, L[l].W, c.co,
+                                            c.ci);
         std::swap(gcur, gtmp);
     };
     for (int l = 0; l <= Lv - 2; l++) {
@@ -1774,10 +1850,15 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         conv_bwd(ci, u->act[ci - 1], lev, true, nullptr, 0, nullptr, true);
         ci--;
         if (u->cat_direct.size() > (size_t)lev && u->cat_direct[lev])
-            conv_bwd(ci, u->act[enc_conv(lev) + 1], lev, true, u->cat_low[lev], L[lev + 1].C);
+            conv_bwd(ci, u->act[enc_conv(lev) + 1], lev, true, u->cat_low[lev], L[lev + 1].C, nullptr, false,
+                     u->gskip[lev], L[lev + 1].C);
         else
-            conv_bwd(ci, u->cat[lev], lev, true);
+            conv_bwd(ci, u->cat[lev], lev, true, nullptr, 0, nullptr, false, u->gskip[lev], L[lev + 1].C);
         ci--;
+        if (split_done) {               // gcur = d(low) at this level's resolution, gskip[lev] written
+            up_pending = true;
+            continue;
+        }
         // gcur = d(cat[lev]); split into d(low) and d(skip)
         const int npix = B * L[lev].H * L[lev].W;
         size_t nsk = (size_t)npix * L[lev].C / 4;
