@@ -827,12 +827,23 @@ inline BiasChunks bias_chunks(int npix) {
 // dup != null (d unused): the incoming gradient is the nearest-upsampling
 // backward of dup, a 2H x 2W gradient with Co channels: the sum of m's four
 // children, (c00 + c01) + (c10 + c11).
+// hdy != null (d unused; the last conv, whose output feeds the 1x1 head):
+// the incoming gradient is the head's input gradient, sum_o hdy[m, o]
+// hw[o, c] in o order (hw staged in LDS as floats, ho <= 8, Co <= 256),
+// formed here rather than written and read back.
 __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__ d, const bf16 *__restrict__ act,
                                                        bf16 *__restrict__ dpre, int npix, int Co, int chunk,
                                                        float *__restrict__ partial, const float *__restrict__ dpool,
-                                                       int H, int Wd, const float *__restrict__ dup = nullptr) {
+                                                       int H, int Wd, const float *__restrict__ dup = nullptr,
+                                                       const float *__restrict__ hdy = nullptr,
+                                                       const bf16 *__restrict__ hw = nullptr, int ho = 0) {
     const int tpp = Co >> 2, nph = 256 / tpp;
     const int t = threadIdx.x, cg = t % tpp, ph = t / tpp;
+    __shared__ float hws[8 * 256];
+    if (hdy) {                           // uniform: the whole workgroup takes it
+        for (int e = t; e < ho * Co; e += 256) hws[e] = from_bf16(hw[e]);
+        __syncthreads();
+    }
     const int pbeg = blockIdx.x * chunk, pend = min(npix, pbeg + chunk);
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     if (ph < nph) {
@@ -847,6 +858,20 @@ __global__ __launch_bounds__(256) void k_relu_bwd_bias(const float *__restrict__
                 const float4 cc = *(const float4 *)(q + (size_t)2 * Wd * Co), dd = *(const float4 *)(q + (size_t)2 * Wd * Co + Co);
                 g = float4{(a.x + bb.x) + (cc.x + dd.x), (a.y + bb.y) + (cc.y + dd.y), (a.z + bb.z) + (cc.z + dd.z),
                            (a.w + bb.w) + (cc.w + dd.w)};
+            } else if (hdy) {
+                float gy[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) gy[k] = k < ho ? hdy[(size_t)m * ho + k] : 0.f;
+                float rr[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float sum = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        if (q < ho) sum += gy[q] * hws[q * Co + 4 * cg + k];
+                    rr[k] = sum;
+                }
+                g = float4{rr[0], rr[1], rr[2], rr[3]};
             } else {
                 g = *(const float4 *)(d + o);
             }
@@ -1779,8 +1804,12 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
         hipLaunchKernelGGL(k_head_scatter, dim3(nb256(np)), dim3(256), 0, u->stream, u->partial + (size_t)nchunk * np, C0, O,
                            u->g + u->head_woff, u->g + u->head_boff);
     }
-    hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0 / 4)), dim3(256), 0, u->stream, u->dy, u->wb + u->head_woff,
-                       u->gA, npix0, C0, O);
+    // the head's input gradient is formed by the first ReLU backward
+    // (k_relu_bwd_bias hdy mode), not written to gA
+    bool head_pending = C0 <= 256 && O <= 8;
+    if (!head_pending)
+        hipLaunchKernelGGL(k_head_bwd_data, dim3(nb256((size_t)npix0 * C0 / 4)), dim3(256), 0, u->stream, u->dy,
+                           u->wb + u->head_woff, u->gA, npix0, C0, O);
     // gA = d(out of current conv), post-activation gradient
     float *gcur = u->gA, *gtmp = u->gB;
     // pre-activation gradients (BF16): the two halves of gC, so a fused input
@@ -1821,8 +1850,10 @@ extern "C" int lb_unet_backward(lb_unet *u, const float *d_t, const float *d_mas
             const BiasChunks bc = bias_chunks(npix);
             hipLaunchKernelGGL(k_relu_bwd_bias, dim3(bc.nchunk), dim3(256), 0, u->stream, dpool ? u->gskip[l] : gcur,
                                u->act[cidx], gpre, npix, c.co, bc.chunk, u->bpart, dpool, L[l].H, L[l].W,
-                               up_pending ? (const float *)gcur : nullptr);
+                               up_pending ? (const float *)gcur : nullptr, head_pending ? (const float *)u->dy : nullptr,
+                               u->wb + u->head_woff, O);
             up_pending = false;
+            head_pending = false;
             bseg = SumSeg{u->bpart, bc.nchunk, (size_t)c.co, u->g + c.boff};
         }
         wgrad_launch<bf16, bf16>(u, gpre, input, u->g + c.woff, npix, L[l].H, L[l].W, c.ci, c.co, u->tA, u->tB, -1, bseg,
