@@ -330,10 +330,27 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
                 p.step = step;
                 p.seed = a.round_mode ? sr_seed(step, p.id) : 0u;
                 auto put = [&](auto sr) {
-                    res[j].fill([&](auto I) {
-                        constexpr int i = decltype(I)::value;
-                        return store_val_m<S, decltype(sr)::value>(p, f[i], weight<Q, float>(L::wc(i)), i);
-                    });
+                    if constexpr (decltype(sr)::value && std::is_same<S, StBF16>::value) {
+                        // stochastic BF16: the words of the stream for slot
+                        // pairs (2m, 2m + 1) as store_val_m draws them, each
+                        // pair converted straight into its packed register
+                        // (the instruction's high-half form, no packing step)
+                        uint32_t x = p.seed | 1u;
+                        static_for<0, 9>([&](auto M) {
+                            constexpr int m = decltype(M)::value;
+                            x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+                            lb_bf16x2 v = __builtin_amdgcn_cvt_sr_bf16_f32(lb_bf16x2{}, f[2 * m], x, false);
+                            v = __builtin_amdgcn_cvt_sr_bf16_f32(v, f[2 * m + 1], x << 16, true);
+                            res[j].w[m] = __builtin_bit_cast(uint32_t, v);
+                        });
+                        x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+                        res[j].w[9] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(lb_bf16x2{}, f[18], x, false));
+                    } else {
+                        res[j].fill([&](auto I) {
+                            constexpr int i = decltype(I)::value;
+                            return store_val_m<S, decltype(sr)::value>(p, f[i], weight<Q, float>(L::wc(i)), i);
+                        });
+                    }
                 };
                 if constexpr (kRoundable<S>) {
                     if (a.round_mode) put(std::true_type{});
