@@ -345,6 +345,37 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
                         });
                         x ^= x << 13; x ^= x >> 17; x ^= x << 5;
                         res[j].w[9] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(lb_bf16x2{}, f[18], x, false));
+                    } else if constexpr (decltype(sr)::value && std::is_same<S, StF16>::value) {
+                        // stochastic FP16, by pairs: the instruction's results
+                        // straight into the packed register unless a lane of
+                        // the wave needs the subnormal or overflow form for
+                        // either value; then both through f32_to_f16_sr's
+                        // integer forms (equal to the instruction on the
+                        // other lanes), one wave-wide vote per pair
+                        auto rare = [](float v) {
+                            const uint32_t a = __builtin_bit_cast(uint32_t, v) & 0x7fffffffu;
+                            return a - 0x38800000u >= 0x477ff000u - 0x38800000u;   // |v| < 2^-14 or >= 65520
+                        };
+                        uint32_t x = p.seed | 1u;
+                        static_for<0, 10>([&](auto M) {
+                            constexpr int m = decltype(M)::value;
+                            x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+                            const uint32_t r0 = x & 0xffff0000u, r1 = x << 16;
+                            const float v0 = f[2 * m];
+                            lb_f16x2 h = __builtin_amdgcn_cvt_sr_f16_f32(lb_f16x2{}, v0, r0, false);
+                            if constexpr (m < 9) {
+                                const float v1 = f[2 * m + 1];
+                                h = __builtin_amdgcn_cvt_sr_f16_f32(h, v1, r1, true);
+                                uint32_t wd = __builtin_bit_cast(uint32_t, h);
+                                if (__builtin_amdgcn_ballot_w64(rare(v0) || rare(v1)))
+                                    wd = (uint32_t)f32_to_f16_sr<false>(v0, r0) | ((uint32_t)f32_to_f16_sr<false>(v1, r1) << 16);
+                                res[j].w[m] = wd;
+                            } else {
+                                uint32_t wd = __builtin_bit_cast(uint32_t, h);
+                                if (__builtin_amdgcn_ballot_w64(rare(v0))) wd = (uint32_t)f32_to_f16_sr<false>(v0, r0);
+                                res[j].w[m] = wd;
+                            }
+                        });
                     } else {
                         res[j].fill([&](auto I) {
                             constexpr int i = decltype(I)::value;
