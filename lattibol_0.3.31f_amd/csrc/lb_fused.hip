@@ -824,6 +824,23 @@ int run_graph_or_eager(lb_ctx *c, int nsteps) {
     return 0;
 }
 
+// The Shan-Chen forms are timed on a uniform equilibrium at density 0.5 in
+// both population buffers, written in the layout they are timed in, rather
+// than on the zeroed buffers: a zero deviation (or the state other
+// candidates left) would send every FP16 stochastic-rounding store down its
+// subnormal / overflow path, which a multiphase run hardly takes (both phases
+// sit away from density 1), and a uniform state stays uniform over the
+// timed steps.  lb_create zeroes the buffers after tuning.
+int fill_tuning_state(lb_ctx *c) {
+    float *t = c->tmp_fields;
+    int rc = (int)hipMemsetD32Async((hipDeviceptr_t)t, 0x3f000000, c->N, c->stream);   // density 0.5
+    if (!rc) rc = (int)hipMemsetAsync(t + c->N, 0, (size_t)3 * c->N * sizeof(float), c->stream);
+    if (rc) return rc;
+    float *uz = c->Q == 19 ? t + 3 * (size_t)c->N : nullptr;
+    return c->Q == 9 ? launch_init_q<9>(c, t, t + c->N, t + 2 * (size_t)c->N, uz)
+                     : launch_init_q<19>(c, t, t + c->N, t + 2 * (size_t)c->N, uz);
+}
+
 // Launch autotuner: times each candidate (block shape x XCD mapping x
 // load/store hints x population layout) over a few launches on the real
 // lattice and keeps the fastest.  Runs at creation, before any populations
@@ -919,7 +936,9 @@ int autotune(lb_ctx *c) {
     }
     // Shan-Chen D3Q19: the one-pass z-marching sweep (tile shape x z-chunk x
     // XCD remap x hints x layout) against the tuned two passes
-    if (!rc && c->cfg.sc_G != 0.0f && c->Q == 19 && c->cfg.sc_sweep != 0 && scz_storage(c->cfg.storage)) {
+    const bool sc_forms = c->cfg.sc_G != 0.0f && c->Q == 19 && c->cfg.sc_sweep != 0 && scz_storage(c->cfg.storage);
+    if (!rc && sc_forms) rc = fill_tuning_state(c);
+    if (!rc && sc_forms) {
         auto time_steps = [&]() -> float {
             launch_step(c, 0, false);   // warm
             (void)hipEventRecord(e0, c->stream);
@@ -975,6 +994,8 @@ int autotune(lb_ctx *c) {
         if (!rc && (c->cfg.sc_sweep < 0 || c->cfg.sc_sweep == 3) && scr_lattice_ok(c->cfg, c->N) &&
             !any_solid(c->solid)) {
             const int rzcs[8] = {6, 8, 10, 12, 16, 20, 24, 32};
+            c->layout = two_lay;              // the tiled candidates left another layout's state
+            rc = fill_tuning_state(c);
             float rbest = 1e30f;
             ZCand rb{64, 0, 0, 1, 3, two_lay, 8};
             auto try_r = [&](int nw, int by, int zcv, int xcd) {
@@ -1163,7 +1184,13 @@ extern "C" int lb_create(const lb_config *cfg, lb_ctx **out) {
             c->dbx_shift = c->bx_shift; c->dby = c->by; c->dxcd = c->xcd; c->dmem = c->mem == 2 ? 2 : 0;
             rc = upload_args(c);
         } else {
+            const bool sc_state = cfg->sc_G != 0.0f;
             rc = autotune(c);
+            const size_t bytes[3] = {pop_bytes, pop_bytes, (size_t)4 * c->N * sizeof(float)};
+            void *bufs[3] = {c->buf[0], c->buf[1], c->macros};
+            for (int k = 0; k < 3 && sc_state && !rc; k++) rc = (int)hipMemsetAsync(bufs[k], 0, bytes[k], c->stream);
+            if (sc_state && !rc && c->psi)
+                rc = (int)hipMemsetAsync(c->psi, 0, (size_t)c->N * (cfg->storage == LB_STORE_F64 ? 8 : 4), c->stream);
         }
     }
     if (!rc && cfg->time_block > 0 && !tb_eligible(c, cfg->time_block)) rc = LB_ERR_UNSUPPORTED;
