@@ -515,6 +515,23 @@ def _periodic_box(shape, storage, round_mode, force):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_shan_chen_tuning_leaves_zeroed_buffers(gpu, storage):
+    """lb_create times the Shan-Chen forms on a uniform equilibrium it writes
+    into the population buffers; afterwards the buffers are zero again, as
+    for every other lattice, and a run from the caller's initial state
+    matches the oracle."""
+    from lattibol_amd import scenes
+    cfg, solid, init = scenes.spinodal(n=24, storage=storage)
+    sol = gpu.Solver(cfg, solid)
+    assert not np.any(sol.populations())
+    sol.close()
+    sol, orc = _run_pair(gpu, cfg, solid, init, 40)
+    _assert_pop_parity(sol.populations(), orc.A, cfg["storage"])
+    sol.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", ["cavity48_f32", "cavity48_bf16", "sphere_halo_f32"])
 def test_solid_mask_changed_mid_run_matches_oracle(gpu, case):
     """lb_set_solid right after an lb_step that returned with its step graphs
