@@ -588,6 +588,29 @@ def test_row_slab_shan_chen_sweep_matches_oracle(gpu, case, threads, by, zc, lay
     sol.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("threads,by,zc", [(512, 5, 3), (1024, 5, 16)])
+def test_row_slab_stochastic_rounding_near_unit_density(gpu, storage, threads, by, zc):
+    """Density within 2e-5 of 1: the stored deviations are of order 1e-7,
+    FP16 subnormals, so the row slab's paired FP16 stochastic stores take
+    their integer path in every wave (BF16 keeps its instruction path); both
+    stay bit-identical to the oracle."""
+    from lattibol_amd import scenes
+    shape = (70, 13, 11)
+    cfg, solid, init = _periodic_box(shape, storage, "default", None)
+    init["rho"] = scenes.uniform_noise(5, int(np.prod(shape)), 1.0, 2e-5).reshape(shape[::-1])
+    cfg.update(layout=1, sc_sweep=3, sc_zchunk=zc, block_x=threads, block_y=by)
+    print("case near-1 density", storage, threads, by, zc, flush=True)
+    sol, orc = _run_pair(gpu, cfg, solid, init, 100)
+    assert sol.launch_config["sc_sweep"] == 3 and sol.launch_config["round_mode"] == "stochastic"
+    raw = orc.A.view(np.uint16)
+    if storage == "f16":
+        assert np.mean((raw & 0x7c00) == 0) > 0.5       # mostly subnormal (or zero) stored values
+    _assert_pop_parity(sol.populations(), orc.A, cfg["storage"])
+    sol.close()
+
+
 def test_row_slab_sweep_applies_only_to_periodic_boxes(gpu):
     from lattibol_amd import scenes
     # tuned choice on a periodic box: solids set afterwards hand the step to
