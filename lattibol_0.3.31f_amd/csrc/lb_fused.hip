@@ -868,7 +868,11 @@ int autotune(lb_ctx *c) {
             }
         }
     }
-    if (cands.empty()) cands.push_back({64, 4, 0, 0, 0, 0});
+    // (none: block_x names a width only the row-slab sweep takes; the two
+    // passes then run at a default shape with the fields the caller fixed)
+    if (cands.empty())
+        cands.push_back({64, 4, std::max(0, c->cfg.xcd_remap), std::max(0, c->cfg.mem_mode), std::max(0, c->cfg.layout),
+                         std::max(0, c->cfg.node_words)});
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0)) return LB_ERR_INVALID;
     if (hipEventCreate(&e1)) { (void)hipEventDestroy(e0); return LB_ERR_INVALID; }
@@ -990,10 +994,13 @@ int autotune(lb_ctx *c) {
         }
         // the row-slab sweep of fully periodic lattices: waves per workgroup x
         // rows per tile x z-chunk with the XCD remap on, then the remap off for
-        // the best shape; the population layout kept
+        // the best shape; the population layout kept.  The long chunks let
+        // short tiles fill the CUs with one workgroup each (192^3 FP32: 3 rows
+        // x 48 planes = 256 workgroups, 7 % faster than 5 rows x 32 planes =
+        // 234 workgroups; the 16-bit storages keep the 5-row tiles).
         if (!rc && (c->cfg.sc_sweep < 0 || c->cfg.sc_sweep == 3) && scr_lattice_ok(c->cfg, c->N) &&
             !any_solid(c->solid)) {
-            const int rzcs[8] = {6, 8, 10, 12, 16, 20, 24, 32};
+            const int rzcs[11] = {6, 8, 10, 12, 16, 20, 24, 32, 40, 48, 64};
             c->layout = two_lay;              // the tiled candidates left another layout's state
             rc = fill_tuning_state(c);
             float rbest = 1e30f;

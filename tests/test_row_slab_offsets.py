@@ -131,7 +131,8 @@ def emulate(nx, ny, nz, nw, BY, zc, kind):
 # that do not divide the lattice, a single plane, one-row tiles
 CASES = [((24, 24, 24), 8, 0, 5), ((24, 24, 24), 16, 1, 1), ((70, 13, 11), 12, 3, 4),
          ((70, 13, 11), 16, 0, 16), ((130, 5, 3), 8, 0, 2), ((130, 5, 3), 12, 2, 1),
-         ((64, 7, 1), 8, 4, 3), ((192, 14, 5), 6, 6, 4), ((130, 9, 3), 6, 0, 2), ((200, 3, 4), 16, 0, 3), ((256, 2, 3), 8, 0, 2), ((256, 9, 3), 12, 5, 2)]
+         ((64, 7, 1), 8, 4, 3), ((192, 14, 5), 6, 6, 4), ((130, 9, 3), 6, 0, 2), ((200, 3, 4), 16, 0, 3), ((256, 2, 3), 8, 0, 2), ((256, 9, 3), 12, 5, 2),
+         ((192, 6, 49), 16, 3, 48)]
 
 
 @pytest.mark.parametrize("kind", ["soa", "rows"])
