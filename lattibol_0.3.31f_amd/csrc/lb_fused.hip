@@ -419,6 +419,7 @@ struct lb_ctx {
     // rows per tile, scr_nw its waves per workgroup)
     int scz = 0, zc = 24;
     int scr_nw = 8;
+    int scr_frame_rev = 0;            // k_step_scr's frame-segment order (tuned with the shape)
     int sc_req = -1;                  // the caller's sc_sweep
     // the tuned two-pass sweep geometry, restored when solids make the
     // row-slab form inapplicable (lb_set_solid)
@@ -523,6 +524,7 @@ template <class S> StepArgs<S> make_args(const lb_ctx *c, int src, bool dens = f
     a.nblocks = a.ntx * a.nty * c->cfg.nz;
     a.xcd = dens ? c->dxcd : c->xcd;
     a.zc = c->zc;
+    a.frame_rev = c->scz == 3 ? c->scr_frame_rev : 0;
     if (!dens && c->scz == 3) {
         // row-slab tiles: ntx = 64-node segments per row, by rows per tile
         a.bx_shift = 6;
@@ -953,7 +955,7 @@ int autotune(lb_ctx *c) {
             (void)hipEventElapsedTime(&ms, e0, e1);
             return ms;
         };
-        struct ZCand { int bx, by, zc, xcd, form, layout, v; };
+        struct ZCand { int bx, by, zc, xcd, form, layout, v, rev = 0; };
         const int two_bx = c->bx_shift, two_by = c->by, two_xcd = c->xcd, two_mem = c->mem, two_lay = c->layout;
         c->two_bx_shift = two_bx; c->two_by = two_by; c->two_xcd = two_xcd; c->two_mem = two_mem;
         const float two = c->cfg.sc_sweep > 0 ? 1e30f : time_steps();
@@ -993,8 +995,9 @@ int autotune(lb_ctx *c) {
             if (rc) break;
         }
         // the row-slab sweep of fully periodic lattices: waves per workgroup x
-        // rows per tile x z-chunk with the XCD remap on, then the remap off for
-        // the best shape; the population layout kept.  The long chunks let
+        // rows per tile x z-chunk with the XCD remap on, then the remap off and
+        // the reversed frame-segment order for the best shape; the population
+        // layout kept.  The long chunks let
         // short tiles fill the CUs with one workgroup each (192^3 FP32: 3 rows
         // x 48 planes = 256 workgroups, 7 % faster than 5 rows x 32 planes =
         // 234 workgroups; the 16-bit storages keep the 5-row tiles).
@@ -1005,13 +1008,13 @@ int autotune(lb_ctx *c) {
             rc = fill_tuning_state(c);
             float rbest = 1e30f;
             ZCand rb{64, 0, 0, 1, 3, two_lay, 8};
-            auto try_r = [&](int nw, int by, int zcv, int xcd) {
-                c->scz = 3; c->zc = zcv; c->scr_nw = nw;
+            auto try_r = [&](int nw, int by, int zcv, int xcd, int rev = 0) {
+                c->scz = 3; c->zc = zcv; c->scr_nw = nw; c->scr_frame_rev = rev;
                 c->bx_shift = 6; c->by = by; c->xcd = xcd; c->mem = 0; c->layout = two_lay;
                 rc = upload_args(c);
                 if (rc) return;
                 const float ms = time_steps();
-                if (ms < rbest) { rbest = ms; rb = {64, by, zcv, xcd, 3, two_lay, nw}; }
+                if (ms < rbest) { rbest = ms; rb = {64, by, zcv, xcd, 3, two_lay, nw, rev}; }
             };
             const int xcd0 = c->cfg.xcd_remap >= 0 ? c->cfg.xcd_remap : 1;
             for (int nw : {6, 8, 12, 16}) {
@@ -1030,10 +1033,16 @@ int autotune(lb_ctx *c) {
                 }
             }
             if (!rc && rbest < 1e30f && c->cfg.xcd_remap < 0) try_r(rb.v, rb.by, rb.zc, 1 - xcd0);
+            // (which waves take the frame rows moves the step by 3-8 % either
+            // way, depending on the storage and rounding: 192^3 FP16 with
+            // stochastic rounding 177 -> 169 us reversed, FP32 and the
+            // nearest-even storages slower)
+            if (!rc && rbest < 1e30f) try_r(rb.v, rb.by, rb.zc, rb.xcd, 1);
+            c->scr_frame_rev = 0;
             if (!rc && rbest < zbest) { zbest = rbest; zb = rb; }
         }
         if (!rc && zbest < two) {
-            c->scz = zb.form; c->zc = zb.zc; c->scr_nw = zb.v;
+            c->scz = zb.form; c->zc = zb.zc; c->scr_nw = zb.v; c->scr_frame_rev = zb.form == 3 ? zb.rev : 0;
             c->bx_shift = ilog2(zb.bx); c->by = zb.by; c->xcd = zb.xcd; c->mem = 0; c->layout = zb.layout;
         } else {
             c->scz = 0;

@@ -6,8 +6,9 @@
 // frame row on each side (periodic in y), over a chunk of zc z-planes, and
 // marches up through the chunk.  Rows are cut into segments of 64 nodes (one
 // wave each).  Wave w holds the interior segments s = w + j NW (j < SPI, row
-// 1 + s / nseg) and at most one frame segment f = w (row 0 or BY + 1), so
-// there is no frame column at all: the x-wrap stays inside the row (LDS rows
+// 1 + s / nseg) and at most one frame segment, f = w or, in the reversed
+// order the autotuner may pick, f = NW - 1 - w (row 0 or BY + 1), so there
+// is no frame column at all: the x-wrap stays inside the row (LDS rows
 // carry one padding element per side holding the psi of the opposite end).
 //
 // Iteration k of the march:
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
         const int s = w + j * NW;
         scr_seg(a, si[j], 1 + s / nseg, s % nseg, y0, s < bye * nseg);
     });
-    scr_seg(a, sf, w < nseg ? 0 : bye + 1, w % nseg, y0, w < 2 * nseg);
+    const int f = a.frame_rev ? NW - 1 - w : w;   // the wave's frame segment
+    scr_seg(a, sf, f < nseg ? 0 : bye + 1, f % nseg, y0, f < 2 * nseg);
     uint32_t po[19];                // byte offsets of the population slots
     for (int i = 0; i < Q; i++) po[i] = (uint32_t)i * a.ps * ES;
     const rsrc_t rsa = make_rsrc(a.A, (uint64_t)Q * a.N * ES);

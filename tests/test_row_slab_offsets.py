@@ -48,7 +48,7 @@ def max_rows(nw, nx, bf16=True):
     return by
 
 
-def emulate(nx, ny, nz, nw, BY, zc, kind):
+def emulate(nx, ny, nz, nw, BY, zc, kind, rev=False):
     ps, sy, sz = layout(kind, nx, ny, nz)
     N = nx * ny * nz
     nseg = (nx + 63) // 64
@@ -68,7 +68,7 @@ def emulate(nx, ny, nz, nw, BY, zc, kind):
         lds_node = np.full(4 * slot, -1, np.int64)
         lds_k = np.full(4 * slot, -99, np.int64)
         # wave w: interior segments s = w + j nw (row 1 + s / nseg), frame
-        # segment f = w (row 0 or bye + 1)
+        # segment f = w, or nw - 1 - w in the reversed order (row 0 or bye + 1)
         spi = -(-BY * nseg // nw)
         segs = []
 
@@ -80,7 +80,8 @@ def emulate(nx, ny, nz, nw, BY, zc, kind):
             for j in range(spi):
                 s = w + j * nw
                 segs.append(seg(1 + s // nseg, s % nseg, s < bye * nseg, True))
-            segs.append(seg(0 if w < nseg else bye + 1, w % nseg, w < 2 * nseg, False))
+            f = nw - 1 - w if rev else w
+            segs.append(seg(0 if f < nseg else bye + 1, f % nseg, f < 2 * nseg, False))
         for k in range(z0 - 1, z1 + 2):
             zk = int(wrap1(np.array(k), nz))
 
@@ -140,6 +141,15 @@ CASES = [((24, 24, 24), 8, 0, 5), ((24, 24, 24), 16, 1, 1), ((70, 13, 11), 12, 3
 def test_row_slab_indexing(case, kind):
     (nx, ny, nz), nw, by, zc = CASES[case]
     emulate(nx, ny, nz, nw, by or max_rows(nw, nx), zc, kind)
+
+
+@pytest.mark.parametrize("kind", ["soa", "rows"])
+@pytest.mark.parametrize("case", [0, 2, 4, 7, 11])
+def test_row_slab_indexing_reversed_frames(case, kind):
+    """The reversed frame-segment order (wave w computes frame segment
+    nw - 1 - w) covers the same rows and writes the same LDS positions."""
+    (nx, ny, nz), nw, by, zc = CASES[case]
+    emulate(nx, ny, nz, nw, by or max_rows(nw, nx), zc, kind, rev=True)
 
 
 @pytest.mark.parametrize("nw", WAVES)
