@@ -1033,10 +1033,10 @@ int autotune(lb_ctx *c) {
                 }
             }
             if (!rc && rbest < 1e30f && c->cfg.xcd_remap < 0) try_r(rb.v, rb.by, rb.zc, 1 - xcd0);
-            // (which waves take the frame rows moves the step by 3-8 % either
-            // way, depending on the storage and rounding: 192^3 FP16 with
-            // stochastic rounding 177 -> 169 us reversed, FP32 and the
-            // nearest-even storages slower)
+            // (which waves take the frame rows moves the step by a few per
+            // cent either way, depending on the storage and rounding: 192^3
+            // FP16 with stochastic rounding 177 -> 169 us with the frame rows
+            // on the last waves, the nearest-even storages slower)
             if (!rc && rbest < 1e30f) try_r(rb.v, rb.by, rb.zc, rb.xcd, 1);
             c->scr_frame_rev = 0;
             if (!rc && rbest < zbest) { zbest = rbest; zb = rb; }

@@ -65,7 +65,7 @@ template <class S> struct StepArgs {
     float zh_ux, zh_rho;
     int bx_shift, by, ntx, nty, nblocks, xcd;
     int zc;                       // one-pass Shan-Chen sweep: z-planes per workgroup
-    int frame_rev;                // row-slab sweep: wave w computes frame segment NW - 1 - w (1) or w (0)
+    int frame_rev;                // row-slab sweep: the frame rows on the last waves (see k_step_scr)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs; this bijection

@@ -6,8 +6,8 @@
 // frame row on each side (periodic in y), over a chunk of zc z-planes, and
 // marches up through the chunk.  Rows are cut into segments of 64 nodes (one
 // wave each).  Wave w holds the interior segments s = w + j NW (j < SPI, row
-// 1 + s / nseg) and at most one frame segment, f = w or, in the reversed
-// order the autotuner may pick, f = NW - 1 - w (row 0 or BY + 1), so there
+// 1 + s / nseg) and at most one frame segment f = w (row 0 or BY + 1; the
+// autotuner may move the frame rows to the last waves, see below), so there
 // is no frame column at all: the x-wrap stays inside the row (LDS rows
 // carry one padding element per side holding the psi of the opposite end).
 //
@@ -56,11 +56,12 @@ LB_DEV uint16_t scr_ld16(rsrc_t r, uint32_t vo, uint32_t so) { return __builtin_
 LB_DEV float scr_ld32(rsrc_t r, uint32_t vo, uint32_t so) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
 }
-LB_DEV void scr_st(uint16_t v, rsrc_t r, uint32_t vo, uint32_t so) {
-    __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, 0);
+// AUX 2: non-temporal stores (nothing in the step reads them back)
+template <int AUX> LB_DEV void scr_st(uint16_t v, rsrc_t r, uint32_t vo, uint32_t so) {
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, vo, so, AUX);
 }
-LB_DEV void scr_st(float v, rsrc_t r, uint32_t vo, uint32_t so) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, so, 0);
+template <int AUX> LB_DEV void scr_st(float v, rsrc_t r, uint32_t vo, uint32_t so) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, vo, so, AUX);
 }
 
 // Wrapped coordinate v in [-1, n] (one step outside at most).
@@ -236,8 +237,22 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
         const int s = w + j * NW;
         scr_seg(a, si[j], 1 + s / nseg, s % nseg, y0, s < bye * nseg);
     });
-    const int f = a.frame_rev ? NW - 1 - w : w;   // the wave's frame segment
-    scr_seg(a, sf, f < nseg ? 0 : bye + 1, f % nseg, y0, f < 2 * nseg);
+    // the wave's frame segment: f = w, or with the frame rows on the last
+    // waves (frame_rev) f = NW - 1 - w for the 16-bit storages and, for FP32,
+    // f = w - foff with foff the largest multiple of nseg not above
+    // NW - 2 nseg, which keeps the frame segment in the wave's own column
+    // w % nseg (the x offsets of its interior segment serve both: FP32 has
+    // no registers to spare at 16 waves)
+    int f = w, fcol = w % nseg;
+    if (a.frame_rev) {
+        if constexpr (ES == 2) {
+            f = NW - 1 - w;
+            fcol = f % nseg;
+        } else {
+            f = w - (NW - 2 * nseg) / nseg * nseg;
+        }
+    }
+    scr_seg(a, sf, f < nseg ? 0 : bye + 1, fcol, y0, f >= 0 && f < 2 * nseg);
     uint32_t po[19];                // byte offsets of the population slots
     for (int i = 0; i < Q; i++) po[i] = (uint32_t)i * a.ps * ES;
     const rsrc_t rsa = make_rsrc(a.A, (uint64_t)Q * a.N * ES);
@@ -259,7 +274,11 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
             const uint32_t rowe = (uint32_t)g.y * a.sy * ES + qe;
             static_for<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                scr_st(res[j].template get<i>(), rsb, vo, po[i] + rowe);
+                // (non-temporal for FP32 and BF16: 216 against 226 us for
+                // FP32 at 5 rows x 32 planes, BF16 stochastic 151 against 155;
+                // FP16 keeps plain stores, with which its stochastic step runs
+                // fastest)
+                scr_st<std::is_same<S, StF16>::value ? 0 : 2>(res[j].template get<i>(), rsb, vo, po[i] + rowe);
             });
             if (MACROS) {
                 const uint32_t id = (uint32_t)g.x + (uint32_t)nx * ((uint32_t)g.y + (uint32_t)ny * (uint32_t)q);

@@ -48,7 +48,7 @@ def max_rows(nw, nx, bf16=True):
     return by
 
 
-def emulate(nx, ny, nz, nw, BY, zc, kind, rev=False):
+def emulate(nx, ny, nz, nw, BY, zc, kind, rev=None):
     ps, sy, sz = layout(kind, nx, ny, nz)
     N = nx * ny * nz
     nseg = (nx + 63) // 64
@@ -68,7 +68,11 @@ def emulate(nx, ny, nz, nw, BY, zc, kind, rev=False):
         lds_node = np.full(4 * slot, -1, np.int64)
         lds_k = np.full(4 * slot, -99, np.int64)
         # wave w: interior segments s = w + j nw (row 1 + s / nseg), frame
-        # segment f = w, or nw - 1 - w in the reversed order (row 0 or bye + 1)
+        # segment f (row 0 or bye + 1): w; with the frame rows on the last
+        # waves nw - 1 - w ("last", 16-bit storages) or w - foff in the wave's
+        # own column w % nseg, foff the largest multiple of nseg with
+        # foff + 2 nseg <= nw ("offset", FP32)
+        foff = (nw - 2 * nseg) // nseg * nseg if rev == "offset" else 0
         spi = -(-BY * nseg // nw)
         segs = []
 
@@ -80,8 +84,8 @@ def emulate(nx, ny, nz, nw, BY, zc, kind, rev=False):
             for j in range(spi):
                 s = w + j * nw
                 segs.append(seg(1 + s // nseg, s % nseg, s < bye * nseg, True))
-            f = nw - 1 - w if rev else w
-            segs.append(seg(0 if f < nseg else bye + 1, f % nseg, f < 2 * nseg, False))
+            f, col = (nw - 1 - w, (nw - 1 - w) % nseg) if rev == "last" else (w - foff, w % nseg)
+            segs.append(seg(0 if f < nseg else bye + 1, col, 0 <= f < 2 * nseg, False))
         for k in range(z0 - 1, z1 + 2):
             zk = int(wrap1(np.array(k), nz))
 
@@ -145,11 +149,13 @@ def test_row_slab_indexing(case, kind):
 
 @pytest.mark.parametrize("kind", ["soa", "rows"])
 @pytest.mark.parametrize("case", [0, 2, 4, 7, 11])
-def test_row_slab_indexing_reversed_frames(case, kind):
-    """The reversed frame-segment order (wave w computes frame segment
-    nw - 1 - w) covers the same rows and writes the same LDS positions."""
+@pytest.mark.parametrize("rev", ["last", "offset"])
+def test_row_slab_indexing_frames_on_last_waves(case, kind, rev):
+    """The frame rows on the last waves (wave w computes frame segment
+    w - foff in its own column) cover the same rows and write the same LDS
+    positions."""
     (nx, ny, nz), nw, by, zc = CASES[case]
-    emulate(nx, ny, nz, nw, by or max_rows(nw, nx), zc, kind, rev=True)
+    emulate(nx, ny, nz, nw, by or max_rows(nw, nx), zc, kind, rev=rev)
 
 
 @pytest.mark.parametrize("nw", WAVES)
