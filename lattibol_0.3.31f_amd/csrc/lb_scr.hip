@@ -274,11 +274,9 @@ __global__ __launch_bounds__(NW * 64, scr_min_waves(NW)) void k_step_scr(const S
             const uint32_t rowe = (uint32_t)g.y * a.sy * ES + qe;
             static_for<0, Q>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                // (non-temporal for FP32 and BF16: 216 against 226 us for
-                // FP32 at 5 rows x 32 planes, BF16 stochastic 151 against 155;
-                // FP16 keeps plain stores, with which its stochastic step runs
-                // fastest)
-                scr_st<std::is_same<S, StF16>::value ? 0 : 2>(res[j].template get<i>(), rsb, vo, po[i] + rowe);
+                // (non-temporal: 216 against 226 us for FP32 at 5 rows x 32
+                // planes, BF16 / FP16 stochastic 151 / 174 against 155 / 178)
+                scr_st<2>(res[j].template get<i>(), rsb, vo, po[i] + rowe);
             });
             if (MACROS) {
                 const uint32_t id = (uint32_t)g.x + (uint32_t)nx * ((uint32_t)g.y + (uint32_t)ny * (uint32_t)q);
